@@ -1,0 +1,164 @@
+/*
+ * pxh.h -- C ABI of libpxh, the MI355X (gfx950) frequency-response solver.
+ *
+ * A drop-in for the reference's frequency-dependent response path
+ * (src/solvers, src/matrix, src/wfq, src/rhs, src/bem).  The Fortran host
+ * calls it through iso_c_binding (plasmonx_amd/fortran/pxh.f90); Python
+ * through ctypes (plasmonx_amd/_lib.py).
+ *
+ * Conventions (those of the reference's Fortran data):
+ *   - integers are int32 (the reference's default INTEGER), reals double;
+ *   - complex arrays are interleaved (re, im) doubles, the memory layout of
+ *     Fortran complex(dp) / complex(c_double_complex);
+ *   - arrays are column-major; atom types are 1-based as map_atomtypes;
+ *   - per-type-pair tables are n_types x n_types, element (ti, tj) at
+ *     [(ti-1) + n_types*(tj-1)];
+ *   - the unknowns of omegaFQFmu are [q_1..q_N, mu_1x, mu_1y, mu_1z, ...]
+ *     (n_var = 4N), those of omegaFQ [q_1..q_N], those of BEM one value per
+ *     tessera (DPCM: surface charge density, IEF: charge);
+ *   - all lengths in bohr, frequencies in atomic units.
+ *
+ * Ownership: the host owns every array it passes.  Inputs are copied to HBM
+ * by the pxh_set_* calls and stay resident in the context; outputs are
+ * written before the call returns; no pointer is kept after a call returns.
+ * A context is used by one host thread at a time.
+ *
+ * Status codes: 0 ok; 1 bad argument; 2 GMRES not converged within the
+ * Krylov dimension (reference: fatal, gmres_module.f90:216); 3 singular
+ * pivot (reference: info /= 0, algorithm_module.f90:529,539); 4 HBM budget
+ * exceeded (reference: memory_manager_module.f90:394-397); 5 HIP runtime
+ * error; 6 the HIP device or library is unavailable.  pxh_last_error()
+ * returns the message of the last failure.
+ */
+#ifndef PXH_H
+#define PXH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pxh_ctx pxh_ctx;
+
+enum {
+    PXH_OK = 0,
+    PXH_EINVAL = 1,
+    PXH_ENOCONV = 2,
+    PXH_ESINGULAR = 3,
+    PXH_ENOMEM = 4,
+    PXH_EHIP = 5,
+    PXH_ENODEV = 6
+};
+
+/* target models (reference target_%name_: wfq, wfqfmu, bem) */
+enum { PXH_WFQ = 1, PXH_WFQFMU = 2, PXH_BEM = 3 };
+/* charge-charge kernels (keyword 'forcefield kernel') */
+enum { PXH_GAUSSIAN = 1, PXH_COULOMB = 2, PXH_OHNO = 3 };
+/* keyword 'algorithm method': inversion | iterative | iterative on the fly */
+enum { PXH_INVERSION = 0, PXH_ITERATIVE = 1, PXH_ITERATIVE_OTF = 2 };
+/* arithmetic of the solve */
+enum { PXH_FP64 = 0, PXH_MIXED = 1 };
+/* keyword 'field rhs type': field | potential */
+enum { PXH_RHS_FIELD = 1, PXH_RHS_POTENTIAL = 2 };
+/* BEM keywords 'bem variant' and 'bem green function' */
+enum { PXH_DPCM = 1, PXH_IEFPCM = 2 };
+enum { PXH_APPROXIMATE = 1, PXH_ACCURATE = 2 };
+
+/* Create / destroy a context on HIP device `device`.  hbm_budget_bytes = 0
+ * means "all free HBM".  Replaces the reference's global state set up by
+ * inp_%read_ (src/input_output/input_module.f90:156-258). */
+int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out);
+int pxh_ctx_destroy(pxh_ctx *ctx);
+
+/* Atomistic target (omegaFQ / omegaFQFmu).  Replaces the data the reference
+ * keeps in target_ and parameters (target_module.f90:20-131,
+ * parameters_module.f90:64-98) and get_neighbours (target_module.f90:417-464):
+ *   coord_bohr 3 x n_atoms; type_of_atom n_atoms (1-based);
+ *   eta, r_q, r_mu, a_ij, tau, density: n_types (tau and density after the
+ *   'scaling sigma0-tau' of wfq_module.f90:96-127);
+ *   fermi_d, fermi_s, r0: n_types x n_types (r0 in bohr,
+ *   target_module.f90:472-518);
+ *   nb_counts n_types x n_atoms and nb_total n_atoms: neighbours within r0
+ *   (self included), used by the omegaFQFmu diagonal; may be NULL for wfq. */
+int pxh_set_atoms(pxh_ctx *ctx, int model, int kernel, int n_atoms, const double *coord_bohr,
+                  const int32_t *type_of_atom, int n_types, const double *eta, const double *r_q,
+                  const double *r_mu, const double *a_ij, const double *fermi_d, const double *fermi_s,
+                  const double *r0, const double *tau, const double *density, const int32_t *nb_counts,
+                  const int32_t *nb_total);
+
+/* Interband polarizability alpha_t(omega) per type and frequency (complex,
+ * n_types x n_freq).  Replaces read_alpha (wfqfmu_module.f90:97-166). */
+int pxh_set_alpha(pxh_ctx *ctx, int n_types, int n_freq, const double *alpha_w);
+
+/* BEM target.  verts9: 9 x n_tess, the three vertices (in the order of the
+ * GMSH face) of each triangle in bohr.  Replaces read_gmsh_file +
+ * create_tesserae (bem_module.f90:224-491) and the bem keywords. */
+int pxh_set_mesh(pxh_ctx *ctx, int variant, int green, int n_tess, const double *verts9, double normal_factor,
+                 double sphere_r_bohr, double eps_solvent, int charge_constraint);
+
+/* Permittivity eps(omega) of the BEM metal (complex, n_freq).  Replaces
+ * read_permittivity_bem (bem_module.f90:154-219). */
+int pxh_set_permittivity(pxh_ctx *ctx, int n_freq, const double *eps_w);
+
+/* Solve the frequency-dependent response for n_freq frequencies and the
+ * field directions of pol_mask (bit 0 = x, 1 = y, 2 = z).  Replaces the body
+ * of algorithm%solve_dynamic_field (algorithm_module.f90:150-179) with its
+ * three loops solve_w_inversion / solve_w_iter_mem / solve_w_iter_on_the_fly
+ * (:189-465).
+ *   method:      PXH_INVERSION | PXH_ITERATIVE | PXH_ITERATIVE_OTF;
+ *   tol_abs:     absolute GMRES threshold, already scaled by sqrt(n_var) as
+ *                input_module.f90:251 does; the true residual
+ *                ||b - A x|| < tol_abs is required on return;
+ *   krylov_dim:  'algorithm gmres dimension' (no restart, as the reference);
+ *   precision:   PXH_FP64 (every operation in FP64) or PXH_MIXED (FP32
+ *                interaction tiles inside FP64 iterative refinement; the
+ *                stopping test is the same FP64 true residual);
+ *   variables:   n_var x 3 x n_freq complex, may be NULL;
+ *   polar:       3 x 3 x n_freq complex (polar_w per frequency);
+ *   results:     n_freq x 20, column-major (target_%results);
+ *   iters:       3 x n_freq Krylov iterations (0 for inversion), may be NULL;
+ *   true_residual: 3 x n_freq, may be NULL. */
+int pxh_solve_spectrum(pxh_ctx *ctx, int n_freq, const double *freq_au, double e0, int rhs_form, int pol_mask,
+                       int method, double tol_abs, int krylov_dim, int precision, double *variables, double *polar,
+                       double *results, int32_t *iters, double *true_residual);
+
+/* y = A(omega_i) x for n_vec vectors of n_var complex values: the on-the-fly
+ * product plus the diagonal shift.  Replaces target_%product_matrix_vector
+ * followed by target_%new_gmres_diagonal_shift (gmres_module.f90:111-121;
+ * wfq_module.f90:321-348,398-424; wfqfmu_module.f90:584-626,698-746;
+ * bem_module.f90:977-997).  i_freq indexes the table of pxh_set_alpha /
+ * pxh_set_permittivity (0-based). */
+int pxh_matvec(pxh_ctx *ctx, double freq_au, int i_freq, int n_vec, const double *x, double *y);
+
+/* The frequency-independent real matrix, n_var x n_var column-major.
+ * Replaces target_%construct_constant_matrix (wfq_module.f90:192-206,
+ * wfqfmu_module.f90:275-323, bem_module.f90:678-782). */
+int pxh_constant_matrix(pxh_ctx *ctx, double *m);
+
+/* The dynamic right-hand sides, n_var x 3 complex.  Replaces
+ * target_%construct_dynamic_field_rhs(_on_the_fly) and, for alloys,
+ * assign_variables_w (wfq_module.f90:260-301; wfqfmu_module.f90:177-266,
+ * 329-428; rhs_module.f90:43-430; bem_module.f90:788-873). */
+int pxh_dynamic_rhs(pxh_ctx *ctx, double freq_au, double e0, int rhs_form, double *rhs);
+
+/* Copy the message of the last failure into buf (NUL-terminated). */
+int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
+
+/* Tuning knobs (no counterpart in the reference): "kbar_cutoff" (1 = drop
+ * K-bar pairs whose Fermi factor 1-f is below 1e-18, the default; 0 = keep
+ * every pair), "batch" (systems per batched solve), "inner_tol" (relative
+ * tolerance of the FP32 inner solves), "tile_i", "splits". */
+int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
+
+/* Counters of the last pxh_solve_spectrum call: [0] matvecs (FP64 vector
+ * equivalents), [1] FP32 block products, [2] FP64 block products,
+ * [3] pair-kernel ms (FP32), [4] pair-kernel ms (FP64), [5] wall ms,
+ * [6] pair evaluations, [7] outer refinements. */
+int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+"""ctypes binding of libpxh (include/pxh.h).
+
+The HIP library is the only compute path: importing this module does not
+touch the GPU, but every call that computes goes through libpxh.so and
+raises PxhError when the library, the device or the call fails.  There is no
+CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpxh.so")
+
+PXH_OK, PXH_EINVAL, PXH_ENOCONV, PXH_ESINGULAR, PXH_ENOMEM, PXH_EHIP, PXH_ENODEV = range(7)
+WFQ, WFQFMU, BEM = 1, 2, 3
+GAUSSIAN, COULOMB, OHNO = 1, 2, 3
+INVERSION, ITERATIVE, ITERATIVE_OTF = 0, 1, 2
+FP64, MIXED = 0, 1
+RHS_FIELD, RHS_POTENTIAL = 1, 2
+DPCM, IEFPCM = 1, 2
+APPROXIMATE, ACCURATE = 1, 2
+
+EXPORTS = [
+    "pxh_ctx_create", "pxh_ctx_destroy", "pxh_set_atoms", "pxh_set_alpha", "pxh_set_mesh",
+    "pxh_set_permittivity", "pxh_solve_spectrum", "pxh_matvec", "pxh_constant_matrix",
+    "pxh_dynamic_rhs", "pxh_last_error", "pxh_set_option", "pxh_get_stats",
+]
+
+
+class PxhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpxh status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libpxh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PxhError(PXH_ENODEV, f"{LIB_PATH} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    I = ctypes.c_int
+    lib.pxh_ctx_create.argtypes = [I, ctypes.c_size_t, ctypes.POINTER(P)]
+    lib.pxh_ctx_destroy.argtypes = [P]
+    lib.pxh_set_atoms.argtypes = [P, I, I, I, P, P, I, P, P, P, P, P, P, P, P, P, P, P]
+    lib.pxh_set_alpha.argtypes = [P, I, I, P]
+    lib.pxh_set_mesh.argtypes = [P, I, I, I, P, D, D, D, I]
+    lib.pxh_set_permittivity.argtypes = [P, I, P]
+    lib.pxh_solve_spectrum.argtypes = [P, I, P, D, I, I, I, D, I, I, P, P, P, P, P]
+    lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
+    lib.pxh_constant_matrix.argtypes = [P, P]
+    lib.pxh_dynamic_rhs.argtypes = [P, D, D, I, P]
+    lib.pxh_last_error.argtypes = [P, ctypes.c_char_p, I]
+    lib.pxh_set_option.argtypes = [P, ctypes.c_char_p, D]
+    lib.pxh_get_stats.argtypes = [P, P, I]
+    for name in EXPORTS:
+        getattr(lib, name).restype = I
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One libpxh context (one HIP device, resident geometry)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.lib = load()
+        self._h = ctypes.c_void_p()
+        self._check(self.lib.pxh_ctx_create(device, hbm_budget, ctypes.byref(self._h)), null=True)
+
+    def _check(self, st, null=False):
+        if st != PXH_OK:
+            buf = ctypes.create_string_buffer(512)
+            self.lib.pxh_last_error(None if null else self._h, buf, 512)
+            raise PxhError(st, buf.value.decode())
+
+    def close(self):
+        if self._h:
+            self.lib.pxh_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- inputs
+    def set_atoms(self, sysm):
+        """sysm: plasmonx_amd.model.AtomSystem."""
+        c = np.ascontiguousarray(sysm.coord_bohr, dtype=np.float64)
+        t = np.ascontiguousarray(sysm.type_idx + 1, dtype=np.int32)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        fd = f(np.asarray(sysm.fermi_d).T.reshape(-1))  # column-major nt x nt
+        fs = f(np.asarray(sysm.fermi_s).T.reshape(-1))
+        r0 = f(np.asarray(sysm.r0).T.reshape(-1))
+        nbc = np.ascontiguousarray(sysm.nb_counts, dtype=np.int32)  # (n, nt) row-major == nt x n col-major
+        nbt = np.ascontiguousarray(sysm.nb_total, dtype=np.int32)
+        arrs = [f(sysm.eta), f(sysm.r_q), f(sysm.r_mu), f(sysm.a_ij), fd, fs, r0, f(sysm.tau), f(sysm.density)]
+        self._keep = (c, t, nbc, nbt, arrs)
+        st = self.lib.pxh_set_atoms(self._h, sysm.model_id, sysm.kernel_id, sysm.n_atoms, _ptr(c), _ptr(t),
+                                    sysm.n_types, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3]),
+                                    _ptr(fd), _ptr(fs), _ptr(r0), _ptr(arrs[7]), _ptr(arrs[8]), _ptr(nbc),
+                                    _ptr(nbt))
+        self._check(st)
+
+    def set_alpha(self, alpha_w):
+        """alpha_w: complex (n_types, n_freq)."""
+        a = np.asfortranarray(np.asarray(alpha_w, dtype=np.complex128))
+        self._check(self.lib.pxh_set_alpha(self._h, a.shape[0], a.shape[1], _ptr(a)))
+
+    def set_mesh(self, mesh):
+        v = np.ascontiguousarray(mesh.verts_bohr, dtype=np.float64)
+        self._check(self.lib.pxh_set_mesh(self._h, mesh.variant_id, mesh.green_id, mesh.n, _ptr(v),
+                                          float(mesh.normal_factor), float(mesh.sphere_r_bohr),
+                                          float(mesh.eps_solvent), int(mesh.charge_constraint)))
+
+    def set_permittivity(self, eps_w):
+        e = np.ascontiguousarray(np.asarray(eps_w, dtype=np.complex128))
+        self._check(self.lib.pxh_set_permittivity(self._h, e.shape[0], _ptr(e)))
+
+    def set_option(self, key, value):
+        self._check(self.lib.pxh_set_option(self._h, key.encode(), float(value)))
+
+    # ---- compute
+    def solve_spectrum(self, n_var, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7, method=ITERATIVE_OTF, tol_abs=1e-8,
+                       krylov_dim=1000, precision=FP64, want_variables=False):
+        freq = np.ascontiguousarray(freq_au, dtype=np.float64)
+        nf = freq.shape[0]
+        var = np.zeros((nf, 3, n_var), dtype=np.complex128) if want_variables else None
+        polar = np.zeros((nf, 3, 3), dtype=np.complex128)  # [f][b][a] = alpha_ab
+        res = np.zeros((20, nf), dtype=np.float64)  # column-major n_freq x 20
+        iters = np.zeros((nf, 3), dtype=np.int32)
+        tres = np.zeros((nf, 3), dtype=np.float64)
+        st = self.lib.pxh_solve_spectrum(self._h, nf, _ptr(freq), float(e0), rhs_form, pol_mask, method,
+                                         float(tol_abs), int(krylov_dim), precision, _ptr(var), _ptr(polar),
+                                         _ptr(res), _ptr(iters), _ptr(tres))
+        self._check(st)
+        out = {"polar": np.transpose(polar, (0, 2, 1)), "results": res.T.copy(), "iters": iters,
+               "true_residual": tres}
+        if want_variables:
+            out["variables"] = var
+        return out
+
+    def matvec(self, freq_au, i_freq, x):
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+        y = np.zeros_like(x)
+        self._check(self.lib.pxh_matvec(self._h, float(freq_au), int(i_freq), x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def constant_matrix(self, n_var):
+        m = np.zeros((n_var, n_var), dtype=np.float64, order="F")
+        self._check(self.lib.pxh_constant_matrix(self._h, _ptr(m)))
+        return m
+
+    def dynamic_rhs(self, n_var, freq_au, e0, rhs_form=RHS_FIELD):
+        r = np.zeros((3, n_var), dtype=np.complex128)
+        self._check(self.lib.pxh_dynamic_rhs(self._h, float(freq_au), float(e0), rhs_form, _ptr(r)))
+        return r
+
+    def stats(self):
+        s = np.zeros(8)
+        self._check(self.lib.pxh_get_stats(self._h, _ptr(s), 8))
+        return {"matvecs": s[0], "fp32_products": s[1], "fp64_products": s[2], "wall_ms": s[5],
+                "pair_evaluations": s[6], "outer": s[7]}

@@ -1,0 +1,318 @@
+// C-ABI entry points of libpxh (declared in include/pxh.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "pxh_internal.h"
+
+using pxh::Error;
+
+namespace {
+
+thread_local std::string g_last_error;  // failures before a context exists
+
+template <typename F>
+int guarded(pxh_ctx *c, F &&f) {
+    try {
+        if (c) PXH_HIP(hipSetDevice(c->device));
+        f();
+        return PXH_OK;
+    } catch (const Error &e) {
+        if (c) c->last_error = e.msg;
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (c) c->last_error = "host allocation failed";
+        g_last_error = "host allocation failed";
+        return PXH_ENOMEM;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out) {
+    if (!out) return PXH_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        g_last_error = "no HIP device available";
+        return PXH_ENODEV;
+    }
+    if (device < 0 || device >= count) {
+        g_last_error = "device index out of range";
+        return PXH_EINVAL;
+    }
+    pxh_ctx *c = new (std::nothrow) pxh_ctx();
+    if (!c) return PXH_ENOMEM;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    const int st = guarded(c, [&] {
+        PXH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        PXH_HIP(hipEventCreate(&c->ev0));
+        PXH_HIP(hipEventCreate(&c->ev1));
+    });
+    if (st != PXH_OK) {
+        delete c;
+        return st;
+    }
+    *out = c;
+    return PXH_OK;
+}
+
+int pxh_ctx_destroy(pxh_ctx *c) {
+    if (!c) return PXH_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PXH_OK;
+}
+
+int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *coord_bohr,
+                  const int32_t *type_of_atom, int n_types, const double *eta, const double *r_q, const double *r_mu,
+                  const double *a_ij, const double *fermi_d, const double *fermi_s, const double *r0,
+                  const double *tau, const double *density, const int32_t *nb_counts, const int32_t *nb_total) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(model == PXH_WFQ || model == PXH_WFQFMU, PXH_EINVAL, "model must be PXH_WFQ or PXH_WFQFMU");
+        PXH_CHECK(kernel >= PXH_GAUSSIAN && kernel <= PXH_OHNO, PXH_EINVAL, "unknown kernel");
+        PXH_CHECK(n_atoms > 0 && n_types > 0 && n_types <= pxh::kMaxTypes, PXH_EINVAL,
+                  "n_atoms must be > 0 and 1 <= n_types <= 8");
+        PXH_CHECK(coord_bohr && type_of_atom && eta && a_ij && fermi_d && fermi_s && r0 && tau && density, PXH_EINVAL,
+                  "null input array");
+        PXH_CHECK(kernel != PXH_GAUSSIAN || r_q, PXH_EINVAL, "the Gaussian kernel needs r_q");
+        PXH_CHECK(model != PXH_WFQFMU || (r_q && r_mu), PXH_EINVAL, "omegaFQFmu needs r_q and r_mu");
+        pxh::Atoms &A = c->atoms;
+        A.model = model;
+        A.kernel = kernel;
+        A.n = n_atoms;
+        A.nt = n_types;
+        A.h_coord.assign(coord_bohr, coord_bohr + 3 * (size_t)n_atoms);
+        A.h_type.resize(n_atoms);
+        std::vector<int> seen(n_types, 0);
+        for (int i = 0; i < n_atoms; ++i) {
+            PXH_CHECK(type_of_atom[i] >= 1 && type_of_atom[i] <= n_types, PXH_EINVAL, "type_of_atom out of range");
+            A.h_type[i] = type_of_atom[i] - 1;
+            seen[A.h_type[i]] = 1;
+        }
+        int used = 0;
+        for (int t = 0; t < n_types; ++t) used += seen[t];
+        A.hetero = used > 1;
+        // plain omegaFQ has no alloy form (wfq_module.f90:203-204)
+        PXH_CHECK(!(A.hetero && model == PXH_WFQ), PXH_EINVAL, "heterogeneous omegaFQ is not supported");
+        auto cp = [&](const double *p, size_t k) { return p ? std::vector<double>(p, p + k) : std::vector<double>(k, 0.0); };
+        A.eta = cp(eta, n_types);
+        A.rq = cp(r_q, n_types);
+        A.rmu = cp(r_mu, n_types);
+        A.aij = cp(a_ij, n_types);
+        A.tau = cp(tau, n_types);
+        A.dens = cp(density, n_types);
+        A.fd = cp(fermi_d, (size_t)n_types * n_types);
+        A.fs = cp(fermi_s, (size_t)n_types * n_types);
+        A.r0 = cp(r0, (size_t)n_types * n_types);
+        for (int t = 0; t < n_types; ++t) {
+            PXH_CHECK(A.tau[t] > 0.0 && A.dens[t] > 0.0, PXH_EINVAL, "tau and density must be positive");
+            if (kernel == PXH_GAUSSIAN) PXH_CHECK(A.rq[t] > 0.0, PXH_EINVAL, "r_q must be positive");
+            if (model == PXH_WFQFMU) PXH_CHECK(A.rq[t] > 0.0 && A.rmu[t] > 0.0, PXH_EINVAL, "r_q, r_mu must be positive");
+        }
+        A.h_nbfrac.assign((size_t)n_types * n_atoms, 0.0);
+        for (int i = 0; i < n_atoms; ++i) {
+            if (nb_counts && nb_total) {
+                PXH_CHECK(nb_total[i] > 0, PXH_EINVAL, "nb_total must be positive");
+                for (int t = 0; t < n_types; ++t)
+                    A.h_nbfrac[t + (size_t)n_types * i] =
+                        (double)nb_counts[t + (size_t)n_types * i] / (double)nb_total[i];
+            } else {
+                A.h_nbfrac[A.h_type[i] + (size_t)n_types * i] = 1.0;
+            }
+        }
+        c->target = model;
+        pxh::build_atoms_device(c);
+    });
+}
+
+int pxh_set_alpha(pxh_ctx *c, int n_types, int n_freq, const double *alpha_w) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(n_types == c->atoms.nt && n_freq >= 0 && (alpha_w || n_freq == 0), PXH_EINVAL,
+                  "alpha table must be n_types x n_freq of the current atoms");
+        c->alpha.assign(alpha_w, alpha_w + 2 * (size_t)n_types * n_freq);
+        c->alpha_nf = n_freq;
+    });
+}
+
+int pxh_set_mesh(pxh_ctx *c, int variant, int green, int n_tess, const double *verts9, double normal_factor,
+                 double sphere_r_bohr, double eps_solvent, int charge_constraint) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(variant == PXH_DPCM || variant == PXH_IEFPCM, PXH_EINVAL, "variant must be DPCM or IEFPCM");
+        PXH_CHECK(green == PXH_APPROXIMATE || green == PXH_ACCURATE, PXH_EINVAL, "unknown green function");
+        PXH_CHECK(n_tess > 0 && verts9, PXH_EINVAL, "empty mesh");
+        PXH_CHECK(green == PXH_ACCURATE || sphere_r_bohr > 0.0, PXH_EINVAL,
+                  "the approximate Green function needs a sphere radius");
+        pxh::Mesh &M = c->mesh;
+        M.n = n_tess;
+        M.variant = variant;
+        M.green = green;
+        M.normal_factor = normal_factor;
+        M.sphere_r = sphere_r_bohr;
+        M.eps_solvent = eps_solvent;
+        M.charge_constraint = charge_constraint != 0;
+        M.h_verts.assign(verts9, verts9 + 9 * (size_t)n_tess);
+        c->target = PXH_BEM;
+        pxh::build_mesh_device(c);
+    });
+}
+
+int pxh_set_permittivity(pxh_ctx *c, int n_freq, const double *eps_w) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(n_freq >= 0 && (eps_w || n_freq == 0), PXH_EINVAL, "bad permittivity table");
+        c->eps.assign(eps_w, eps_w + 2 * (size_t)n_freq);
+        c->eps_nf = n_freq;
+    });
+}
+
+int pxh_solve_spectrum(pxh_ctx *c, int n_freq, const double *freq_au, double e0, int rhs_form, int pol_mask,
+                       int method, double tol_abs, int krylov_dim, int precision, double *variables, double *polar,
+                       double *results, int32_t *iters, double *true_residual) {
+    if (!c) return PXH_EINVAL;
+    int status = PXH_OK;
+    const int g = guarded(c, [&] {
+        PXH_CHECK(n_freq > 0 && freq_au, PXH_EINVAL, "no frequencies");
+        PXH_CHECK(pol_mask > 0 && pol_mask < 8, PXH_EINVAL, "pol_mask must select x, y and/or z");
+        PXH_CHECK(method >= PXH_INVERSION && method <= PXH_ITERATIVE_OTF, PXH_EINVAL, "unknown method");
+        PXH_CHECK(method == PXH_INVERSION || (tol_abs > 0.0 && krylov_dim > 0), PXH_EINVAL,
+                  "GMRES needs tol_abs > 0 and krylov_dim > 0");
+        PXH_CHECK(!(c->target == PXH_BEM && method == PXH_ITERATIVE_OTF), PXH_EINVAL,
+                  "BEM has no on-the-fly product (bem_module.f90:91-135)");
+        PXH_CHECK(rhs_form == PXH_RHS_FIELD || rhs_form == PXH_RHS_POTENTIAL, PXH_EINVAL, "unknown rhs_form");
+        status = pxh::solve_spectrum(c, n_freq, freq_au, e0, rhs_form, pol_mask, method, tol_abs, krylov_dim,
+                                     precision, variables, polar, results, iters, true_residual);
+        if (status == PXH_ENOCONV) c->last_error = "GMRES not converged within the Krylov dimension";
+        if (status == PXH_ESINGULAR) c->last_error = "singular pivot in the LU factorisation";
+    });
+    return g != PXH_OK ? g : status;
+}
+
+int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *x, double *y) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(n_vec > 0 && x && y, PXH_EINVAL, "bad vectors");
+        const bool bem = c->target == PXH_BEM;
+        const int nv = bem ? c->mesh.n : c->atoms.n_var();
+        PXH_CHECK(nv > 0, PXH_EINVAL, "no target set");
+        pxh::DevBuf<double2> dx, dy;
+        dx.upload(reinterpret_cast<const double2 *>(x), (size_t)nv * n_vec, c->stream);
+        dy.alloc((size_t)nv * n_vec);
+        std::vector<int> list(n_vec);
+        for (int k = 0; k < n_vec; ++k) list[k] = k;
+        pxh::DevBuf<int> dl;
+        dl.upload(list.data(), n_vec, c->stream);
+        if (bem) {
+            PXH_CHECK(i_freq >= 0 && i_freq < c->eps_nf, PXH_EINVAL, "i_freq outside the permittivity table");
+            const double *e = c->eps.data() + 2 * (size_t)i_freq;
+            const double es = c->mesh.eps_solvent;
+            const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
+            const double den = dr * dr + di * di;
+            const double2 l = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
+            std::vector<double2> hl(n_vec, l);
+            pxh::DevBuf<double2> dlam;
+            dlam.upload(hl.data(), n_vec, c->stream);
+            pxh::apply_bem(c, n_vec, dl.p, dlam.p, dx.p, dy.p, nv, false);
+        } else {
+            std::vector<pxh::SysParam> hp(n_vec, pxh::make_sysparam(c, freq_au, i_freq));
+            pxh::DevBuf<pxh::SysParam> dp;
+            dp.upload(hp.data(), n_vec, c->stream);
+            pxh::apply_atoms_fp64(c, n_vec, dl.p, dp.p, dx.p, dy.p, nv);
+        }
+        PXH_HIP(hipMemcpyAsync(y, dy.p, sizeof(double2) * nv * n_vec, hipMemcpyDeviceToHost, c->stream));
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pxh_constant_matrix(pxh_ctx *c, double *m) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(m, PXH_EINVAL, "null output");
+        if (c->target == PXH_BEM) {
+            const size_t nn = (size_t)c->mesh.n * c->mesh.n;
+            PXH_HIP(hipMemcpyAsync(m, c->mesh.D.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        } else {
+            PXH_CHECK(c->atoms.n > 0, PXH_EINVAL, "no target set");
+            const size_t nv = c->atoms.n_var();
+            pxh::DevBuf<double> dm;
+            dm.alloc(nv * nv);
+            pxh::constant_matrix_atoms(c, dm.p);
+            PXH_HIP(hipMemcpyAsync(m, dm.p, sizeof(double) * nv * nv, hipMemcpyDeviceToHost, c->stream));
+            PXH_HIP(hipStreamSynchronize(c->stream));
+        }
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pxh_dynamic_rhs(pxh_ctx *c, double freq_au, double e0, int rhs_form, double *rhs) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(rhs, PXH_EINVAL, "null output");
+        const bool bem = c->target == PXH_BEM;
+        const int nv = bem ? c->mesh.n : c->atoms.n_var();
+        PXH_CHECK(nv > 0, PXH_EINVAL, "no target set");
+        pxh::DevBuf<double2> d;
+        d.alloc((size_t)nv * 3);
+        if (bem)
+            pxh::dynamic_rhs_bem(c, e0, d.p);
+        else
+            pxh::dynamic_rhs_atoms(c, freq_au, e0, rhs_form, d.p);
+        PXH_HIP(hipMemcpyAsync(rhs, d.p, sizeof(double2) * nv * 3, hipMemcpyDeviceToHost, c->stream));
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pxh_last_error(pxh_ctx *c, char *buf, int len) {
+    if (!buf || len <= 0) return PXH_EINVAL;
+    const std::string &s = c ? c->last_error : g_last_error;
+    const int k = std::min<int>(len - 1, (int)s.size());
+    std::memcpy(buf, s.data(), k);
+    buf[k] = '\0';
+    return PXH_OK;
+}
+
+int pxh_set_option(pxh_ctx *c, const char *key, double value) {
+    if (!c || !key) return PXH_EINVAL;
+    return guarded(c, [&] {
+        const std::string k(key);
+        if (k == "kbar_cutoff") {
+            c->opt.kbar_cutoff = value != 0.0;
+            if (c->atoms.n > 0) pxh::build_kbar(c);
+        } else if (k == "batch") {
+            PXH_CHECK(value >= 1, PXH_EINVAL, "batch must be >= 1");
+            c->opt.batch = (int)value;
+        } else if (k == "inner_tol") {
+            PXH_CHECK(value > 0 && value < 1, PXH_EINVAL, "inner_tol must be in (0, 1)");
+            c->opt.inner_tol = value;
+        } else if (k == "splits") {
+            c->opt.splits = (int)value;
+        } else if (k == "max_outer") {
+            c->opt.max_outer = std::max(1, (int)value);
+        } else {
+            throw Error{PXH_EINVAL, "unknown option: " + k};
+        }
+    });
+}
+
+int pxh_get_stats(pxh_ctx *c, double *stats, int n) {
+    if (!c || !stats) return PXH_EINVAL;
+    for (int k = 0; k < n && k < 8; ++k) stats[k] = c->stats.v[k];
+    return PXH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,516 @@
+// Dense per-frequency solves on gfx950: assembly of A(omega) for every
+// frequency, a blocked complex LU with partial pivoting batched over the
+// frequencies (LAPACK ZGETRF/ZGETRS semantics, called by the reference at
+// src/solvers/algorithm_module.f90:510-541), the BEM charge constraint
+// (:266-319), and GMRES on the stored matrix (solve_w_iter_mem, :334-410).
+//
+// Layout: one n x n column-major complex (double2) matrix per frequency,
+// frequencies contiguous; every kernel takes blockIdx.z as the frequency.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pxh_internal.h"
+
+namespace pxh {
+
+namespace {
+
+constexpr int NB = 32;  // panel width
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+
+// Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
+// frequency.  Pivots (global row indices) go to piv[f][k..k+nb).  Row swaps
+// are applied inside the panel only.
+__global__ __launch_bounds__(256) void k_panel(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f,
+                                               int *__restrict__ piv, int *__restrict__ info) {
+    const int f = blockIdx.z;
+    double2 *Af = A + lda_f * f;
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    const int tid = threadIdx.x;
+    for (int c = k; c < k + nb; ++c) {
+        // pivot search in column c, rows c..n-1 (first maximum, as IZAMAX)
+        double best = -1.0;
+        int bi = c;
+        for (int r = c + tid; r < n; r += 256) {
+            const double v = cabs2(Af[r + (size_t)n * c]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (tid < off) {
+                if (sval[tid + off] > sval[tid] || (sval[tid + off] == sval[tid] && sidx[tid + off] < sidx[tid])) {
+                    sval[tid] = sval[tid + off];
+                    sidx[tid] = sidx[tid + off];
+                }
+            }
+            __syncthreads();
+        }
+        const int p = sidx[0];
+        const double pv = sval[0];
+        __syncthreads();
+        if (tid == 0) {
+            piv[(size_t)n * f + c] = p;
+            if (pv == 0.0 && info[f] == 0) info[f] = c + 1;
+        }
+        if (pv == 0.0) continue;
+        // swap rows c and p inside the panel
+        if (p != c)
+            for (int cc = k + tid; cc < k + nb; cc += 256) {
+                const double2 t = Af[c + (size_t)n * cc];
+                Af[c + (size_t)n * cc] = Af[p + (size_t)n * cc];
+                Af[p + (size_t)n * cc] = t;
+            }
+        __syncthreads();
+        const double2 d = Af[c + (size_t)n * c];
+        for (int r = c + 1 + tid; r < n; r += 256) Af[r + (size_t)n * c] = cdiv(Af[r + (size_t)n * c], d);
+        __syncthreads();
+        // rank-1 update of the remaining panel columns
+        for (int cc = c + 1; cc < k + nb; ++cc) {
+            const double2 u = Af[c + (size_t)n * cc];
+            for (int r = c + 1 + tid; r < n; r += 256) {
+                const double2 l = Af[r + (size_t)n * c];
+                const double2 pr = cmul(l, u);
+                Af[r + (size_t)n * cc].x -= pr.x;
+                Af[r + (size_t)n * cc].y -= pr.y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Apply the panel's row swaps to the columns outside [k, k+nb).
+__global__ void k_swap_rest(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f, const int *__restrict__ piv) {
+    const int f = blockIdx.z;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n || (col >= k && col < k + nb)) return;
+    double2 *Af = A + lda_f * f;
+    for (int c = k; c < k + nb; ++c) {
+        const int p = piv[(size_t)n * f + c];
+        if (p != c) {
+            const double2 t = Af[c + (size_t)n * col];
+            Af[c + (size_t)n * col] = Af[p + (size_t)n * col];
+            Af[p + (size_t)n * col] = t;
+        }
+    }
+}
+
+// U12 = L11^-1 A12 (unit lower L11 of the panel), one work-item per column.
+__global__ void k_trsm_u12(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f) {
+    const int f = blockIdx.z;
+    double2 *Af = A + lda_f * f;
+    __shared__ double2 L[NB][NB];
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+        const int r = e % nb, c = e / nb;
+        L[r][c] = Af[(k + r) + (size_t)n * (k + c)];
+    }
+    __syncthreads();
+    const int col = k + nb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    double2 x[NB];
+    for (int r = 0; r < nb; ++r) x[r] = Af[(k + r) + (size_t)n * col];
+    for (int r = 0; r < nb; ++r)
+        for (int q = r + 1; q < nb; ++q) {
+            const double2 p = cmul(L[q][r], x[r]);
+            x[q].x -= p.x;
+            x[q].y -= p.y;
+        }
+    for (int r = 0; r < nb; ++r) Af[(k + r) + (size_t)n * col] = x[r];
+}
+
+// A22 -= L21 U12, 64 x 64 output tile per workgroup of 256 (4 x 4 per item).
+__global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f) {
+    const int f = blockIdx.z;
+    double2 *Af = A + lda_f * f;
+    const int r0 = k + nb + blockIdx.x * 64, c0 = k + nb + blockIdx.y * 64;
+    if (r0 >= n || c0 >= n) return;
+    __shared__ double2 sL[64][NB + 1];
+    __shared__ double2 sU[NB][64 + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * nb; e += 256) {
+        const int r = e % 64, q = e / 64;
+        sL[r][q] = (r0 + r < n) ? Af[(r0 + r) + (size_t)n * (k + q)] : make_double2(0, 0);
+    }
+    for (int e = tid; e < nb * 64; e += 256) {
+        const int q = e % nb, c = e / nb;
+        sU[q][c] = (c0 + c < n) ? Af[(k + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
+    }
+    __syncthreads();
+    const int tr = tid % 16, tc = tid / 16;
+    double2 acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0, 0);
+    for (int q = 0; q < nb; ++q) {
+        double2 l[4], u[4];
+        for (int a = 0; a < 4; ++a) l[a] = sL[tr + 16 * a][q];
+        for (int b = 0; b < 4; ++b) u[b] = sU[q][tc + 16 * b];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                acc[a][b].x += l[a].x * u[b].x - l[a].y * u[b].y;
+                acc[a][b].y += l[a].x * u[b].y + l[a].y * u[b].x;
+            }
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            const int r = r0 + tr + 16 * a, c = c0 + tc + 16 * b;
+            if (r < n && c < n) {
+                Af[r + (size_t)n * c].x -= acc[a][b].x;
+                Af[r + (size_t)n * c].y -= acc[a][b].y;
+            }
+        }
+}
+
+// Solve (LU) x = P b in place for nrhs columns; one workgroup per (rhs, frequency).
+__global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restrict__ A, size_t lda_f,
+                                                  const int *__restrict__ piv, double2 *__restrict__ X, size_t ldx_f,
+                                                  int ldx) {
+    const int f = blockIdx.z, rcol = blockIdx.x;
+    const double2 *Af = A + lda_f * f;
+    double2 *x = X + ldx_f * f + (size_t)ldx * rcol;
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        for (int c = 0; c < n; ++c) {
+            const int p = piv[(size_t)n * f + c];
+            if (p != c) {
+                const double2 t = x[c];
+                x[c] = x[p];
+                x[p] = t;
+            }
+        }
+    __syncthreads();
+    for (int c = 0; c < n; ++c) {  // unit lower
+        const double2 v = x[c];
+        for (int r = c + 1 + tid; r < n; r += 256) {
+            const double2 p = cmul(Af[r + (size_t)n * c], v);
+            x[r].x -= p.x;
+            x[r].y -= p.y;
+        }
+        __syncthreads();
+    }
+    for (int c = n - 1; c >= 0; --c) {  // upper
+        if (tid == 0) x[c] = cdiv(x[c], Af[c + (size_t)n * c]);
+        __syncthreads();
+        const double2 v = x[c];
+        for (int r = tid; r < c; r += 256) {
+            const double2 p = cmul(Af[r + (size_t)n * c], v);
+            x[r].x -= p.x;
+            x[r].y -= p.y;
+        }
+        __syncthreads();
+    }
+}
+
+// A_f = M + diag shift (atoms, homogeneous): q rows -z_q, mu rows Lambda.
+__global__ void k_assemble_atoms(int n_at, int nv, int nt, int fmu, const double *__restrict__ M,
+                                 const double4 *__restrict__ pos, const double *__restrict__ nbfrac,
+                                 const SysParam *__restrict__ par, double2 *__restrict__ A, size_t lda_f) {
+    const int f = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (r >= nv) return;
+    double2 v = make_double2(M[r + (size_t)nv * col], 0.0);
+    if (r == col) {
+        if (r < n_at) {
+            const double2 z = par[f].zq[(int)pos[r].w];
+            v.x -= z.x;
+            v.y -= z.y;
+        } else if (fmu) {
+            const int i = (r - n_at) / 3;
+            double2 lam = make_double2(0, 0);
+            for (int t = 0; t < nt; ++t) {
+                lam.x += nbfrac[t + (size_t)nt * i] * par[f].inv_alpha[t].x;
+                lam.y += nbfrac[t + (size_t)nt * i] * par[f].inv_alpha[t].y;
+            }
+            v = lam;  // wfqfmu_module.f90:525-528 sets the mu diagonal
+        }
+    }
+    A[lda_f * f + r + (size_t)nv * col] = v;
+}
+
+// Alloys: q rows of A_f = 1/2 (K + H(omega_f)) T(q rows), via the K-bar CSR.
+__global__ void k_assemble_hetero_q(int n_at, int nv, int nt, const int *__restrict__ rowptr,
+                                    const int *__restrict__ col, const double *__restrict__ kij,
+                                    const double *__restrict__ kji, const double4 *__restrict__ pos,
+                                    const double *__restrict__ T, const SysParam *__restrict__ par,
+                                    double2 *__restrict__ A, size_t lda_f) {
+    const int f = blockIdx.z;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (c >= nv) return;
+    const int ti = (int)pos[i].w;
+    const double tic = T[i + (size_t)nv * c];
+    double2 acc = make_double2(0, 0);
+    for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) {
+        const int j = col[z];
+        const double2 wr = par[f].wratio[ti + nt * (int)pos[j].w];
+        const double d = T[j + (size_t)nv * c] - tic;
+        acc.x += 0.5 * (kij[z] + wr.x * kji[z]) * d;
+        acc.y += 0.5 * wr.y * kji[z] * d;
+    }
+    double2 v = acc;
+    if (c == i) {
+        const double2 z = par[f].zq[ti];
+        v.x -= z.x;
+        v.y -= z.y;
+    }
+    A[lda_f * f + i + (size_t)nv * c] = v;
+}
+
+// BEM: DPCM A = D + lambda I; IEF A = -(D + lambda I) S A^-1.
+__global__ void k_assemble_bem_dpcm(int n, const double *__restrict__ D, const double2 *__restrict__ lam,
+                                    double2 *__restrict__ A, size_t lda_f) {
+    const int f = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n) return;
+    double2 v = make_double2(D[r + (size_t)n * c], 0.0);
+    if (r == c) {
+        v.x += lam[f].x;
+        v.y += lam[f].y;
+    }
+    A[lda_f * f + r + (size_t)n * c] = v;
+}
+
+__global__ __launch_bounds__(256) void k_assemble_bem_ief(int n, const double *__restrict__ D,
+                                                          const double *__restrict__ SA,
+                                                          const double2 *__restrict__ lam, double2 *__restrict__ A,
+                                                          size_t lda_f) {
+    // (D S)(r, c) computed with a 64 x 64 tile per workgroup; then - (DS + lambda S)
+    const int f = blockIdx.z;
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    __shared__ double sD[64][33], sS[32][65];
+    const int tid = threadIdx.x, tr = tid % 16, tc = tid / 16;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < n; k0 += 32) {
+        for (int e = tid; e < 64 * 32; e += 256) {
+            const int r = e % 64, q = e / 64;
+            sD[r][q] = (r0 + r < n && k0 + q < n) ? D[(r0 + r) + (size_t)n * (k0 + q)] : 0.0;
+        }
+        for (int e = tid; e < 32 * 64; e += 256) {
+            const int q = e % 32, cc = e / 32;
+            sS[q][cc] = (k0 + q < n && c0 + cc < n) ? SA[(k0 + q) + (size_t)n * (c0 + cc)] : 0.0;
+        }
+        __syncthreads();
+        for (int q = 0; q < 32; ++q)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) acc[a][b] += sD[tr + 16 * a][q] * sS[q][tc + 16 * b];
+        __syncthreads();
+    }
+    const double2 l = lam[f];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            const int r = r0 + tr + 16 * a, c = c0 + tc + 16 * b;
+            if (r < n && c < n) {
+                const double s = SA[r + (size_t)n * c];
+                A[lda_f * f + r + (size_t)n * c] = make_double2(-(acc[a][b] + l.x * s), -(l.y * s));
+            }
+        }
+}
+
+// Copy right-hand sides of one frequency: X_f(:, k) = rhs_f(:, k) for k in pol_mask, 0 otherwise.
+__global__ void k_copy_rhs(int n, int nrhs, int mask, const double2 *__restrict__ rhs, size_t ld_rhs_f,
+                           double2 *__restrict__ X, size_t ldx_f) {
+    const int f = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (r >= n) return;
+    const bool on = (mask >> k) & 1;
+    X[ldx_f * f + (size_t)n * k + r] = on ? rhs[ld_rhs_f * f + (size_t)n * k + r] : make_double2(0, 0);
+}
+
+__global__ void k_fill_one(int n, double2 *__restrict__ X, size_t ldx_f) {
+    const int f = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) X[ldx_f * f + r] = make_double2(1.0, 0.0);
+}
+
+// Charge constraint: x_k -= x1 * sum(x_k) / sum(x1)   (algorithm_module.f90:306-311)
+__global__ __launch_bounds__(256) void k_charge_constraint(int n, double2 *__restrict__ X, size_t ldx_f,
+                                                           const double2 *__restrict__ X1, size_t ldx1_f) {
+    const int f = blockIdx.z, k = blockIdx.x;
+    double2 *x = X + ldx_f * f + (size_t)n * k;
+    const double2 *x1 = X1 + ldx1_f * f;
+    __shared__ double s[4][256];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int r = threadIdx.x; r < n; r += 256) {
+        a += x[r].x;
+        b += x[r].y;
+        c += x1[r].x;
+        d += x1[r].y;
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    s[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    const double2 sx = make_double2(s[0][0], s[1][0]), s1 = make_double2(s[2][0], s[3][0]);
+    const double2 ratio = cdiv(sx, s1);
+    for (int r = threadIdx.x; r < n; r += 256) {
+        const double2 p = cmul(x1[r], ratio);
+        x[r].x -= p.x;
+        x[r].y -= p.y;
+    }
+}
+
+// Stored-matrix operator: y_s = A_{f(s)} x_s.
+__global__ void k_dense_matvec(int n, const double2 *__restrict__ A, size_t lda_f, const int *__restrict__ slots,
+                               const int *__restrict__ slot_freq, const double2 *__restrict__ x, int ld,
+                               double2 *__restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= n) return;
+    const double2 *Af = A + lda_f * slot_freq[s];
+    const double2 *xs = x + (size_t)ld * s;
+    double2 acc = make_double2(0, 0);
+    for (int c = 0; c < n; ++c) {
+        const double2 p = cmul(Af[r + (size_t)n * c], xs[c]);
+        acc.x += p.x;
+        acc.y += p.y;
+    }
+    y[(size_t)ld * s + r] = acc;
+}
+
+void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, int *info) {
+    hipStream_t st = c->stream;
+    const size_t ldf = (size_t)n * n;
+    for (int k = 0; k < n; k += NB) {
+        const int nb = std::min(NB, n - k);
+        k_panel<<<dim3(1, 1, nf), 256, 0, st>>>(n, k, nb, A, ldf, piv, info);
+        k_swap_rest<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, k, nb, A, ldf, piv);
+        const int rest = n - k - nb;
+        if (rest > 0) {
+            k_trsm_u12<<<dim3((rest + 127) / 128, 1, nf), 128, 0, st>>>(n, k, nb, A, ldf);
+            const int tiles = (rest + 63) / 64;
+            k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, nb, A, ldf);
+        }
+    }
+    PXH_HIP(hipGetLastError());
+}
+
+// Assemble A(omega_f) for frequencies [f0, f0 + nf) into A.
+void assemble(pxh_ctx *c, int f0, int nf, const double *freq, double2 *A, DevBuf<double> &M, DevBuf<double> &T,
+              DevBuf<SysParam> &par, DevBuf<double2> &lam) {
+    hipStream_t st = c->stream;
+    if (c->target == PXH_BEM) {
+        const Mesh &m = c->mesh;
+        const int n = m.n;
+        std::vector<double2> hl(nf);
+        for (int f = 0; f < nf; ++f) {
+            const double *e = c->eps.data() + 2 * (size_t)(f0 + f);
+            const double es = m.eps_solvent;
+            const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
+            const double den = dr * dr + di * di;
+            hl[f] = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
+        }
+        lam.upload(hl.data(), nf, st);
+        if (m.variant == PXH_DPCM)
+            k_assemble_bem_dpcm<<<dim3((n + 255) / 256, n, nf), 256, 0, st>>>(n, m.D.p, lam.p, A, (size_t)n * n);
+        else
+            k_assemble_bem_ief<<<dim3((n + 63) / 64, (n + 63) / 64, nf), 256, 0, st>>>(n, m.D.p, m.SA.p, lam.p, A,
+                                                                                        (size_t)n * n);
+        PXH_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    Atoms &a = c->atoms;
+    const int nv = a.n_var();
+    std::vector<SysParam> hp(nf);
+    for (int f = 0; f < nf; ++f) hp[f] = make_sysparam(c, freq[f0 + f], f0 + f);
+    par.upload(hp.data(), nf, st);
+    k_assemble_atoms<<<dim3((nv + 255) / 256, nv, nf), 256, 0, st>>>(
+        a.n, nv, a.nt, a.model == PXH_WFQFMU, M.p, a.pos.p, a.d_nbfrac.p, par.p, A, (size_t)nv * nv);
+    if (a.hetero)
+        k_assemble_hetero_q<<<dim3((nv + 255) / 256, a.n, nf), 256, 0, st>>>(
+            a.n, nv, a.nt, a.kb_rowptr.p, a.kb_col.p, a.kb_kij.p, a.kb_kji.p, a.pos.p, T.p, par.p, A,
+            (size_t)nv * nv);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(st));
+}
+
+// Dense T of the atoms (for the alloy q rows), n_var x n_var.
+void dense_t(pxh_ctx *c, DevBuf<double> &T) {
+    const size_t nv = c->atoms.n_var();
+    T.alloc(nv * nv);
+    dense_t_atoms(c, T.p, false);
+}
+
+size_t freq_chunk(pxh_ctx *c, int n, int n_freq) {
+    size_t free_b = 0, total_b = 0;
+    PXH_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per = (size_t)n * n * sizeof(double2) * 2 + 1;
+    size_t k = std::max<size_t>(1, (free_b / 2) / per);
+    if (c->budget) k = std::max<size_t>(1, std::min(k, c->budget / per));
+    return std::min<size_t>(k, (size_t)n_freq);
+}
+
+}  // namespace
+
+void constant_matrix_atoms(pxh_ctx *c, double *d_m);
+
+int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_stride,
+                          double2 *d_sol, int pol_mask) {
+    hipStream_t st = c->stream;
+    const bool bem = c->target == PXH_BEM;
+    const int n = bem ? c->mesh.n : c->atoms.n_var();
+    DevBuf<double> M, T;
+    if (!bem) {
+        M.alloc((size_t)n * n);
+        constant_matrix_atoms(c, M.p);
+        if (c->atoms.hetero) dense_t(c, T);
+    }
+    const int chunk = (int)freq_chunk(c, n, n_freq);
+    DevBuf<double2> A, A1;
+    DevBuf<int> piv, info;
+    DevBuf<SysParam> par;
+    DevBuf<double2> lam, X1;
+    A.alloc((size_t)n * n * chunk);
+    piv.alloc((size_t)n * chunk);
+    info.alloc(chunk);
+    const bool cc = bem && c->mesh.charge_constraint;
+    if (cc) X1.alloc((size_t)n * chunk);
+    int status = 0;
+    for (int f0 = 0; f0 < n_freq; f0 += chunk) {
+        const int nf = std::min(chunk, n_freq - f0);
+        assemble(c, f0, nf, freq, A.p, M, T, par, lam);
+        PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, st));
+        lu_factor(c, n, nf, A.p, piv.p, info.p);
+        double2 *X = d_sol + (size_t)rhs_stride * f0;
+        k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, 3, pol_mask, d_rhs_all + (size_t)rhs_stride * f0,
+                                                                 (size_t)rhs_stride, X, (size_t)rhs_stride);
+        k_lu_solve<<<dim3(3, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X, (size_t)rhs_stride, n);
+        if (cc) {
+            k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, X1.p, (size_t)n);
+            k_lu_solve<<<dim3(1, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X1.p, (size_t)n, n);
+            k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, X, (size_t)rhs_stride, X1.p, (size_t)n);
+        }
+        PXH_HIP(hipGetLastError());
+        std::vector<int> hi(nf);
+        PXH_HIP(hipMemcpyAsync(hi.data(), info.p, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipStreamSynchronize(st));
+        for (int f = 0; f < nf; ++f)
+            if (hi[f] != 0) status = PXH_ESINGULAR;
+    }
+    return status;
+}
+
+}  // namespace pxh

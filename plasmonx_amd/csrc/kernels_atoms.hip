@@ -1,0 +1,780 @@
+// Interaction kernels of the atomistic models (omegaFQ, omegaFQFmu) for
+// gfx950: on-the-fly T sweep, sparse K-bar, diagonal shifts, dynamic RHS and
+// the dense constant matrix.
+//
+// The arithmetic restates src/matrix/matrix_module.f90 (T_qq :58-115,
+// T_qmu/T_muq :120-234, T_mumu :240-345, K-bar :393-426, H-bar :435-482,
+// Fermi :694-715, on-the-fly products :727-1125), src/rhs/rhs_module.f90
+// (:43-430) and the diagonal shifts of wfq_module.f90:398-424 and
+// wfqfmu_module.f90:698-746.
+//
+// Layout in HBM: positions as double4 (x, y, z, type); a block of systems is
+// [system][n_var] complex (double2), each system in the reference's order
+// [q_1..q_N, mu_1x, mu_1y, mu_1z, ...].  The T sweep is a gather: one
+// work-item owns a target atom i and walks the source atoms j through LDS
+// tiles; the source range is split over blockIdx.y so that a launch has
+// thousands of workgroups, and the split partial sums are added in a fixed
+// order afterwards (deterministic results, no atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "pxh_internal.h"
+
+namespace pxh {
+
+namespace {
+
+constexpr double kTwoOverSqrtPi = 1.1283791670955126;
+constexpr double kFourOverSqrtPi = 2.2567583341910252;
+constexpr int TB = 256;  // work-items per workgroup = target atoms per tile = source atoms per LDS tile
+
+template <typename R>
+struct Cx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ R d_erf(R x);
+template <>
+__device__ __forceinline__ double d_erf<double>(double x) {
+    return erf(x);
+}
+template <>
+__device__ __forceinline__ float d_erf<float>(float x) {
+    return erff(x);
+}
+template <typename R>
+__device__ __forceinline__ R d_exp(R x);
+template <>
+__device__ __forceinline__ double d_exp<double>(double x) {
+    return exp(x);
+}
+template <>
+__device__ __forceinline__ float d_exp<float>(float x) {
+    return __expf(x);
+}
+template <typename R>
+__device__ __forceinline__ R d_rsqrt(R x);
+template <>
+__device__ __forceinline__ double d_rsqrt<double>(double x) {
+    return 1.0 / sqrt(x);
+}
+template <>
+__device__ __forceinline__ float d_rsqrt<float>(float x) {
+    return rsqrtf(x);
+}
+
+template <typename R>
+struct PairR {
+    R inv_rqq, inv_rqmu, inv_rmumu, eta_bar;
+};
+
+// Source-atom record staged in LDS for KV systems.
+template <typename R, int KV>
+struct SrcTile {
+    double4 pos[TB];
+    Cx<R> q[KV][TB];
+    Cx<R> m[KV][3][TB];
+};
+
+// T sweep: part[split][k][row] = sum_{j in split, j != i} T(i, j) x_k(j).
+// KERN: 1 Gaussian, 2 Coulomb, 3 Ohno.  FMU: omegaFQFmu (4 unknowns/atom).
+template <typename R, int KV, bool FMU, int KERN>
+__global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const double4 *__restrict__ pos,
+                                               const PairConst *__restrict__ pc, int nsys,
+                                               const int *__restrict__ sys_list, const double2 *__restrict__ x,
+                                               int ld, double2 *__restrict__ part, int nv, int j_chunk) {
+    __shared__ SrcTile<R, KV> tile;
+    __shared__ PairR<R> spc[kMaxTypes * kMaxTypes];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * TB + tid;
+    const int split = blockIdx.y;
+    const int g0 = blockIdx.z * KV;  // first system slot of this group
+    const int nk = min(KV, nsys - g0);
+    for (int e = tid; e < nt * nt; e += TB) {
+        spc[e].inv_rqq = (R)pc[e].inv_rqq;
+        spc[e].inv_rqmu = (R)pc[e].inv_rqmu;
+        spc[e].inv_rmumu = (R)pc[e].inv_rmumu;
+        spc[e].eta_bar = (R)pc[e].eta_bar;
+    }
+    const bool active = i < n;
+    double4 pi = active ? pos[i] : make_double4(0, 0, 0, 0);
+    const int ti = (int)pi.w;
+    double aq_re[KV], aq_im[KV], am_re[KV][3], am_im[KV][3];
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+        aq_re[v] = aq_im[v] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) am_re[v][a] = am_im[v][a] = 0.0;
+    }
+    const int j_begin = split * j_chunk;
+    const int j_end = min(n, j_begin + j_chunk);
+    for (int jt = j_begin; jt < j_end; jt += TB) {
+        __syncthreads();
+        {
+            const int j = jt + tid;
+            if (j < j_end) {
+                tile.pos[tid] = pos[j];
+#pragma unroll
+                for (int v = 0; v < KV; ++v) {
+                    if (v < nk) {
+                        const double2 *xs = x + (size_t)sys_list[g0 + v] * ld;
+                        double2 q = xs[j];
+                        tile.q[v][tid].re = (R)q.x;
+                        tile.q[v][tid].im = (R)q.y;
+                        if (FMU) {
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                double2 m = xs[n + 3 * j + a];
+                                tile.m[v][a][tid].re = (R)m.x;
+                                tile.m[v][a][tid].im = (R)m.y;
+                            }
+                        }
+                    } else {
+                        tile.q[v][tid].re = tile.q[v][tid].im = (R)0;
+                        if (FMU) {
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) tile.m[v][a][tid].re = tile.m[v][a][tid].im = (R)0;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int cnt = min(TB, j_end - jt);
+        R tq_re[KV], tq_im[KV], tm_re[KV][3], tm_im[KV][3];
+#pragma unroll
+        for (int v = 0; v < KV; ++v) {
+            tq_re[v] = tq_im[v] = (R)0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) tm_re[v][a] = tm_im[v][a] = (R)0;
+        }
+        if (active) {
+            for (int jj = 0; jj < cnt; ++jj) {
+                const int j = jt + jj;
+                if (j == i) continue;
+                const double4 pj = tile.pos[jj];
+                const R dx = (R)(pi.x - pj.x), dy = (R)(pi.y - pj.y), dz = (R)(pi.z - pj.z);
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R inv_r = d_rsqrt<R>(r2);
+                const R r = r2 * inv_r;
+                const int tj = (int)pj.w;
+                const PairR<R> p = spc[ti + nt * tj];
+                R tqq;
+                if (KERN == 1)
+                    tqq = d_erf<R>(r * p.inv_rqq) * inv_r;
+                else if (KERN == 2)
+                    tqq = inv_r;
+                else
+                    tqq = p.eta_bar * d_rsqrt<R>((R)1 + p.eta_bar * p.eta_bar * r2);
+                if (!FMU) {
+#pragma unroll
+                    for (int v = 0; v < KV; ++v) {
+                        tq_re[v] += tqq * tile.q[v][jj].re;
+                        tq_im[v] += tqq * tile.q[v][jj].im;
+                    }
+                    continue;
+                }
+                const R inv_r2 = inv_r * inv_r;
+                const R inv_r3 = inv_r2 * inv_r;
+                // charge on i, dipole on j
+                const R a1 = r * p.inv_rqmu;
+                const R e1 = d_exp<R>(-a1 * a1);
+                const R g1 = (d_erf<R>(a1) - (R)kTwoOverSqrtPi * a1 * e1) * inv_r3;
+                R g2 = g1;
+                if (hetero) {  // charge on j, dipole on i
+                    const R a2 = r * spc[tj + nt * ti].inv_rqmu;
+                    const R e2 = d_exp<R>(-a2 * a2);
+                    g2 = (d_erf<R>(a2) - (R)kTwoOverSqrtPi * a2 * e2) * inv_r3;
+                }
+                const R a3 = r * p.inv_rmumu;
+                const R e3 = d_exp<R>(-a3 * a3);
+                const R fd1 = d_erf<R>(a3) - (R)kTwoOverSqrtPi * a3 * e3;
+                const R fd2 = (R)kFourOverSqrtPi * p.inv_rmumu * p.inv_rmumu * p.inv_rmumu * e3;
+                const R c0 = fd1 * inv_r3;
+                const R c1 = fd2 * inv_r2 - (R)3 * fd1 * inv_r3 * inv_r2;
+#pragma unroll
+                for (int v = 0; v < KV; ++v) {
+                    const Cx<R> q = tile.q[v][jj];
+                    const Cx<R> mx = tile.m[v][0][jj], my = tile.m[v][1][jj], mz = tile.m[v][2][jj];
+                    const R dm_re = dx * mx.re + dy * my.re + dz * mz.re;
+                    const R dm_im = dx * mx.im + dy * my.im + dz * mz.im;
+                    tq_re[v] += tqq * q.re + g1 * dm_re;
+                    tq_im[v] += tqq * q.im + g1 * dm_im;
+                    const R s_re = c1 * dm_re - g2 * q.re;
+                    const R s_im = c1 * dm_im - g2 * q.im;
+                    tm_re[v][0] += s_re * dx + c0 * mx.re;
+                    tm_im[v][0] += s_im * dx + c0 * mx.im;
+                    tm_re[v][1] += s_re * dy + c0 * my.re;
+                    tm_im[v][1] += s_im * dy + c0 * my.im;
+                    tm_re[v][2] += s_re * dz + c0 * mz.re;
+                    tm_im[v][2] += s_im * dz + c0 * mz.im;
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < KV; ++v) {
+            aq_re[v] += (double)tq_re[v];
+            aq_im[v] += (double)tq_im[v];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                am_re[v][a] += (double)tm_re[v][a];
+                am_im[v][a] += (double)tm_im[v][a];
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+        if (v >= nk) break;
+        double2 *out = part + ((size_t)split * nsys + g0 + v) * nv;
+        out[i] = make_double2(aq_re[v], aq_im[v]);
+        if (FMU) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[n + 3 * i + a] = make_double2(am_re[v][a], am_im[v][a]);
+        }
+    }
+}
+
+// t[k][row] = sum_split part[split][k][row] + (q rows) eta_i x_q(i).
+__global__ void k_reduce_splits(int n, int nv, int nsys, int splits, const double2 *__restrict__ part,
+                                const int *__restrict__ sys_list, const double2 *__restrict__ x, int ld,
+                                const double4 *__restrict__ pos, const double *__restrict__ eta,
+                                double2 *__restrict__ t) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (row >= nv) return;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < splits; ++s) {
+        double2 p = part[((size_t)s * nsys + k) * nv + row];
+        re += p.x;
+        im += p.y;
+    }
+    if (row < n) {
+        double e = eta[(int)pos[row].w];
+        double2 xq = x[(size_t)sys_list[k] * ld + row];
+        re += e * xq.x;
+        im += e * xq.y;
+    }
+    t[(size_t)k * nv + row] = make_double2(re, im);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// q rows: y_k(i) = sum_j c_ij (t_k(j) - t_k(i)) - z_q(t_i) x_k(i), one wavefront per row.
+// c_ij = K_ij, or 1/2 (K_ij + (w_tj / w_ti) K_ji) for alloys.
+__global__ __launch_bounds__(64) void k_kbar_q(int n, int nt, int hetero, const int *__restrict__ rowptr,
+                                               const int *__restrict__ col, const double *__restrict__ kij,
+                                               const double *__restrict__ kji, const double4 *__restrict__ pos,
+                                               int nsys, const int *__restrict__ sys_list,
+                                               const SysParam *__restrict__ par, const double2 *__restrict__ t,
+                                               int nv, const double2 *__restrict__ x, int ld,
+                                               double2 *__restrict__ y) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int ti = (int)pos[i].w;
+    const int b = rowptr[i], e = rowptr[i + 1];
+    for (int k = 0; k < nsys; ++k) {
+        const int s = sys_list[k];
+        const double2 *tk = t + (size_t)k * nv;
+        const double2 tii = tk[i];
+        double re = 0.0, im = 0.0;
+        for (int z = b + lane; z < e; z += 64) {
+            const int j = col[z];
+            const double2 tj = tk[j];
+            const double2 d = make_double2(tj.x - tii.x, tj.y - tii.y);
+            if (!hetero) {
+                re += kij[z] * d.x;
+                im += kij[z] * d.y;
+            } else {
+                const int tj_t = (int)pos[j].w;
+                const double2 wr = par[s].wratio[ti + nt * tj_t];
+                const double2 c = make_double2(0.5 * (kij[z] + wr.x * kji[z]), 0.5 * wr.y * kji[z]);
+                const double2 p = cmul(c, d);
+                re += p.x;
+                im += p.y;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_down(re, off, 64);
+            im += __shfl_down(im, off, 64);
+        }
+        if (lane == 0) {
+            const double2 xi = x[(size_t)s * ld + i];
+            const double2 zx = cmul(par[s].zq[ti], xi);
+            y[(size_t)s * ld + i] = make_double2(re - zx.x, im - zx.y);
+        }
+    }
+}
+
+// mu rows: y_k = t_k + Lambda_i x_k, Lambda_i = sum_t nbfrac(t, i) / alpha_t.
+__global__ void k_mu_rows(int n, int nt, const double *__restrict__ nbfrac, int nsys,
+                          const int *__restrict__ sys_list, const SysParam *__restrict__ par,
+                          const double2 *__restrict__ t, int nv, const double2 *__restrict__ x, int ld,
+                          double2 *__restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (r >= 3 * n) return;
+    const int s = sys_list[k];
+    const int i = r / 3;
+    double2 lam = make_double2(0, 0);
+    for (int tt = 0; tt < nt; ++tt) {
+        const double f = nbfrac[tt + (size_t)nt * i];
+        lam.x += f * par[s].inv_alpha[tt].x;
+        lam.y += f * par[s].inv_alpha[tt].y;
+    }
+    const double2 xv = x[(size_t)s * ld + n + r];
+    const double2 tv = t[(size_t)k * nv + n + r];
+    const double2 lx = cmul(lam, xv);
+    y[(size_t)s * ld + n + r] = make_double2(tv.x + lx.x, tv.y + lx.y);
+}
+
+// ---------------------------------------------------------------- K-bar CSR
+__device__ __forceinline__ double fermi_one_minus(double d, double s, double r0, double r) {
+    // 1 - f with f = 1/(1+exp(-d (r/(s r0) - 1)))  (matrix_module.f90:694-715)
+    const double f = 1.0 / (1.0 + exp(-d * ((r / (s * r0)) - 1.0)));
+    return 1.0 - f;
+}
+
+struct FermiTab {
+    double d[kMaxTypes * kMaxTypes], s[kMaxTypes * kMaxTypes], r0[kMaxTypes * kMaxTypes],
+        rcut2[kMaxTypes * kMaxTypes];
+    double a[kMaxTypes];
+};
+
+__global__ void k_kbar_count(int n, int nt, const double4 *__restrict__ pos, FermiTab ft, int *__restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    const int ti = (int)pi.w;
+    int c = 0;
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double4 pj = pos[j];
+        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int tj = (int)pj.w;
+        if (r2 <= ft.rcut2[ti + nt * tj] || r2 <= ft.rcut2[tj + nt * ti]) ++c;
+    }
+    cnt[i] = c;
+}
+
+__global__ void k_kbar_fill(int n, int nt, const double4 *__restrict__ pos, FermiTab ft,
+                            const int *__restrict__ rowptr, int *__restrict__ col, double *__restrict__ kij,
+                            double *__restrict__ kji) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    const int ti = (int)pi.w;
+    int z = rowptr[i];
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double4 pj = pos[j];
+        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int tj = (int)pj.w;
+        if (r2 <= ft.rcut2[ti + nt * tj] || r2 <= ft.rcut2[tj + nt * ti]) {
+            const double r = sqrt(r2);
+            const int a = ti + nt * tj, b = tj + nt * ti;
+            col[z] = j;
+            kij[z] = fermi_one_minus(ft.d[a], ft.s[a], ft.r0[a], r) * ft.a[ti] / r;
+            kji[z] = fermi_one_minus(ft.d[b], ft.s[b], ft.r0[b], r) * ft.a[tj] / r;
+            ++z;
+        }
+    }
+}
+
+// Dynamic RHS, q rows: rhs(i,k) = -sum_j c_ij (V_k(j) - V_k(i)), c_ij as in k_kbar_q.
+__global__ __launch_bounds__(64) void k_rhs_q(int n, int nt, int hetero, const int *__restrict__ rowptr,
+                                              const int *__restrict__ col, const double *__restrict__ kij,
+                                              const double *__restrict__ kji, const double4 *__restrict__ pos,
+                                              SysParam par, double e0, int potential, double3 mn, int nv,
+                                              double2 *__restrict__ rhs) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double4 pi = pos[i];
+    const int ti = (int)pi.w;
+    auto V = [&](const double4 &p, int k) {
+        const double c = k == 0 ? p.x : (k == 1 ? p.y : p.z);
+        const double m = k == 0 ? mn.x : (k == 1 ? mn.y : mn.z);
+        return potential ? -e0 * fabs(c - m) : -e0 * c;
+    };
+    double acc_re[3] = {0, 0, 0}, acc_im[3] = {0, 0, 0};
+    for (int z = rowptr[i] + lane; z < rowptr[i + 1]; z += 64) {
+        const int j = col[z];
+        const double4 pj = pos[j];
+        double2 c = make_double2(kij[z], 0.0);
+        if (hetero) {
+            const double2 wr = par.wratio[ti + nt * (int)pj.w];
+            c = make_double2(0.5 * (kij[z] + wr.x * kji[z]), 0.5 * wr.y * kji[z]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            const double dv = V(pj, k) - V(pi, k);
+            acc_re[k] -= c.x * dv;
+            acc_im[k] -= c.y * dv;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        double re = acc_re[k], im = acc_im[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_down(re, off, 64);
+            im += __shfl_down(im, off, 64);
+        }
+        if (lane == 0) rhs[(size_t)k * nv + i] = make_double2(re, im);
+    }
+}
+
+__global__ void k_rhs_mu(int n, int nv, double e0, double2 *__restrict__ rhs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * n) return;
+    const int a = r % 3;
+    for (int k = 0; k < 3; ++k) rhs[(size_t)k * nv + n + r] = make_double2(k == a ? e0 : 0.0, 0.0);
+}
+
+// Dense T (n_var x n_var, column-major): T_qq, T_qmu, T_muq, T_mumu blocks.
+__global__ void k_dense_t(int n, int nt, int fmu, int kern, const double4 *__restrict__ pos,
+                          const PairConst *__restrict__ pc, const double *__restrict__ eta, int nv,
+                          double *__restrict__ T) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // row atom
+    const int j = blockIdx.y;                              // column atom
+    if (i >= n) return;
+    const double4 pi = pos[i], pj = pos[j];
+    const int ti = (int)pi.w, tj = (int)pj.w;
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const PairConst p = pc[ti + nt * tj];
+    double tqq;
+    if (kern == 3)
+        tqq = p.eta_bar / sqrt(1.0 + p.eta_bar * p.eta_bar * r2);
+    else if (i == j)
+        tqq = 0.5 * (eta[ti] + eta[tj]);
+    else if (kern == 2)
+        tqq = 1.0 / sqrt(r2);
+    else
+        tqq = erf(sqrt(r2) * p.inv_rqq) / sqrt(r2);
+    T[i + (size_t)nv * j] = tqq;
+    if (!fmu) return;
+    double d[3] = {dx, dy, dz};
+    double g1 = 0, g2 = 0, c0 = 0, c1 = 0;
+    if (i != j) {
+        const double r = sqrt(r2), ir = 1.0 / r, ir3 = ir * ir * ir;
+        const double a1 = r * p.inv_rqmu, e1 = exp(-a1 * a1);
+        g1 = (erf(a1) - kTwoOverSqrtPi * a1 * e1) * ir3;
+        const double a2 = r * pc[tj + nt * ti].inv_rqmu, e2 = exp(-a2 * a2);
+        g2 = (erf(a2) - kTwoOverSqrtPi * a2 * e2) * ir3;
+        const double a3 = r * p.inv_rmumu, e3 = exp(-a3 * a3);
+        const double fd1 = erf(a3) - kTwoOverSqrtPi * a3 * e3;
+        const double fd2 = kFourOverSqrtPi * p.inv_rmumu * p.inv_rmumu * p.inv_rmumu * e3;
+        c0 = fd1 * ir3;
+        c1 = fd2 * ir * ir - 3.0 * fd1 * ir3 * ir * ir;
+    }
+    for (int a = 0; a < 3; ++a) {
+        T[i + (size_t)nv * (n + 3 * j + a)] = g1 * d[a];         // T_qmu(i, j)
+        T[(n + 3 * i + a) + (size_t)nv * j] = -g2 * d[a];        // T_muq(i, j) = T_qmu(j, i)
+        for (int b = 0; b < 3; ++b)
+            T[(n + 3 * i + a) + (size_t)nv * (n + 3 * j + b)] = c1 * d[a] * d[b] + (a == b ? c0 : 0.0);
+    }
+}
+
+// M(q rows) = K-bar T(q rows):  M(i, c) = sum_j K_ij (T(j, c) - T(i, c)).
+__global__ void k_kbar_dense(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                             const double *__restrict__ kij, int nv, const double *__restrict__ T,
+                             double *__restrict__ M) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (c >= nv) return;
+    const double tic = T[i + (size_t)nv * c];
+    double acc = 0.0;
+    for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) acc += kij[z] * (T[col[z] + (size_t)nv * c] - tic);
+    M[i + (size_t)nv * c] = acc;
+}
+
+__global__ void k_copy_mu_rows(int n, int nv, const double *__restrict__ T, double *__restrict__ M) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;  // mu row index 0..3n-1
+    if (c >= nv) return;
+    M[(n + r) + (size_t)nv * c] = T[(n + r) + (size_t)nv * c];
+}
+
+template <typename R, int KV, bool FMU>
+void launch_tsweep_k(int kern, dim3 grid, hipStream_t st, int n, int nt, int hetero, const double4 *pos,
+                     const PairConst *pc, int nsys, const int *sys_list, const double2 *x, int ld, double2 *part,
+                     int nv, int j_chunk) {
+    switch (kern) {
+        case PXH_COULOMB:
+            k_tsweep<R, KV, FMU, 2><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
+                                                         j_chunk);
+            break;
+        case PXH_OHNO:
+            k_tsweep<R, KV, FMU, 3><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
+                                                         j_chunk);
+            break;
+        default:
+            k_tsweep<R, KV, FMU, 1><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
+                                                         j_chunk);
+    }
+}
+
+int choose_splits(const pxh_ctx *c, int n, int groups) {
+    if (c->opt.splits > 0) return c->opt.splits;
+    const int tiles = (n + TB - 1) / TB;
+    // aim at >= 2048 workgroups, each split spanning at least two LDS tiles
+    int s = std::max(1, 2048 / std::max(1, tiles * groups));
+    const int max_s = std::max(1, (n + 2 * TB - 1) / (2 * TB));
+    return std::min(s, max_s);
+}
+
+// Second stage of every product: t = sum of the split partials (+ eta x_q),
+// then y_q = K-bar t_q - z_q x_q and y_mu = t_mu + Lambda x_mu.
+void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
+                    int ld, int splits) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    double2 *part = c->work_part.p, *tt = c->work_t.p;
+    k_reduce_splits<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(n, nv, nsys, splits, part, d_sys_list, x,
+                                                                        ld, A.pos.p, A.d_eta.p, tt);
+    k_kbar_q<<<n, 64, 0, c->stream>>>(n, A.nt, A.hetero ? 1 : 0, A.kb_rowptr.p, A.kb_col.p, A.kb_kij.p, A.kb_kji.p,
+                                      A.pos.p, nsys, d_sys_list, d_par, tt, nv, x, ld, y);
+    if (A.model == PXH_WFQFMU)
+        k_mu_rows<<<dim3((3 * n + 255) / 256, nsys), 256, 0, c->stream>>>(n, A.nt, A.d_nbfrac.p, nsys, d_sys_list,
+                                                                          d_par, tt, nv, x, ld, y);
+    PXH_HIP(hipGetLastError());
+}
+
+template <typename R, int KV>
+void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    const bool fmu = A.model == PXH_WFQFMU;
+    const int groups = (nsys + KV - 1) / KV;
+    const int splits = choose_splits(c, n, groups);
+    const int j_chunk = ((n + splits - 1) / splits + TB - 1) / TB * TB;
+    const int splits_eff = (n + j_chunk - 1) / j_chunk;
+    c->work_part.alloc((size_t)splits_eff * nsys * nv);
+    c->work_t.alloc((size_t)nsys * nv);
+    double2 *part = c->work_part.p;
+    dim3 grid((n + TB - 1) / TB, splits_eff, groups);
+    if (fmu)
+        launch_tsweep_k<R, KV, true>(A.kernel, grid, c->stream, n, A.nt, A.hetero ? 1 : 0, A.pos.p, A.pair.p, nsys,
+                                     d_sys_list, x, ld, part, nv, j_chunk);
+    else
+        launch_tsweep_k<R, KV, false>(A.kernel, grid, c->stream, n, A.nt, 0, A.pos.p, A.pair.p, nsys, d_sys_list,
+                                      x, ld, part, nv, j_chunk);
+    PXH_HIP(hipGetLastError());
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff);
+    c->stats.v[6] += (double)n * (double)n * groups;
+}
+
+// part[0][k][row] = sum_c T[row, c] x_k[c] with the stored T (q-q diagonal zero).
+__global__ void k_stored_t(int nv, const double *__restrict__ T, int nsys, const int *__restrict__ sys_list,
+                           const double2 *__restrict__ x, int ld, double2 *__restrict__ part) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (row >= nv) return;
+    const double2 *xs = x + (size_t)sys_list[k] * ld;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < nv; ++c) {
+        const double t = T[row + (size_t)nv * c];
+        re += t * xs[c].x;
+        im += t * xs[c].y;
+    }
+    part[(size_t)k * nv + row] = make_double2(re, im);
+}
+
+}  // namespace
+
+void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld) {
+    if (nsys <= 0) return;
+    if (nsys == 1)
+        apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
+    else
+        apply_atoms_impl<double, 2>(c, nsys, d_sys_list, d_par, x, y, ld);
+}
+
+void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld) {
+    if (nsys <= 0) return;
+    if (nsys == 1)
+        apply_atoms_impl<float, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
+    else if (nsys == 2)
+        apply_atoms_impl<float, 2>(c, nsys, d_sys_list, d_par, x, y, ld);
+    else
+        apply_atoms_impl<float, 4>(c, nsys, d_sys_list, d_par, x, y, ld);
+}
+
+SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq) {
+    const Atoms &A = c->atoms;
+    SysParam p{};
+    for (int t = 0; t < A.nt; ++t) {
+        // z_q(omega) = -(omega / (2 n0 tau)) (omega tau + i)   (wfq_module.f90:239-242)
+        const double f = -(freq / (2.0 * A.dens[t] * A.tau[t]));
+        p.zq[t] = make_double2(f * freq * A.tau[t], f);
+        double2 al = make_double2(1.0, 0.0);
+        if (A.model == PXH_WFQFMU && i_freq >= 0 && i_freq < c->alpha_nf) {
+            const size_t o = 2 * ((size_t)t + (size_t)A.nt * i_freq);
+            al = make_double2(c->alpha[o], c->alpha[o + 1]);
+        }
+        const double den = al.x * al.x + al.y * al.y;
+        p.inv_alpha[t] = make_double2(al.x / den, -al.y / den);
+    }
+    // w_t = 2 n_t / (1/tau_t - i omega)   (matrix_module.f90:455-466)
+    double wre[kMaxTypes], wim[kMaxTypes];
+    for (int t = 0; t < A.nt; ++t) {
+        const double a = 1.0 / A.tau[t], b = -freq;
+        const double den = a * a + b * b;
+        wre[t] = 2.0 * A.dens[t] * a / den;
+        wim[t] = -2.0 * A.dens[t] * b / den;
+    }
+    for (int ti = 0; ti < A.nt; ++ti)
+        for (int tj = 0; tj < A.nt; ++tj) {
+            const double den = wre[ti] * wre[ti] + wim[ti] * wim[ti];
+            const double re = (wre[tj] * wre[ti] + wim[tj] * wim[ti]) / den;
+            const double im = (wim[tj] * wre[ti] - wre[tj] * wim[ti]) / den;
+            p.wratio[ti + A.nt * tj] = make_double2(re, im);
+        }
+    return p;
+}
+
+void build_atoms_device(pxh_ctx *c) {
+    Atoms &A = c->atoms;
+    std::vector<double4> hp(A.n);
+    for (int i = 0; i < A.n; ++i)
+        hp[i] = make_double4(A.h_coord[3 * i], A.h_coord[3 * i + 1], A.h_coord[3 * i + 2], (double)A.h_type[i]);
+    A.pos.upload(hp.data(), hp.size(), c->stream);
+    A.h_pair.resize((size_t)A.nt * A.nt);
+    for (int ti = 0; ti < A.nt; ++ti)
+        for (int tj = 0; tj < A.nt; ++tj) {
+            PairConst &p = A.h_pair[ti + A.nt * tj];
+            p.inv_rqq = 1.0 / std::sqrt(A.rq[ti] * A.rq[ti] + A.rq[tj] * A.rq[tj]);
+            p.inv_rqmu = 1.0 / std::sqrt(A.rq[ti] * A.rq[ti] + A.rmu[tj] * A.rmu[tj]);
+            p.inv_rmumu = 1.0 / std::sqrt(A.rmu[ti] * A.rmu[ti] + A.rmu[tj] * A.rmu[tj]);
+            p.eta_bar = 0.5 * (A.eta[ti] + A.eta[tj]);
+        }
+    A.pair.upload(A.h_pair.data(), A.h_pair.size(), c->stream);
+    A.d_eta.upload(A.eta.data(), A.eta.size(), c->stream);
+    if (A.h_nbfrac.empty()) {
+        A.h_nbfrac.assign((size_t)A.nt * A.n, 0.0);
+        for (int i = 0; i < A.n; ++i) A.h_nbfrac[A.h_type[i] + (size_t)A.nt * i] = 1.0;
+    }
+    A.d_nbfrac.upload(A.h_nbfrac.data(), A.h_nbfrac.size(), c->stream);
+    build_kbar(c);
+}
+
+void build_kbar(pxh_ctx *c) {
+    Atoms &A = c->atoms;
+    FermiTab ft{};
+    for (int t = 0; t < A.nt; ++t) ft.a[t] = A.aij[t];
+    for (int e = 0; e < A.nt * A.nt; ++e) {
+        ft.d[e] = A.fd[e];
+        ft.s[e] = A.fs[e];
+        ft.r0[e] = A.r0[e];
+        // 1 - f < 1e-18 beyond x = 1 + ln(1e18)/d, x = r / (s r0): those pairs add
+        // less than 1e-18 of the nearest-neighbour coupling and are dropped.
+        double rc = 1e300;
+        if (c->opt.kbar_cutoff && A.fd[e] > 0.0) rc = (1.0 + std::log(1.0e18) / A.fd[e]) * A.fs[e] * A.r0[e];
+        ft.rcut2[e] = rc >= 1e150 ? 1e300 : rc * rc;
+    }
+    DevBuf<int> cnt;
+    cnt.alloc(A.n);
+    const int blk = 128;
+    k_kbar_count<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.pos.p, ft, cnt.p);
+    PXH_HIP(hipGetLastError());
+    std::vector<int> hc(A.n), rp(A.n + 1, 0);
+    PXH_HIP(hipMemcpyAsync(hc.data(), cnt.p, sizeof(int) * A.n, hipMemcpyDeviceToHost, c->stream));
+    PXH_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < A.n; ++i) rp[i + 1] = rp[i] + hc[i];
+    A.kb_nnz = (size_t)rp[A.n];
+    A.kb_rowptr.upload(rp.data(), rp.size(), c->stream);
+    A.kb_col.alloc(std::max<size_t>(1, A.kb_nnz));
+    A.kb_kij.alloc(std::max<size_t>(1, A.kb_nnz));
+    A.kb_kji.alloc(std::max<size_t>(1, A.kb_nnz));
+    k_kbar_fill<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.pos.p, ft, A.kb_rowptr.p, A.kb_col.p,
+                                                              A.kb_kij.p, A.kb_kji.p);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));
+}
+
+void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    SysParam p = make_sysparam(c, freq, -1);
+    double3 mn = make_double3(0, 0, 0);
+    if (rhs_form == PXH_RHS_POTENTIAL) {
+        double m[3] = {A.h_coord[0], A.h_coord[1], A.h_coord[2]};
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) m[k] = std::min(m[k], A.h_coord[3 * i + k]);
+        mn = make_double3(m[0], m[1], m[2]);
+    }
+    PXH_HIP(hipMemsetAsync(d_rhs, 0, sizeof(double2) * (size_t)nv * 3, c->stream));
+    k_rhs_q<<<n, 64, 0, c->stream>>>(n, A.nt, A.hetero ? 1 : 0, A.kb_rowptr.p, A.kb_col.p, A.kb_kij.p, A.kb_kji.p,
+                                     A.pos.p, p, e0, rhs_form == PXH_RHS_POTENTIAL ? 1 : 0, mn, nv, d_rhs);
+    if (A.model == PXH_WFQFMU) k_rhs_mu<<<(3 * n + 255) / 256, 256, 0, c->stream>>>(n, nv, e0, d_rhs);
+    PXH_HIP(hipGetLastError());
+}
+
+__global__ void k_zero_qq_diag(int n, int nv, double *__restrict__ T) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) T[i + (size_t)nv * i] = 0.0;
+}
+
+// Dense T (n_var x n_var); with zero_qq_diag the q-q diagonal (eta) is left
+// out so that the shared second stage adds it, as for the on-the-fly sweep.
+void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    k_dense_t<<<dim3((n + 127) / 128, n), 128, 0, c->stream>>>(n, A.nt, A.model == PXH_WFQFMU, A.kernel, A.pos.p,
+                                                               A.pair.p, A.d_eta.p, nv, d_T);
+    if (zero_qq_diag) k_zero_qq_diag<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nv, d_T);
+    PXH_HIP(hipGetLastError());
+}
+
+// y = A(omega) x with the stored T (the 'iterative' in-memory method,
+// solve_w_iter_mem algorithm_module.f90:334-410): t = T x, then K-bar and
+// the shifts.  T must have been built by prepare_stored_t.
+void prepare_stored_t(pxh_ctx *c) {
+    Atoms &A = c->atoms;
+    const size_t nv = A.n_var();
+    A.Tstore.alloc(nv * nv);
+    dense_t_atoms(c, A.Tstore.p, true);
+    PXH_HIP(hipStreamSynchronize(c->stream));
+}
+
+void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                        double2 *y, int ld) {
+    if (nsys <= 0) return;
+    Atoms &A = c->atoms;
+    const int nv = A.n_var();
+    c->work_part.alloc((size_t)nsys * nv);
+    c->work_t.alloc((size_t)nsys * nv);
+    k_stored_t<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(nv, A.Tstore.p, nsys, d_sys_list, x, ld,
+                                                                    c->work_part.p);
+    PXH_HIP(hipGetLastError());
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1);
+}
+
+void constant_matrix_atoms(pxh_ctx *c, double *d_m) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    DevBuf<double> T;
+    T.alloc((size_t)nv * nv);
+    dense_t_atoms(c, T.p, false);
+    PXH_HIP(hipMemsetAsync(d_m, 0, sizeof(double) * (size_t)nv * nv, c->stream));
+    if (!A.hetero)
+        k_kbar_dense<<<dim3((nv + 127) / 128, n), 128, 0, c->stream>>>(n, A.kb_rowptr.p, A.kb_col.p, A.kb_kij.p, nv,
+                                                                       T.p, d_m);
+    if (A.model == PXH_WFQFMU)
+        k_copy_mu_rows<<<dim3((nv + 127) / 128, 3 * n), 128, 0, c->stream>>>(n, nv, T.p, d_m);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace pxh

@@ -1,0 +1,174 @@
+// Internal declarations of libpxh (gfx950).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pxh.h"
+
+namespace pxh {
+
+// Error carried from deep helpers up to the C-ABI boundary.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define PXH_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::pxh::Error{PXH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)};  \
+    } while (0)
+
+#define PXH_CHECK(cond, code, msg)                           \
+    do {                                                     \
+        if (!(cond)) throw ::pxh::Error{(code), (msg)};     \
+    } while (0)
+
+// Maximum number of atom types handled by the pair kernels.
+constexpr int kMaxTypes = 8;
+
+// Per type-pair constants of the interaction kernels (index ti + nt*tj).
+struct PairConst {
+    double inv_rqq;    // 1/sqrt(Rq_i^2 + Rq_j^2)           (T_qq, Gaussian)
+    double inv_rqmu;   // 1/sqrt(Rq_i^2 + Rmu_j^2)          (T_qmu(i,j): charge on i, dipole on j)
+    double inv_rmumu;  // 1/sqrt(Rmu_i^2 + Rmu_j^2)         (T_mumu)
+    double eta_bar;    // (eta_i + eta_j)/2                  (Ohno)
+};
+
+// Device buffer with RAII.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        PXH_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) PXH_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// Atomistic system resident in HBM.
+struct Atoms {
+    int model = 0, kernel = 0, n = 0, nt = 0;
+    bool hetero = false;
+    std::vector<double> h_coord;        // 3n, bohr (host copy)
+    std::vector<int> h_type;            // n, 0-based
+    std::vector<double> eta, rq, rmu, aij, tau, dens;
+    std::vector<double> fd, fs, r0;     // nt*nt
+    std::vector<double> h_nbfrac;       // nt*n: nb_counts/nb_total
+    std::vector<PairConst> h_pair;      // nt*nt
+    DevBuf<double4> pos;                // x, y, z, type (as double)
+    DevBuf<float4> posf;                // centred single-precision copy
+    DevBuf<PairConst> pair;
+    DevBuf<double> d_eta;               // per type
+    DevBuf<double> d_nbfrac;            // nt*n (row: type, column: atom)
+    // K-bar in CSR: row i holds j != i with (1-f) above the cutoff.
+    DevBuf<int> kb_rowptr, kb_col;
+    DevBuf<double> kb_kij, kb_kji;      // K_ij = (1-f_ij) A_i / r, K_ji = (1-f_ji) A_j / r
+    size_t kb_nnz = 0;
+    DevBuf<double> Tstore;              // stored T for the in-memory method (n_var^2)
+    int n_var() const { return model == PXH_WFQFMU ? 4 * n : n; }
+};
+
+struct Mesh {
+    int n = 0, variant = 0, green = 0;
+    bool charge_constraint = false;
+    double normal_factor = 1.0, sphere_r = 0.0, eps_solvent = 1.0;
+    std::vector<double> h_verts;        // 9n
+    std::vector<double> h_coord, h_normal, h_t1, h_t2, h_area;
+    DevBuf<double> D;                   // n*n column-major, D (refined)
+    DevBuf<double> Dief;                // n*n, IEF: D (unrefined diagonal) - 2 pi I
+    DevBuf<double> SA;                  // n*n, IEF: S A^-1
+    DevBuf<float> Df, SAf;              // FP32 copies for the mixed path
+    bool built = false;
+};
+
+struct Options {
+    int kbar_cutoff = 1;
+    int batch = 48;          // slots of the batched solver
+    double inner_tol = 1e-5;  // relative tolerance of the FP32 inner GMRES
+    int tile_i = 256;
+    int splits = 0;          // 0 = automatic
+    int kv = 4;              // systems per pair-kernel pass (FP32)
+    int max_outer = 30;
+};
+
+struct Stats {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+}  // namespace pxh
+
+struct pxh_ctx {
+    int device = 0;
+    size_t budget = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pxh::Atoms atoms;
+    pxh::Mesh mesh;
+    int target = 0;                       // PXH_WFQ / PXH_WFQFMU / PXH_BEM
+    std::vector<double> alpha;            // nt x n_freq complex (interleaved)
+    int alpha_nf = 0;
+    std::vector<double> eps;              // n_freq complex
+    int eps_nf = 0;
+    pxh::Options opt;
+    pxh::Stats stats;
+    std::string last_error;
+    pxh::DevBuf<double2> work_part, work_t;  // scratch of the operator products
+    bool use_stored = false;                  // in-memory method: products use the stored T
+};
+
+namespace pxh {
+// kernels_atoms.hip
+void build_atoms_device(pxh_ctx *c);
+void build_kbar(pxh_ctx *c);
+// Per-system description for the batched operator application.
+struct SysParam {
+    double2 zq[kMaxTypes];      // z_q(omega) per type
+    double2 inv_alpha[kMaxTypes];
+    double2 wratio[kMaxTypes * kMaxTypes];  // w_tj / w_ti (alloys), index ti + nt*tj
+};
+// y[s] = A(omega_s) x[s] for the listed systems; x, y are [nsys_total][nv]
+// complex (double2); sys_list holds the systems to process (device).
+void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld);
+void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld);
+void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                        double2 *y, int ld);
+void prepare_stored_t(pxh_ctx *c);
+void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag);
+void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs /* nv x 3 */);
+void constant_matrix_atoms(pxh_ctx *c, double *d_m);
+SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq);
+// kernels_bem.hip
+void build_mesh_device(pxh_ctx *c);
+void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lambda, const double2 *x, double2 *y,
+               int ld, bool fp32);
+void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
+// solver.cpp
+int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
+                   double tol, int kdim, int precision, double *variables, double *polar, double *results,
+                   int32_t *iters, double *true_res);
+}  // namespace pxh

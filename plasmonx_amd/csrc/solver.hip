@@ -1,0 +1,770 @@
+// Per-frequency complex solves on gfx950: the reference's GMRES(m)
+// (src/solvers/gmres_module.f90:30-228) for one system at a time in FP64,
+// and a batched GMRES with FP64 iterative refinement in which many
+// (frequency, field) systems share every operator application.
+//
+// Batched layout in HBM: slot-major.  Slot s owns
+//   X[s], B[s], R[s], W[s]            n_var complex each,
+//   V[s][0..m]                        the Krylov basis (FP64),
+//   H[s] (m+1) x m, g[s] (m+1), cs[s], sn[s] (m)   Hessenberg and rotations.
+// Dots and norms are reduced in a fixed order (one workgroup per dot), so
+// a system's result does not depend on which other systems share the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "pxh_internal.h"
+
+namespace pxh {
+
+namespace {
+
+constexpr int RB = 256;  // threads of the reduction workgroups
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+    __shared__ double sre[RB / 64], sim[RB / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_down(v.x, off, 64);
+        v.y += __shfl_down(v.y, off, 64);
+    }
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+        sre[w] = v.x;
+        sim[w] = v.y;
+    }
+    __syncthreads();
+    double2 r = make_double2(0, 0);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < RB / 64; ++k) {
+            r.x += sre[k];
+            r.y += sim[k];
+        }
+    return r;
+}
+
+// h[s][i] = <V[s][i], w[s]> for i in [i0, jcount[s]) ; one workgroup per (i, slot).
+__global__ __launch_bounds__(RB) void k_dots(int nv, const int *__restrict__ slots, const int *__restrict__ jcount,
+                                             int i0, const double2 *__restrict__ V, size_t ldv_slot,
+                                             const double2 *__restrict__ W, double2 *__restrict__ h, int ldh) {
+    const int i = i0 + blockIdx.x;
+    const int s = slots[blockIdx.y];
+    if (i >= jcount[s]) return;
+    const double2 *v = V + ldv_slot * s + (size_t)nv * i;
+    const double2 *w = W + (size_t)nv * s;
+    double2 acc = make_double2(0, 0);
+    for (int r = threadIdx.x; r < nv; r += RB) {
+        const double2 p = cconjmul(v[r], w[r]);
+        acc.x += p.x;
+        acc.y += p.y;
+    }
+    acc = block_sum2(acc);
+    if (threadIdx.x == 0) h[(size_t)s * ldh + i] = acc;
+}
+
+// W[s] -= sum_{i < jcount[s]} V[s][i] h[s][i]  (and optionally hacc[s][i] += h[s][i]).
+__global__ void k_gs_update(int nv, const int *__restrict__ slots, const int *__restrict__ jcount,
+                            const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ h, int ldh,
+                            double2 *__restrict__ W) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    const int jc = jcount[s];
+    const double2 *v = V + ldv_slot * s + r;
+    const double2 *hs = h + (size_t)s * ldh;
+    double2 acc = make_double2(0, 0);
+    for (int i = 0; i < jc; ++i) {
+        const double2 p = cmul(v[(size_t)nv * i], hs[i]);
+        acc.x += p.x;
+        acc.y += p.y;
+    }
+    double2 &wv = W[(size_t)nv * s + r];
+    wv.x -= acc.x;
+    wv.y -= acc.y;
+}
+
+// Single-vector MGS step: W[s] -= h V[s][i] with h = hbuf[s][i].
+__global__ void k_axpy_col(int nv, const int *__restrict__ slots, int i, const double2 *__restrict__ V,
+                           size_t ldv_slot, const double2 *__restrict__ h, int ldh, double2 *__restrict__ W) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    const double2 hv = h[(size_t)s * ldh + i];
+    const double2 p = cmul(V[ldv_slot * s + (size_t)nv * i + r], hv);
+    double2 &wv = W[(size_t)nv * s + r];
+    wv.x -= p.x;
+    wv.y -= p.y;
+}
+
+// nrm[s] = ||W[s]||_2 (sum of squares in a fixed order).
+__global__ __launch_bounds__(RB) void k_norm(int nv, const int *__restrict__ slots, const double2 *__restrict__ W,
+                                             size_t ldw, double *__restrict__ nrm) {
+    const int s = slots[blockIdx.x];
+    const double2 *w = W + ldw * s;
+    double2 acc = make_double2(0, 0);
+    for (int r = threadIdx.x; r < nv; r += RB) acc.x += w[r].x * w[r].x + w[r].y * w[r].y;
+    acc = block_sum2(acc);
+    if (threadIdx.x == 0) nrm[s] = sqrt(acc.x);
+}
+
+// V[s][j] = W[s] / nrm[s]   (nothing when nrm == 0, as gmres_module.f90:148-151)
+__global__ void k_scale_into(int nv, const int *__restrict__ slots, const int *__restrict__ jcol,
+                             const double2 *__restrict__ W, size_t ldw, const double *__restrict__ nrm,
+                             double2 *__restrict__ V, size_t ldv_slot) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    const double d = nrm[s];
+    if (d == 0.0) return;
+    const double2 w = W[ldw * s + r];
+    V[ldv_slot * s + (size_t)nv * jcol[s] + r] = make_double2(w.x / d, w.y / d);
+}
+
+// R[s] = B[s] - W[s]
+__global__ void k_residual(int nv, const int *__restrict__ slots, const double2 *__restrict__ Bv,
+                           const double2 *__restrict__ W, double2 *__restrict__ R) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    const size_t o = (size_t)nv * s + r;
+    R[o] = make_double2(Bv[o].x - W[o].x, Bv[o].y - W[o].y);
+}
+
+// BLAS zrotg (reference algorithm) and zrot on one pair.
+__device__ void zrotg_d(double2 &ca, double2 cb, double &c, double2 &s) {
+    const double aa = hypot(ca.x, ca.y);
+    if (aa == 0.0) {
+        c = 0.0;
+        s = make_double2(1.0, 0.0);
+        ca = cb;
+        return;
+    }
+    const double ab = hypot(cb.x, cb.y);
+    const double scale = aa + ab;
+    const double ra = aa / scale, rb = ab / scale;
+    const double norm = scale * sqrt(ra * ra + rb * rb);
+    const double2 alpha = make_double2(ca.x / aa, ca.y / aa);
+    c = aa / norm;
+    const double2 cbc = make_double2(cb.x, -cb.y);
+    const double2 t = cmul(alpha, cbc);
+    s = make_double2(t.x / norm, t.y / norm);
+    ca = make_double2(alpha.x * norm, alpha.y * norm);
+}
+__device__ void zrot_d(double2 &x, double2 &y, double c, double2 s) {
+    const double2 sy = cmul(s, y);
+    const double2 sc = make_double2(s.x, -s.y);
+    const double2 scx = cmul(sc, x);
+    const double2 t = make_double2(c * x.x + sy.x, c * x.y + sy.y);
+    y = make_double2(c * y.x - scx.x, c * y.y - scx.y);
+    x = t;
+}
+
+// Givens update of column j of H[s] (gmres_module.f90:153-167), one thread per slot.
+// hcol holds the Gram-Schmidt coefficients h_0..h_j, hn the new norm.
+__global__ void k_givens(int nsl, const int *__restrict__ slots, const int *__restrict__ jcol,
+                         const double2 *__restrict__ hcol, int ldh, const double *__restrict__ hn,
+                         double2 *__restrict__ H, int ldH, size_t ldH_slot, double2 *__restrict__ g, int ldg,
+                         double *__restrict__ cs, double2 *__restrict__ sn, int ldcs, double *__restrict__ resid) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsl) return;
+    const int s = slots[k];
+    const int j = jcol[s];
+    double2 *Hc = H + ldH_slot * s + (size_t)ldH * j;
+    const double2 *hs = hcol + (size_t)s * ldh;
+    for (int i = 0; i <= j; ++i) Hc[i] = hs[i];
+    Hc[j + 1] = make_double2(hn[s], 0.0);
+    double *css = cs + (size_t)s * ldcs;
+    double2 *sns = sn + (size_t)s * ldcs;
+    for (int i = 0; i < j; ++i) zrot_d(Hc[i], Hc[i + 1], css[i], sns[i]);
+    double2 a = Hc[j];
+    zrotg_d(a, Hc[j + 1], css[j], sns[j]);
+    double2 *gs = g + (size_t)s * ldg;
+    zrot_d(gs[j], gs[j + 1], css[j], sns[j]);
+    zrot_d(Hc[j], Hc[j + 1], css[j], sns[j]);
+    Hc[j + 1] = make_double2(0, 0);
+    resid[s] = hypot(gs[j + 1].x, gs[j + 1].y);
+}
+
+// Back substitution H(0:j,0:j) y = g(0:j) for each slot (one workgroup per slot).
+__global__ __launch_bounds__(64) void k_trsv(const int *__restrict__ slots, const int *__restrict__ jlen,
+                                             const double2 *__restrict__ H, int ldH, size_t ldH_slot,
+                                             const double2 *__restrict__ g, int ldg, double2 *__restrict__ y,
+                                             int ldy) {
+    const int s = slots[blockIdx.x];
+    const int n = jlen[s];
+    const double2 *Hs = H + ldH_slot * s;
+    double2 *ys = y + (size_t)s * ldy;
+    const double2 *gs = g + (size_t)s * ldg;
+    for (int i = threadIdx.x; i < n; i += 64) ys[i] = gs[i];
+    __syncthreads();
+    for (int i = n - 1; i >= 0; --i) {
+        if (threadIdx.x == 0) {
+            const double2 hii = Hs[i + (size_t)ldH * i];
+            const double den = hii.x * hii.x + hii.y * hii.y;
+            const double2 v = ys[i];
+            ys[i] = make_double2((v.x * hii.x + v.y * hii.y) / den, (v.y * hii.x - v.x * hii.y) / den);
+        }
+        __syncthreads();
+        const double2 yi = ys[i];
+        for (int r = threadIdx.x; r < i; r += 64) {
+            const double2 p = cmul(Hs[r + (size_t)ldH * i], yi);
+            ys[r].x -= p.x;
+            ys[r].y -= p.y;
+        }
+        __syncthreads();
+    }
+}
+
+// X[s] (+)= V[s][0:jlen] y[s]
+__global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__restrict__ jlen,
+                           const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ y, int ldy,
+                           double2 *__restrict__ X, int accumulate) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    const int n = jlen[s];
+    const double2 *v = V + ldv_slot * s + r;
+    const double2 *ys = y + (size_t)s * ldy;
+    double2 acc = make_double2(0, 0);
+    for (int i = 0; i < n; ++i) {
+        const double2 p = cmul(v[(size_t)nv * i], ys[i]);
+        acc.x += p.x;
+        acc.y += p.y;
+    }
+    double2 &xv = X[(size_t)nv * s + r];
+    if (accumulate) {
+        xv.x += acc.x;
+        xv.y += acc.y;
+    } else {
+        xv = acc;
+    }
+}
+
+// Start a Krylov cycle: V[s][0] = R[s] / beta[s], g[s] = (beta, 0, ...).
+__global__ void k_start_cycle(int nv, const int *__restrict__ slots, const double2 *__restrict__ R,
+                              const double *__restrict__ beta, double2 *__restrict__ V, size_t ldv_slot,
+                              double2 *__restrict__ g, int ldg) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    const double b = beta[s];
+    if (r < ldg) g[(size_t)s * ldg + r] = make_double2(r == 0 ? b : 0.0, 0.0);
+    if (r >= nv) return;
+    const double2 v = R[(size_t)nv * s + r];
+    V[ldv_slot * s + r] = make_double2(v.x / b, v.y / b);
+}
+
+// Copy a right-hand side (column k of rhs, nv x 3) into slot s of B and R, zero X.
+__global__ void k_load_rhs(int nv, int s, const double2 *__restrict__ rhs, int k, double2 *__restrict__ Bv,
+                           double2 *__restrict__ R, double2 *__restrict__ X) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nv) return;
+    const double2 v = rhs[(size_t)nv * k + r];
+    const size_t o = (size_t)nv * s + r;
+    Bv[o] = v;
+    R[o] = v;
+    X[o] = make_double2(0, 0);
+}
+
+// ----------------------------------------------------------------- driver
+struct System {
+    int ifreq, pol;
+};
+
+struct Batched {
+    pxh_ctx *c;
+    int nv, m, B;
+    bool mixed, plain;
+    DevBuf<double2> X, Bv, R, W, V, H, g, hbuf, hacc, y, sn;
+    DevBuf<double> cs, nrm, resid;
+    DevBuf<int> slots_d, jcount_d, jp1_d, jl_d;
+    DevBuf<double2> stage;
+    DevBuf<SysParam> par;
+    std::vector<int> h_list, h_jp1, h_jl, h_jc;
+    std::vector<SysParam> h_par;
+    std::vector<double2> h_lam;
+    DevBuf<double2> lam;
+    std::vector<int> jcount;
+    size_t ldv_slot = 0, ldH_slot = 0;
+    int ldH = 0, ldg = 0, ldh = 0;
+};
+
+// The host copies stay alive in S until the next synchronisation point.
+void upload_list(Batched &S, const std::vector<int> &list) {
+    S.h_list = list;
+    S.slots_d.alloc(std::max<size_t>(1, (size_t)S.B));
+    PXH_HIP(hipMemcpyAsync(S.slots_d.p, S.h_list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
+                           S.c->stream));
+    PXH_HIP(hipStreamSynchronize(S.c->stream));
+}
+
+// hbuf[s][i] += hacc[s][i] for i <= j_s
+__global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ jcount, double2 *__restrict__ h,
+                        const double2 *__restrict__ h2, int ldh) {
+    const int s = slots[blockIdx.x];
+    for (int i = threadIdx.x; i < jcount[s]; i += blockDim.x) {
+        h[(size_t)s * ldh + i].x += h2[(size_t)s * ldh + i].x;
+        h[(size_t)s * ldh + i].y += h2[(size_t)s * ldh + i].y;
+    }
+}
+
+}  // namespace
+
+// Operator application for slots in `list` (device copy in S.slots_d).
+static void apply_op(Batched &S, int nsl, bool fp64, const double2 *x, double2 *y) {
+    pxh_ctx *c = S.c;
+    if (c->target == PXH_BEM) {
+        apply_bem(c, nsl, S.slots_d.p, S.lam.p, x, y, S.nv, !fp64);
+    } else if (c->use_stored) {
+        apply_atoms_stored(c, nsl, S.slots_d.p, S.par.p, x, y, S.nv);
+    } else if (fp64) {
+        apply_atoms_fp64(c, nsl, S.slots_d.p, S.par.p, x, y, S.nv);
+    } else {
+        apply_atoms_fp32(c, nsl, S.slots_d.p, S.par.p, x, y, S.nv);
+    }
+}
+
+// Solve all (ifreq, pol) systems.  rhs_all: device, [n_freq][nv x 3].
+static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
+                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
+                       double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
+                       std::vector<double> &tres_out) {
+    Batched S;
+    S.c = c;
+    S.nv = nv;
+    S.mixed = precision == PXH_MIXED;
+    S.plain = plain;
+    S.B = plain ? 1 : std::max(1, std::min<int>(c->opt.batch, (int)systems.size()));
+    S.m = std::max(1, std::min(kdim, nv));
+    const int m = S.m, B = S.B;
+    hipStream_t st = c->stream;
+    S.ldv_slot = (size_t)nv * (m + 1);
+    S.ldH = m + 2;
+    S.ldH_slot = (size_t)S.ldH * (m + 1);
+    S.ldg = m + 2;
+    S.ldh = m + 2;
+    S.X.alloc((size_t)nv * B);
+    S.Bv.alloc((size_t)nv * B);
+    S.R.alloc((size_t)nv * B);
+    S.W.alloc((size_t)nv * B);
+    S.V.alloc(S.ldv_slot * B);
+    S.H.alloc(S.ldH_slot * B);
+    S.g.alloc((size_t)S.ldg * B);
+    S.hbuf.alloc((size_t)S.ldh * B);
+    S.hacc.alloc((size_t)S.ldh * B);
+    S.y.alloc((size_t)S.ldh * B);
+    S.cs.alloc((size_t)S.ldg * B);
+    S.sn.alloc((size_t)S.ldg * B);
+    S.nrm.alloc(B);
+    S.resid.alloc(B);
+    S.jcount_d.alloc(B);
+    S.jp1_d.alloc(B);
+    S.jl_d.alloc(B);
+    S.stage.alloc((size_t)nv * B);
+    S.h_jp1.assign(B, 0);
+    S.h_jl.assign(B, 0);
+    S.h_jc.assign(B, 0);
+    if (c->target == PXH_BEM) {
+        S.lam.alloc(B);
+        S.h_lam.assign(B, make_double2(0, 0));
+    } else {
+        S.par.alloc(B);
+        S.h_par.assign(B, SysParam{});
+    }
+    S.jcount.assign(B, 0);
+
+    enum Phase { FREE, INNER, NEED_RES };
+    std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0);
+    std::vector<double> beta(B, 0.0), target(B, 0.0);
+    std::vector<int> total_it(B, 0);
+    size_t next = 0, done = 0;
+    iters_out.assign(systems.size(), 0);
+    tres_out.assign(systems.size(), 0.0);
+    const double inner_rel = S.mixed ? c->opt.inner_tol : 0.0;
+    auto grid_rows = [&](int nsl) { return dim3((nv + 255) / 256, nsl); };
+    std::vector<double> hbeta(B);
+    int status = 0;
+
+    auto admit = [&](int s) {
+        const System &sy = systems[next];
+        sys_of[s] = (int)next++;
+        if (c->target == PXH_BEM) {
+            const double *e = c->eps.data() + 2 * (size_t)sy.ifreq;
+            // lambda(omega) = 2 pi (eps_s + eps)/(eps_s - eps)  (bem_module.f90:898-906)
+            const double es = c->mesh.eps_solvent;
+            const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
+            const double den = dr * dr + di * di;
+            S.h_lam[s] = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
+            PXH_HIP(hipMemcpyAsync(S.lam.p + s, &S.h_lam[s], sizeof(double2), hipMemcpyHostToDevice, st));
+        } else {
+            S.h_par[s] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
+            PXH_HIP(hipMemcpyAsync(S.par.p + s, &S.h_par[s], sizeof(SysParam), hipMemcpyHostToDevice, st));
+        }
+        k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, s, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
+                                                    S.Bv.p, S.R.p, S.X.p);
+        outer[s] = 0;
+        total_it[s] = 0;
+        phase[s] = NEED_RES;  // the residual of x = 0 is b: computed below without a product
+    };
+
+    std::vector<int> res_first;  // slots whose residual is b (x = 0)
+    while (done < systems.size()) {
+        // fill free slots
+        res_first.clear();
+        for (int s = 0; s < B && next < systems.size(); ++s)
+            if (phase[s] == FREE) {
+                admit(s);
+                res_first.push_back(s);
+            }
+        // ----- residual checks (FP64)
+        std::vector<int> need;
+        for (int s = 0; s < B; ++s)
+            if (phase[s] == NEED_RES) need.push_back(s);
+        if (!need.empty()) {
+            std::vector<int> prod;  // slots that need A x (x != 0)
+            for (int s : need)
+                if (std::find(res_first.begin(), res_first.end(), s) == res_first.end()) prod.push_back(s);
+            if (!prod.empty()) {
+                upload_list(S, prod);
+                apply_op(S, (int)prod.size(), true, S.X.p, S.W.p);
+                k_residual<<<grid_rows((int)prod.size()), 256, 0, st>>>(nv, S.slots_d.p, S.Bv.p, S.W.p, S.R.p);
+                c->stats.v[2] += 1;
+                c->stats.v[0] += prod.size();
+            }
+            upload_list(S, need);
+            k_norm<<<(int)need.size(), RB, 0, st>>>(nv, S.slots_d.p, S.R.p, (size_t)nv, S.nrm.p);
+            PXH_HIP(hipMemcpyAsync(hbeta.data(), S.nrm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipStreamSynchronize(st));
+            std::vector<int> restart;
+            for (int s : need) {
+                const bool first = std::find(res_first.begin(), res_first.end(), s) != res_first.end();
+                const double rn = hbeta[s];
+                const int id = sys_of[s];
+                if (rn == 0.0 || (!first && rn < tol)) {
+                    // converged (or zero RHS: the solution is zero, gmres_module.f90:84-85)
+                    tres_out[id] = rn;
+                    iters_out[id] = total_it[s];
+                    const System &sy = systems[id];
+                    PXH_HIP(hipMemcpyAsync(d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
+                                           S.X.p + (size_t)nv * s, sizeof(double2) * nv, hipMemcpyDeviceToDevice,
+                                           st));
+                    phase[s] = FREE;
+                    ++done;
+                    continue;
+                }
+                if (outer[s] >= c->opt.max_outer) {
+                    tres_out[id] = rn;
+                    iters_out[id] = total_it[s];
+                    status = PXH_ENOCONV;
+                    phase[s] = FREE;
+                    ++done;
+                    continue;
+                }
+                beta[s] = rn;
+                target[s] = plain ? tol : std::max(0.5 * tol, inner_rel * rn);
+                restart.push_back(s);
+            }
+            if (!restart.empty()) {
+                upload_list(S, restart);
+                k_start_cycle<<<dim3((std::max(nv, S.ldg) + 255) / 256, (int)restart.size()), 256, 0, st>>>(
+                    nv, S.slots_d.p, S.R.p, S.nrm.p, S.V.p, S.ldv_slot, S.g.p, S.ldg);
+                for (int s : restart) {
+                    phase[s] = INNER;
+                    S.jcount[s] = 0;
+                    outer[s] += 1;
+                    c->stats.v[7] += 1;
+                }
+            }
+        }
+        // ----- one Arnoldi step for every slot in INNER
+        std::vector<int> act;
+        for (int s = 0; s < B; ++s)
+            if (phase[s] == INNER) act.push_back(s);
+        if (act.empty()) continue;
+        const int na = (int)act.size();
+        int jmax = 0;
+        for (int s : act) jmax = std::max(jmax, S.jcount[s]);
+        upload_list(S, act);
+        // W = A V_j: V_j of each slot is staged at stage + nv * slot (the operator's layout)
+        S.h_jc = S.jcount;
+        PXH_HIP(hipMemcpyAsync(S.jcount_d.p, S.h_jc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+        for (int s : act)
+            PXH_HIP(hipMemcpyAsync(S.stage.p + (size_t)nv * s, S.V.p + S.ldv_slot * s + (size_t)nv * S.jcount[s],
+                                   sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
+        apply_op(S, na, !S.mixed, S.stage.p, S.W.p);
+        if (S.mixed)
+            c->stats.v[1] += 1;
+        else
+            c->stats.v[2] += 1;
+        c->stats.v[0] += na;
+        // jcount_d holds j; the GS uses j+1 columns
+        for (int s : act) S.h_jp1[s] = S.jcount[s] + 1;
+        PXH_HIP(hipMemcpyAsync(S.jp1_d.p, S.h_jp1.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+        int *jp1_dp = S.jp1_d.p;
+        const bool mgs = plain && (jmax + 1 < 25);
+        if (mgs) {
+            // modified Gram-Schmidt (gmres_module.f90:123-127); plain path has one slot
+            for (int i = 0; i <= jmax; ++i) {
+                k_dots<<<dim3(1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, i, S.V.p, S.ldv_slot, S.W.p,
+                                                   S.hbuf.p, S.ldh);
+                k_axpy_col<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, i, S.V.p, S.ldv_slot, S.hbuf.p, S.ldh,
+                                                         S.W.p);
+            }
+        } else {
+            // classical Gram-Schmidt with one re-orthogonalisation (:128-142)
+            k_dots<<<dim3(jmax + 1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, 0, S.V.p, S.ldv_slot, S.W.p,
+                                                      S.hbuf.p, S.ldh);
+            k_gs_update<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.V.p, S.ldv_slot, S.hbuf.p,
+                                                      S.ldh, S.W.p);
+            k_dots<<<dim3(jmax + 1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, 0, S.V.p, S.ldv_slot, S.W.p,
+                                                      S.hacc.p, S.ldh);
+            k_gs_update<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.V.p, S.ldv_slot, S.hacc.p,
+                                                      S.ldh, S.W.p);
+            k_add_h<<<na, 64, 0, st>>>(S.slots_d.p, jp1_dp, S.hbuf.p, S.hacc.p, S.ldh);
+        }
+        k_norm<<<na, RB, 0, st>>>(nv, S.slots_d.p, S.W.p, (size_t)nv, S.nrm.p);
+        k_scale_into<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p, S.V.p,
+                                                   S.ldv_slot);
+        k_givens<<<(na + 63) / 64, 64, 0, st>>>(na, S.slots_d.p, S.jcount_d.p, S.hbuf.p, S.ldh, S.nrm.p, S.H.p,
+                                                S.ldH, S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg, S.resid.p);
+        PXH_HIP(hipGetLastError());
+        std::vector<double> hres(B);
+        PXH_HIP(hipMemcpyAsync(hres.data(), S.resid.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipStreamSynchronize(st));
+        std::vector<int> fin;
+        for (int s : act) {
+            S.jcount[s] += 1;
+            total_it[s] += 1;
+            if (hres[s] < target[s] || S.jcount[s] >= m) fin.push_back(s);
+        }
+        if (!fin.empty()) {
+            upload_list(S, fin);
+            for (int s : fin) S.h_jl[s] = S.jcount[s];
+            PXH_HIP(hipMemcpyAsync(S.jl_d.p, S.h_jl.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+            int *jl_dp = S.jl_d.p;
+            k_trsv<<<(int)fin.size(), 64, 0, st>>>(S.slots_d.p, jl_dp, S.H.p, S.ldH, S.ldH_slot, S.g.p, S.ldg,
+                                                   S.y.p, S.ldh);
+            k_update_x<<<grid_rows((int)fin.size()), 256, 0, st>>>(nv, S.slots_d.p, jl_dp, S.V.p, S.ldv_slot,
+                                                                  S.y.p, S.ldh, S.X.p, 1);
+            PXH_HIP(hipGetLastError());
+            for (int s : fin) phase[s] = NEED_RES;
+        }
+    }
+    PXH_HIP(hipStreamSynchronize(st));
+    return status;
+}
+
+// polar[a + 3 b] = sum_i var_b(i) x_a(i) / E0 (+ dipoles); BEM: area-weighted for DPCM.
+__global__ void k_polar_partial(int n, int nv, int fmu, int bem_dpcm, const double4 *__restrict__ pos,
+                                const double *__restrict__ area, const double2 *__restrict__ var,
+                                double2 *__restrict__ out /* [blocks][9] */) {
+    __shared__ double2 sh[9][RB];
+    const int tid = threadIdx.x;
+    double2 acc[9];
+    for (int k = 0; k < 9; ++k) acc[k] = make_double2(0, 0);
+    for (int i = blockIdx.x * RB + tid; i < n; i += gridDim.x * RB) {
+        const double4 p = pos[i];
+        const double w = bem_dpcm ? area[i] : 1.0;
+        for (int b = 0; b < 3; ++b) {
+            const double2 v = var[(size_t)nv * b + i];
+            acc[0 + 3 * b].x += w * v.x * p.x;
+            acc[0 + 3 * b].y += w * v.y * p.x;
+            acc[1 + 3 * b].x += w * v.x * p.y;
+            acc[1 + 3 * b].y += w * v.y * p.y;
+            acc[2 + 3 * b].x += w * v.x * p.z;
+            acc[2 + 3 * b].y += w * v.y * p.z;
+            if (fmu)
+                for (int a = 0; a < 3; ++a) {
+                    const double2 mu = var[(size_t)nv * b + n + 3 * i + a];
+                    acc[a + 3 * b].x += mu.x;
+                    acc[a + 3 * b].y += mu.y;
+                }
+        }
+    }
+    for (int k = 0; k < 9; ++k) sh[k][tid] = acc[k];
+    __syncthreads();
+    for (int off = RB / 2; off > 0; off >>= 1) {
+        if (tid < off)
+            for (int k = 0; k < 9; ++k) {
+                sh[k][tid].x += sh[k][tid + off].x;
+                sh[k][tid].y += sh[k][tid + off].y;
+            }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int k = 0; k < 9; ++k) out[(size_t)blockIdx.x * 9 + k] = sh[k][0];
+}
+
+}  // namespace pxh
+
+namespace pxh {
+
+void results_row(double freq, const double *pol /* 18: complex 3x3 */, double *r);
+
+// Host: the 20 result columns (wfq_module.f90:561-581, target_module.f90:1989-2054).
+void results_row(double freq, const double *pol, double *r) {
+    const double light = 137.035227, pi = M_PI;
+    auto re = [&](int a) { return pol[2 * a]; };
+    auto im = [&](int a) { return pol[2 * a + 1]; };
+    r[0] = (re(0) + re(4) + re(8)) / 3.0;
+    r[1] = (im(0) + im(4) + im(8)) / 3.0;
+    r[2] = re(0);
+    r[3] = re(4);
+    r[4] = re(8);
+    r[5] = im(0);
+    r[6] = im(4);
+    r[7] = im(8);
+    const double m_iso = r[0] * r[0] + r[1] * r[1], m_x = r[2] * r[2] + r[5] * r[5], m_y = r[3] * r[3] + r[6] * r[6],
+                 m_z = r[4] * r[4] + r[7] * r[7];
+    r[8] = 4.0 * pi * freq * r[1] / light;
+    r[9] = 4.0 * pi * freq * r[5] / light;
+    r[10] = 4.0 * pi * freq * r[6] / light;
+    r[11] = 4.0 * pi * freq * r[7] / light;
+    const double c4 = 3.0 * light * light * light * light, f4 = 8.0 * pi * freq * freq * freq * freq;
+    r[12] = f4 * m_iso / c4;
+    r[13] = f4 * m_x / c4;
+    r[14] = f4 * m_y / c4;
+    r[15] = f4 * m_z / c4;
+    for (int k = 0; k < 4; ++k) r[16 + k] = r[8 + k] + r[12 + k];
+}
+
+int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_stride,
+                          double2 *d_sol, int pol_mask);
+
+int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
+                   double tol, int kdim, int precision, double *variables, double *polar, double *results,
+                   int32_t *iters, double *true_res) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (double &v : c->stats.v) v = 0.0;
+    hipStream_t st = c->stream;
+    const bool bem = c->target == PXH_BEM;
+    const int nv = bem ? c->mesh.n : c->atoms.n_var();
+    PXH_CHECK(nv > 0, PXH_EINVAL, "no target set (pxh_set_atoms / pxh_set_mesh)");
+    if (bem) PXH_CHECK(c->eps_nf >= n_freq, PXH_EINVAL, "permittivity table shorter than the frequency list");
+    if (!bem && c->atoms.model == PXH_WFQFMU)
+        PXH_CHECK(c->alpha_nf >= n_freq, PXH_EINVAL, "alpha(omega) table shorter than the frequency list");
+    const size_t stride = (size_t)nv * 3;
+    DevBuf<double2> rhs_all, sol;
+    rhs_all.alloc(stride * n_freq);
+    sol.alloc(stride * n_freq);
+    PXH_HIP(hipMemsetAsync(sol.p, 0, sizeof(double2) * stride * n_freq, st));
+    // right-hand sides per frequency (frequency dependent only for alloys)
+    if (bem) {
+        dynamic_rhs_bem(c, e0, rhs_all.p);
+        for (int f = 1; f < n_freq; ++f)
+            PXH_HIP(hipMemcpyAsync(rhs_all.p + stride * f, rhs_all.p, sizeof(double2) * stride,
+                                   hipMemcpyDeviceToDevice, st));
+    } else {
+        for (int f = 0; f < n_freq; ++f) {
+            if (f > 0 && !c->atoms.hetero) {
+                PXH_HIP(hipMemcpyAsync(rhs_all.p + stride * f, rhs_all.p, sizeof(double2) * stride,
+                                       hipMemcpyDeviceToDevice, st));
+                continue;
+            }
+            dynamic_rhs_atoms(c, freq[f], e0, rhs_form, rhs_all.p + stride * f);
+        }
+    }
+    std::vector<int> it_sys;
+    std::vector<double> tr_sys;
+    std::vector<System> systems;
+    for (int f = 0; f < n_freq; ++f)
+        for (int k = 0; k < 3; ++k)
+            if (pol_mask & (1 << k)) systems.push_back({f, k});
+    int status = 0;
+    if (method == PXH_INVERSION) {
+        status = solve_dense_inversion(c, n_freq, freq, rhs_all.p, (int)stride, sol.p, pol_mask);
+        it_sys.assign(systems.size(), 0);
+        tr_sys.assign(systems.size(), 0.0);
+    } else {
+        // 'iterative' keeps the frequency-independent T (atoms) or D (BEM) in
+        // HBM; 'iterative on the fly' recomputes T in every product.
+        c->use_stored = method == PXH_ITERATIVE && !bem;
+        if (c->use_stored) prepare_stored_t(c);
+        const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
+        try {
+            status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, c->use_stored ? PXH_FP64
+                                                                                                        : precision,
+                                 plain, sol.p, it_sys, tr_sys);
+        } catch (...) {
+            c->use_stored = false;
+            c->atoms.Tstore.release();
+            throw;
+        }
+        c->use_stored = false;
+        c->atoms.Tstore.release();
+    }
+    // polarizabilities and results
+    const int nb = 64;
+    DevBuf<double2> part;
+    part.alloc((size_t)nb * 9 * n_freq);
+    const double4 *posp = nullptr;
+    const double *area = nullptr;
+    DevBuf<double4> bpos;
+    DevBuf<double> barea;
+    if (bem) {
+        std::vector<double4> hp(nv);
+        for (int i = 0; i < nv; ++i)
+            hp[i] = make_double4(c->mesh.h_coord[3 * i], c->mesh.h_coord[3 * i + 1], c->mesh.h_coord[3 * i + 2], 0.0);
+        bpos.upload(hp.data(), hp.size(), st);
+        barea.upload(c->mesh.h_area.data(), nv, st);
+        posp = bpos.p;
+        area = barea.p;
+    } else {
+        posp = c->atoms.pos.p;
+    }
+    const int n_sites = bem ? nv : c->atoms.n;
+    const int fmu = (!bem && c->atoms.model == PXH_WFQFMU) ? 1 : 0;
+    const int dpcm = (bem && c->mesh.variant == PXH_DPCM) ? 1 : 0;
+    for (int f = 0; f < n_freq; ++f)
+        k_polar_partial<<<nb, RB, 0, st>>>(n_sites, nv, fmu, dpcm, posp, area, sol.p + stride * f,
+                                           part.p + (size_t)nb * 9 * f);
+    PXH_HIP(hipGetLastError());
+    std::vector<double2> hp((size_t)nb * 9 * n_freq);
+    PXH_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, st));
+    if (variables)
+        PXH_HIP(hipMemcpyAsync(variables, sol.p, sizeof(double2) * stride * n_freq, hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipStreamSynchronize(st));
+    for (int f = 0; f < n_freq; ++f) {
+        double pol[18];
+        for (int k = 0; k < 9; ++k) {
+            double re = 0, im = 0;
+            for (int b = 0; b < nb; ++b) {
+                re += hp[((size_t)nb * f + b) * 9 + k].x;
+                im += hp[((size_t)nb * f + b) * 9 + k].y;
+            }
+            pol[2 * k] = re / e0;
+            pol[2 * k + 1] = im / e0;
+        }
+        if (polar) std::memcpy(polar + 18 * (size_t)f, pol, sizeof(pol));
+        if (results) {
+            double r[20];
+            results_row(freq[f], pol, r);
+            for (int k = 0; k < 20; ++k) results[f + (size_t)n_freq * k] = r[k];
+        }
+    }
+    if (iters || true_res) {
+        for (int f = 0; f < 3 * n_freq; ++f) {
+            if (iters) iters[f] = 0;
+            if (true_res) true_res[f] = 0.0;
+        }
+        for (size_t k = 0; k < systems.size(); ++k) {
+            const int o = systems[k].pol + 3 * systems[k].ifreq;
+            if (iters && k < it_sys.size()) iters[o] = it_sys[k];
+            if (true_res && k < tr_sys.size()) true_res[o] = tr_sys[k];
+        }
+    }
+    c->stats.v[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return status;
+}
+
+}  // namespace pxh
