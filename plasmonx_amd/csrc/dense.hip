@@ -140,31 +140,36 @@ __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, doubl
     double2 *Af = A + lda_f * f;
     const int r0 = k + nb + blockIdx.x * 64, c0 = k + nb + blockIdx.y * 64;
     if (r0 >= n || c0 >= n) return;
-    __shared__ double2 sL[64][NB + 1];
-    __shared__ double2 sU[NB][64 + 1];
+    constexpr int KC = 16;  // k-chunk staged in LDS (34 KB per workgroup)
+    __shared__ double2 sL[64][KC + 1];
+    __shared__ double2 sU[KC][64 + 1];
     const int tid = threadIdx.x;
-    for (int e = tid; e < 64 * nb; e += 256) {
-        const int r = e % 64, q = e / 64;
-        sL[r][q] = (r0 + r < n) ? Af[(r0 + r) + (size_t)n * (k + q)] : make_double2(0, 0);
-    }
-    for (int e = tid; e < nb * 64; e += 256) {
-        const int q = e % nb, c = e / nb;
-        sU[q][c] = (c0 + c < n) ? Af[(k + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
-    }
-    __syncthreads();
     const int tr = tid % 16, tc = tid / 16;
     double2 acc[4][4];
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0, 0);
-    for (int q = 0; q < nb; ++q) {
-        double2 l[4], u[4];
-        for (int a = 0; a < 4; ++a) l[a] = sL[tr + 16 * a][q];
-        for (int b = 0; b < 4; ++b) u[b] = sU[q][tc + 16 * b];
-        for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 4; ++b) {
-                acc[a][b].x += l[a].x * u[b].x - l[a].y * u[b].y;
-                acc[a][b].y += l[a].x * u[b].y + l[a].y * u[b].x;
-            }
+    for (int q0 = 0; q0 < nb; q0 += KC) {
+        const int kc = min(KC, nb - q0);
+        __syncthreads();
+        for (int e = tid; e < 64 * KC; e += 256) {
+            const int r = e % 64, q = e / 64;
+            sL[r][q] = (r0 + r < n && q < kc) ? Af[(r0 + r) + (size_t)n * (k + q0 + q)] : make_double2(0, 0);
+        }
+        for (int e = tid; e < KC * 64; e += 256) {
+            const int q = e % KC, c = e / KC;
+            sU[q][c] = (c0 + c < n && q < kc) ? Af[(k + q0 + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
+        }
+        __syncthreads();
+        for (int q = 0; q < kc; ++q) {
+            double2 l[4], u[4];
+            for (int a = 0; a < 4; ++a) l[a] = sL[tr + 16 * a][q];
+            for (int b = 0; b < 4; ++b) u[b] = sU[q][tc + 16 * b];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    acc[a][b].x += l[a].x * u[b].x - l[a].y * u[b].y;
+                    acc[a][b].y += l[a].x * u[b].y + l[a].y * u[b].x;
+                }
+        }
     }
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) {

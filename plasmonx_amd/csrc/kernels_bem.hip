@@ -122,6 +122,7 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
 
 // per-tessera maximum distance centroid -> polar quadrature point
 __global__ void k_polar_dmax(int n, MeshDev m, const Quad *__restrict__ q, double *__restrict__ dmax) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *v = m.verts + 9 * (size_t)i;
@@ -139,7 +140,13 @@ __global__ void k_polar_dmax(int n, MeshDev m, const Quad *__restrict__ q, doubl
 }
 
 // green_diagonal_accurate (refinement_diagonal_elements :662-737)
+// The normal component sums (c_i - p).n / d^3 over points p in the plane of
+// tessera i: exactly zero in exact arithmetic, so its value is rounding
+// noise amplified by 1/d^3 (d ~ 1e-6 near the centroid).  The reference
+// keeps it in the unrefined IEF diagonal; products are not fused here so
+// that the noise is the one of plain IEEE multiply-then-add evaluation.
 __global__ void k_polar_diag(int n, MeshDev m, const Quad *__restrict__ q, double clamp, double *__restrict__ gdiag) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *v = m.verts + 9 * (size_t)i;
@@ -164,6 +171,7 @@ __global__ void k_polar_diag(int n, MeshDev m, const Quad *__restrict__ q, doubl
 __global__ void k_bem_d(int n, MeshDev m, const Quad *__restrict__ q, int accurate, double sphere_r,
                         const double *__restrict__ gdiag, int ief, double *__restrict__ D, double *__restrict__ Dief,
                         double *__restrict__ SA) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= n) return;

@@ -1,0 +1,170 @@
+"""libpxh (HIP, gfx950) against the FP64 oracle and the reference logs.
+
+Bars: FP64 products and inversions agree with the oracle to 1e-10
+(relative, rounding-order differences only); GMRES results to the solver
+tolerance; mixed precision (FP32 interaction tiles inside FP64 refinement)
+to <= 1e-6 relative error of the polarizability tensor.
+"""
+import numpy as np
+import pytest
+
+from conftest import all_cases, load_fixture, method_id, rel_close
+from oracle import pxo
+from plasmonx_amd import _lib
+from plasmonx_amd import permittivity as pm
+from plasmonx_amd.model import mesh_from_fixture, system_from_fixture
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["wfqfmu_ag_inversion", "wfq_sodium_iterative_on_the_fly", "wfqfmu_hetero_agau_inversion",
+         "wfq_graphene_triangle_inversion", "wfqfmu_hetero_agau_read_1param_inversion"]
+
+
+def setup(ctx, fx, nf=None):
+    s = system_from_fixture(fx)
+    freq = np.array(fx["freq_au"])[:nf]
+    ctx.set_option("kbar_cutoff", 0)
+    ctx.set_atoms(s)
+    aw = None
+    if s.model == "wfqfmu":
+        aw = s.alpha_table(freq)
+        ctx.set_alpha(aw)
+    return s, freq, aw
+
+
+def relerr(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("case", SMALL)
+def test_rhs_matches_oracle(gpu_ctx, case):
+    fx = load_fixture(case)
+    s, freq, aw = setup(gpu_ctx, fx, 1)
+    got = gpu_ctx.dynamic_rhs(s.n_var, freq[0], fx["e0"])
+    ref = pxo.rhs(s, fx["e0"], freq[0])
+    assert relerr(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("case", SMALL)
+def test_matvec_fp64_matches_oracle(gpu_ctx, case):
+    fx = load_fixture(case)
+    s, freq, aw = setup(gpu_ctx, fx, 1)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(2, s.n_var)) + 1j * rng.normal(size=(2, s.n_var))
+    got = gpu_ctx.matvec(freq[0], 0, x)
+    for k in range(2):
+        ref = pxo.matvec_otf(s, freq[0], aw[:, 0] if aw is not None else np.ones(1), x[k])
+        assert relerr(got[k], ref) < 1e-12
+
+
+def test_kbar_cutoff_is_exact_to_rounding(gpu_ctx):
+    fx = load_fixture("wfq_sodium_iterative_on_the_fly")
+    s, freq, aw = setup(gpu_ctx, fx, 1)
+    x = np.random.default_rng(2).normal(size=(1, s.n_var)) + 0j
+    full = gpu_ctx.matvec(freq[0], 0, x)
+    gpu_ctx.set_option("kbar_cutoff", 1)
+    cut = gpu_ctx.matvec(freq[0], 0, x)
+    gpu_ctx.set_option("kbar_cutoff", 0)
+    assert relerr(cut, full) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["wfqfmu_ag_inversion", "wfq_sodium_iterative_on_the_fly"])
+def test_constant_matrix_matches_oracle(gpu_ctx, case):
+    fx = load_fixture(case)
+    s, freq, aw = setup(gpu_ctx, fx, 1)
+    got = gpu_ctx.constant_matrix(s.n_var)
+    ref = pxo.constant_matrix(s)
+    assert relerr(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("case", [c for c in all_cases("atoms") if load_fixture(c)["method"] == "inversion"])
+def test_inversion_matches_oracle_and_log(gpu_ctx, case):
+    fx = load_fixture(case)
+    nf = min(len(fx["freq_au"]), 20)
+    s, freq, aw = setup(gpu_ctx, fx, nf)
+    got = gpu_ctx.solve_spectrum(s.n_var, freq, fx["e0"], method=_lib.INVERSION, want_variables=True)
+    ref = pxo.solve_spectrum(s, freq, fx["e0"], alpha_w=aw, method=0, want_variables=True)
+    assert relerr(got["polar"], ref["polar"]) < 1e-10
+    assert relerr(got["variables"], ref["variables"]) < 1e-9
+    exp = np.array(fx["expected"]["results"])[:nf]
+    assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5, 1e-6).all()
+
+
+@pytest.mark.parametrize("case", ["wfq_sodium_iterative_on_the_fly", "wfqfmu_ag_iterative_on_the_fly",
+                                  "wfqfmu_hetero_agau_iterative_on_the_fly", "wfqfmu_ag_iterative_on_memory",
+                                  "wfqfmu_hetero_agau_iterative_on_memory"])
+def test_gmres_fp64_matches_oracle(gpu_ctx, case):
+    fx = load_fixture(case)
+    nf = min(len(fx["freq_au"]), 4)
+    s, freq, aw = setup(gpu_ctx, fx, nf)
+    tol = fx["tol"] * np.sqrt(s.n_var)
+    gpu_ctx.set_option("batch", 1)
+    got = gpu_ctx.solve_spectrum(s.n_var, freq, fx["e0"], method=method_id(fx["method"]), tol_abs=tol,
+                                 krylov_dim=fx["gmres_dim"])
+    ref = pxo.solve_spectrum(s, freq, fx["e0"], alpha_w=aw, method=0)
+    assert np.all(got["true_residual"] < tol)
+    assert relerr(got["polar"], ref["polar"]) < 1e-4
+    exp = np.array(fx["expected"]["results"])[:nf]
+    assert rel_close(got["results"][:, :12], exp[:, :12], 1e-4, 1e-6).all()
+    # the reference's iteration counts (printed GMRES histories), within one
+    if "gmres" in fx["expected"]:
+        ev = freq * 27.211386245981
+        for h in fx["expected"]["gmres"]:
+            f = int(np.argmin(np.abs(ev - h["ev"])))
+            if abs(ev[f] - h["ev"]) < 1e-6:
+                assert abs(got["iters"][f, "XYZ".index(h["pol"])] - len(h["hist"])) <= 1
+
+
+@pytest.mark.parametrize("case", ["wfqfmu_ag_inversion", "wfq_sodium_inversion", "wfqfmu_hetero_agau_inversion"])
+def test_mixed_batched_within_1e6(gpu_ctx, case):
+    fx = load_fixture(case)
+    nf = min(len(fx["freq_au"]), 24)
+    s, freq, aw = setup(gpu_ctx, fx, nf)
+    tol = 1e-12 * np.sqrt(s.n_var)
+    gpu_ctx.set_option("batch", 16)
+    got = gpu_ctx.solve_spectrum(s.n_var, freq, fx["e0"], method=_lib.ITERATIVE_OTF, tol_abs=tol, krylov_dim=200,
+                                 precision=_lib.MIXED)
+    gpu_ctx.set_option("batch", 1)
+    ref = pxo.solve_spectrum(s, freq, fx["e0"], alpha_w=aw, method=0)
+    assert np.all(got["true_residual"] < tol)
+    for f in range(nf):
+        err = np.linalg.norm(got["polar"][f] - ref["polar"][f]) / np.linalg.norm(ref["polar"][f])
+        assert err < 1e-6, (f, err)
+
+
+def test_batched_result_independent_of_batch(gpu_ctx):
+    fx = load_fixture("wfqfmu_ag_inversion")
+    freq = np.linspace(2.5, 3.5, 9) / 27.211386245981
+    s = system_from_fixture(fx)
+    gpu_ctx.set_option("kbar_cutoff", 0)
+    gpu_ctx.set_atoms(s)
+    gpu_ctx.set_alpha(pm.silver_etchegoin(freq, polar=True)[None, :])
+    tol = 1e-11 * np.sqrt(s.n_var)
+    out = []
+    for b in (1, 3, 27):
+        gpu_ctx.set_option("batch", b)
+        out.append(gpu_ctx.solve_spectrum(s.n_var, freq, fx["e0"], method=_lib.ITERATIVE_OTF, tol_abs=tol,
+                                          krylov_dim=200, precision=_lib.MIXED)["polar"])
+    gpu_ctx.set_option("batch", 1)
+    assert relerr(out[1], out[0]) < 1e-8
+    assert relerr(out[2], out[0]) < 1e-8
+
+
+@pytest.mark.parametrize("case", all_cases("bem"))
+def test_bem_matches_oracle_and_log(gpu_ctx, case):
+    fx = load_fixture(case)
+    m = mesh_from_fixture(fx)
+    freq = np.array(fx["freq_au"])
+    eps = np.array([complex(a, b) for a, b in fx["eps_w"]]) if "eps_w" in fx else pm.silver_etchegoin(freq)
+    gpu_ctx.set_mesh(m)
+    gpu_ctx.set_permittivity(eps)
+    D, Di, SA = pxo.bem_d(m)
+    assert relerr(gpu_ctx.constant_matrix(m.n), D) < 1e-11
+    meth = method_id(fx["method"])
+    tol = fx["tol"] * np.sqrt(m.n)
+    gpu_ctx.set_option("batch", 1)
+    got = gpu_ctx.solve_spectrum(m.n, freq, fx["e0"], method=meth, tol_abs=tol, krylov_dim=fx["gmres_dim"])
+    ref = pxo.bem_solve_spectrum(m, freq, eps, fx["e0"], method=0)
+    assert relerr(got["polar"], ref["polar"]) < (1e-10 if meth == 0 else 1e-5)
+    exp = np.array(fx["expected"]["results"])
+    assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5 if meth == 0 else 1e-4, 1e-6).all()
