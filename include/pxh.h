@@ -152,10 +152,11 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * tolerance of the FP32 inner solves), "tile_i", "splits". */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
-/* Counters of the last pxh_solve_spectrum call: [0] matvecs (FP64 vector
- * equivalents), [1] FP32 block products, [2] FP64 block products,
- * [3] pair-kernel ms (FP32), [4] pair-kernel ms (FP64), [5] wall ms,
- * [6] pair evaluations, [7] outer refinements. */
+/* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector
+ * products, [1] FP32 block products, [2] FP64 block products, [3] / [4]
+ * pair-kernel ms (FP32 / FP64, HIP events on the context's stream),
+ * [5] wall ms, [6] pair evaluations, [7] outer refinements, [8] / [9]
+ * pair-kernel launches (FP32 / FP64), [10] / [11] their algorithmic flops. */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
 #ifdef __cplusplus

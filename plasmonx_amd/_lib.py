@@ -177,7 +177,9 @@ class Context:
         return r
 
     def stats(self):
-        s = np.zeros(8)
-        self._check(self.lib.pxh_get_stats(self._h, _ptr(s), 8))
-        return {"matvecs": s[0], "fp32_products": s[1], "fp64_products": s[2], "wall_ms": s[5],
-                "pair_evaluations": s[6], "outer": s[7]}
+        s = np.zeros(16)
+        self._check(self.lib.pxh_get_stats(self._h, _ptr(s), 16))
+        return {"matvecs": s[0], "fp32_products": s[1], "fp64_products": s[2], "sweep_ms_fp32": s[3],
+                "sweep_ms_fp64": s[4], "wall_ms": s[5], "pair_evaluations": s[6], "outer": s[7],
+                "sweep_launches_fp32": s[8], "sweep_launches_fp64": s[9], "sweep_flops_fp32": s[10],
+                "sweep_flops_fp64": s[11]}

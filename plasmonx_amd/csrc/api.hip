@@ -301,6 +301,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.inner_tol = value;
         } else if (k == "splits") {
             c->opt.splits = (int)value;
+        } else if (k == "precond") {
+            c->opt.precond = value != 0.0;
         } else if (k == "max_outer") {
             c->opt.max_outer = std::max(1, (int)value);
         } else {
@@ -311,7 +313,7 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
 
 int pxh_get_stats(pxh_ctx *c, double *stats, int n) {
     if (!c || !stats) return PXH_EINVAL;
-    for (int k = 0; k < n && k < 8; ++k) stats[k] = c->stats.v[k];
+    for (int k = 0; k < n && k < 16; ++k) stats[k] = c->stats.v[k];
     return PXH_OK;
 }
 

@@ -30,6 +30,15 @@ namespace {
 constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 constexpr double kFourOverSqrtPi = 2.2567583341910252;
 constexpr int TB = 256;  // work-items per workgroup = target atoms per tile = source atoms per LDS tile
+// Algorithmic flops of the T sweep, counted from k_tsweep with each erf /
+// exp / rsqrt counted as one: the interaction elements of one ordered pair
+// (omegaFQFmu: distance 10, T_qq 3, charge-dipole 12, dipole-dipole 17;
+// omegaFQ: 13) and the contraction with one complex vector (omegaFQFmu:
+// d.mu 10, q row 8, dipole coefficient 6, mu rows 24; omegaFQ: 4).
+constexpr int kPairEvalFlopsFmu = 42;
+constexpr int kPairVecFlopsFmu = 48;
+constexpr int kPairEvalFlopsQ = 13;
+constexpr int kPairVecFlopsQ = 4;
 
 template <typename R>
 struct Cx {
@@ -390,6 +399,74 @@ __global__ void k_kbar_fill(int n, int nt, const double4 *__restrict__ pos, Ferm
     }
 }
 
+// Diagonal of K-bar T_qq per atom (the preconditioner of the batched path):
+// (K T)_ii = sum_j K_ij (T_qq(j,i) - T_qq(i,i)) with T_qq(i,i) = eta_i.
+__global__ void k_diag_sums(int n, int nt, int kern, const int *__restrict__ rowptr, const int *__restrict__ col,
+                            const double *__restrict__ kij, const double *__restrict__ kji,
+                            const double4 *__restrict__ pos, const PairConst *__restrict__ pc,
+                            const double *__restrict__ eta, double *__restrict__ s1, double *__restrict__ s2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    const int ti = (int)pi.w;
+    double a = 0.0, b[kMaxTypes];
+    for (int t = 0; t < nt; ++t) b[t] = 0.0;
+    for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) {
+        const int j = col[z];
+        const double4 pj = pos[j];
+        const int tj = (int)pj.w;
+        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        const PairConst p = pc[tj + nt * ti];
+        double t;
+        if (kern == PXH_OHNO)
+            t = p.eta_bar / sqrt(1.0 + p.eta_bar * p.eta_bar * r2);
+        else if (kern == PXH_COULOMB)
+            t = 1.0 / r;
+        else
+            t = erf(r * p.inv_rqq) / r;
+        a += kij[z] * (t - eta[ti]);
+        b[tj] += kji[z] * (t - eta[ti]);
+    }
+    s1[i] = a;
+    for (int t = 0; t < nt; ++t) s2[t + (size_t)nt * i] = b[t];
+}
+
+// dinv = 1 / diag(A(omega)): q rows diag(c T_qq) - z_q, mu rows Lambda.
+__global__ void k_precond_diag(int n, int nt, int fmu, int hetero, const double4 *__restrict__ pos,
+                               const double *__restrict__ s1, const double *__restrict__ s2,
+                               const double *__restrict__ nbfrac, const SysParam *__restrict__ par,
+                               double2 *__restrict__ dinv) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = fmu ? 4 * n : n;
+    if (r >= nv) return;
+    double2 d;
+    if (r < n) {
+        const int ti = (int)pos[r].w;
+        d = make_double2(s1[r], 0.0);
+        if (hetero) {
+            double re = 0.5 * s1[r], im = 0.0;
+            for (int t = 0; t < nt; ++t) {
+                const double2 wr = par->wratio[ti + nt * t];
+                re += 0.5 * wr.x * s2[t + (size_t)nt * r];
+                im += 0.5 * wr.y * s2[t + (size_t)nt * r];
+            }
+            d = make_double2(re, im);
+        }
+        d.x -= par->zq[ti].x;
+        d.y -= par->zq[ti].y;
+    } else {
+        const int i = (r - n) / 3;
+        d = make_double2(0, 0);
+        for (int t = 0; t < nt; ++t) {
+            d.x += nbfrac[t + (size_t)nt * i] * par->inv_alpha[t].x;
+            d.y += nbfrac[t + (size_t)nt * i] * par->inv_alpha[t].y;
+        }
+    }
+    const double den = d.x * d.x + d.y * d.y;
+    dinv[r] = den > 0.0 ? make_double2(d.x / den, -d.y / den) : make_double2(1.0, 0.0);
+}
+
 // Dynamic RHS, q rows: rhs(i,k) = -sum_j c_ij (V_k(j) - V_k(i)), c_ij as in k_kbar_q.
 __global__ __launch_bounds__(64) void k_rhs_q(int n, int nt, int hetero, const int *__restrict__ rowptr,
                                               const int *__restrict__ col, const double *__restrict__ kij,
@@ -562,6 +639,7 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     c->work_t.alloc((size_t)nsys * nv);
     double2 *part = c->work_part.p;
     dim3 grid((n + TB - 1) / TB, splits_eff, groups);
+    PXH_HIP(hipEventRecord(c->ev0, c->stream));
     if (fmu)
         launch_tsweep_k<R, KV, true>(A.kernel, grid, c->stream, n, A.nt, A.hetero ? 1 : 0, A.pos.p, A.pair.p, nsys,
                                      d_sys_list, x, ld, part, nv, j_chunk);
@@ -569,7 +647,21 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         launch_tsweep_k<R, KV, false>(A.kernel, grid, c->stream, n, A.nt, 0, A.pos.p, A.pair.p, nsys, d_sys_list,
                                       x, ld, part, nv, j_chunk);
     PXH_HIP(hipGetLastError());
+    PXH_HIP(hipEventRecord(c->ev1, c->stream));
     finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff);
+    // time of the pair kernel on the context's stream, and its algorithmic
+    // flops: per ordered pair (i != j) kPairEvalFlops for the interaction
+    // elements plus kPairVecFlops per complex vector for the contraction
+    PXH_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    PXH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const bool f32 = sizeof(R) == 4;
+    const double pairs = (double)n * (double)(n - 1);
+    const double fl = pairs * ((double)(fmu ? kPairEvalFlopsFmu : kPairEvalFlopsQ) * groups +
+                               (double)(fmu ? kPairVecFlopsFmu : kPairVecFlopsQ) * nsys);
+    c->stats.v[f32 ? 3 : 4] += ms;
+    c->stats.v[f32 ? 8 : 9] += 1;
+    c->stats.v[f32 ? 10 : 11] += fl;
     c->stats.v[6] += (double)n * (double)n * groups;
 }
 
@@ -699,8 +791,22 @@ void build_kbar(pxh_ctx *c) {
     A.kb_kji.alloc(std::max<size_t>(1, A.kb_nnz));
     k_kbar_fill<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.pos.p, ft, A.kb_rowptr.p, A.kb_col.p,
                                                               A.kb_kij.p, A.kb_kji.p);
+    A.diag_s1.alloc(A.n);
+    A.diag_s2.alloc((size_t)A.n * A.nt);
+    k_diag_sums<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.kernel, A.kb_rowptr.p, A.kb_col.p,
+                                                              A.kb_kij.p, A.kb_kji.p, A.pos.p, A.pair.p, A.d_eta.p,
+                                                              A.diag_s1.p, A.diag_s2.p);
     PXH_HIP(hipGetLastError());
     PXH_HIP(hipStreamSynchronize(c->stream));
+}
+
+void precond_diag_atoms(pxh_ctx *c, const SysParam *d_par_one, double2 *dinv) {
+    Atoms &A = c->atoms;
+    const int nv = A.n_var();
+    k_precond_diag<<<(nv + 255) / 256, 256, 0, c->stream>>>(A.n, A.nt, A.model == PXH_WFQFMU, A.hetero ? 1 : 0,
+                                                            A.pos.p, A.diag_s1.p, A.diag_s2.p, A.d_nbfrac.p,
+                                                            d_par_one, dinv);
+    PXH_HIP(hipGetLastError());
 }
 
 void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs) {

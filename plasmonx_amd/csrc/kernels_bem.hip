@@ -438,3 +438,25 @@ void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs) {
 }
 
 }  // namespace pxh
+
+namespace pxh {
+
+namespace {
+// dinv = 1 / (D_ii + lambda) (DPCM diagonal)
+__global__ void k_precond_bem(int n, const double *__restrict__ D, const double2 *__restrict__ lam,
+                              double2 *__restrict__ dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 d = make_double2(D[i + (size_t)n * i] + lam->x, lam->y);
+    const double den = d.x * d.x + d.y * d.y;
+    dinv[i] = make_double2(d.x / den, -d.y / den);
+}
+}  // namespace
+
+void precond_diag_bem(pxh_ctx *c, const double2 *d_lambda_one, double2 *dinv) {
+    const int n = c->mesh.n;
+    k_precond_bem<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->mesh.D.p, d_lambda_one, dinv);
+    PXH_HIP(hipGetLastError());
+}
+
+}  // namespace pxh

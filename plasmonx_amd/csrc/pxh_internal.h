@@ -88,6 +88,9 @@ struct Atoms {
     DevBuf<double> kb_kij, kb_kji;      // K_ij = (1-f_ij) A_i / r, K_ji = (1-f_ji) A_j / r
     size_t kb_nnz = 0;
     DevBuf<double> Tstore;              // stored T for the in-memory method (n_var^2)
+    // diagonal of K-bar T_qq: S1_i = sum_j K_ij (T_qq(j,i) - eta_i), and for
+    // alloys S2_i[t] = sum_{j of type t} K_ji (T_qq(j,i) - eta_i)
+    DevBuf<double> diag_s1, diag_s2;
     int n_var() const { return model == PXH_WFQFMU ? 4 * n : n; }
 };
 
@@ -112,10 +115,11 @@ struct Options {
     int splits = 0;          // 0 = automatic
     int kv = 4;              // systems per pair-kernel pass (FP32)
     int max_outer = 30;
+    int precond = 1;         // diagonal right preconditioning in the batched path
 };
 
 struct Stats {
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double v[16] = {};
 };
 
 }  // namespace pxh
@@ -162,6 +166,9 @@ void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag);
 void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs /* nv x 3 */);
 void constant_matrix_atoms(pxh_ctx *c, double *d_m);
 SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq);
+// dinv = 1 / diag(A(omega)) of one system (n_var complex, device)
+void precond_diag_atoms(pxh_ctx *c, const SysParam *d_par_one, double2 *dinv);
+void precond_diag_bem(pxh_ctx *c, const double2 *d_lambda_one, double2 *dinv);
 // kernels_bem.hip
 void build_mesh_device(pxh_ctx *c);
 void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lambda, const double2 *x, double2 *y,

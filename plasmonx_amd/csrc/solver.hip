@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64) void k_trsv(const int *__restrict__ slots, cons
 // X[s] (+)= V[s][0:jlen] y[s]
 __global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__restrict__ jlen,
                            const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ y, int ldy,
-                           double2 *__restrict__ X, int accumulate) {
+                           const double2 *__restrict__ dinv, double2 *__restrict__ X, int accumulate) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
     if (r >= nv) return;
@@ -242,6 +242,7 @@ __global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__r
         acc.x += p.x;
         acc.y += p.y;
     }
+    if (dinv) acc = cmul(dinv[(size_t)nv * s + r], acc);  // x = D^-1 u (right preconditioning)
     double2 &xv = X[(size_t)nv * s + r];
     if (accumulate) {
         xv.x += acc.x;
@@ -249,6 +250,18 @@ __global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__r
     } else {
         xv = acc;
     }
+}
+
+// Operator input of each slot: stage[s] = V[s][j_s] (times D^-1 when preconditioned).
+__global__ void k_stage(int nv, const int *__restrict__ slots, const int *__restrict__ jcol,
+                        const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ dinv,
+                        double2 *__restrict__ stage) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    double2 v = V[ldv_slot * s + (size_t)nv * jcol[s] + r];
+    if (dinv) v = cmul(dinv[(size_t)nv * s + r], v);
+    stage[(size_t)nv * s + r] = v;
 }
 
 // Start a Krylov cycle: V[s][0] = R[s] / beta[s], g[s] = (beta, 0, ...).
@@ -294,6 +307,8 @@ struct Batched {
     std::vector<SysParam> h_par;
     std::vector<double2> h_lam;
     DevBuf<double2> lam;
+    DevBuf<double2> dinv;  // right preconditioner 1/diag(A(omega_s)) per slot
+    bool precond = false;
     std::vector<int> jcount;
     size_t ldv_slot = 0, ldH_slot = 0;
     int ldH = 0, ldg = 0, ldh = 0;
@@ -371,6 +386,10 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
     S.jp1_d.alloc(B);
     S.jl_d.alloc(B);
     S.stage.alloc((size_t)nv * B);
+    // diagonal right preconditioning (x = D^-1 u) for the batched path; the
+    // plain path is the reference's unpreconditioned GMRES
+    S.precond = !plain && c->opt.precond && !(c->target == PXH_BEM && c->mesh.variant == PXH_IEFPCM);
+    if (S.precond) S.dinv.alloc((size_t)nv * B);
     S.h_jp1.assign(B, 0);
     S.h_jl.assign(B, 0);
     S.h_jc.assign(B, 0);
@@ -412,6 +431,12 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
         }
         k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, s, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
                                                     S.Bv.p, S.R.p, S.X.p);
+        if (S.precond) {
+            if (c->target == PXH_BEM)
+                precond_diag_bem(c, S.lam.p + s, S.dinv.p + (size_t)nv * s);
+            else
+                precond_diag_atoms(c, S.par.p + s, S.dinv.p + (size_t)nv * s);
+        }
         outer[s] = 0;
         total_it[s] = 0;
         phase[s] = NEED_RES;  // the residual of x = 0 is b: computed below without a product
@@ -498,9 +523,8 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
         // W = A V_j: V_j of each slot is staged at stage + nv * slot (the operator's layout)
         S.h_jc = S.jcount;
         PXH_HIP(hipMemcpyAsync(S.jcount_d.p, S.h_jc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
-        for (int s : act)
-            PXH_HIP(hipMemcpyAsync(S.stage.p + (size_t)nv * s, S.V.p + S.ldv_slot * s + (size_t)nv * S.jcount[s],
-                                   sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
+        k_stage<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, S.V.p, S.ldv_slot,
+                                              S.precond ? S.dinv.p : nullptr, S.stage.p);
         apply_op(S, na, !S.mixed, S.stage.p, S.W.p);
         if (S.mixed)
             c->stats.v[1] += 1;
@@ -555,7 +579,8 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
             k_trsv<<<(int)fin.size(), 64, 0, st>>>(S.slots_d.p, jl_dp, S.H.p, S.ldH, S.ldH_slot, S.g.p, S.ldg,
                                                    S.y.p, S.ldh);
             k_update_x<<<grid_rows((int)fin.size()), 256, 0, st>>>(nv, S.slots_d.p, jl_dp, S.V.p, S.ldv_slot,
-                                                                  S.y.p, S.ldh, S.X.p, 1);
+                                                                  S.y.p, S.ldh, S.precond ? S.dinv.p : nullptr,
+                                                                  S.X.p, 1);
             PXH_HIP(hipGetLastError());
             for (int s : fin) phase[s] = NEED_RES;
         }

@@ -1,0 +1,98 @@
+"""The benchmark workloads: deterministic synthetic targets with the
+reference's default parameters (python/input_/validation.py:961-971 atom
+types, :1134-1145 parameters; unit rules :8-140).
+
+C1  omegaFQ Na fcc sphere a=4.23 A, r=2 nm (1745 atoms), 61 w in 2.5-4.0 eV, x field
+C2  omegaFQFmu Ag fcc sphere a=4.09 A, r=4 nm (15683 atoms, 62732 unknowns),
+    201 w in 2.8-4.2 eV, 3 fields, interband alpha(w) from the Etchegoin
+    (Drude + critical-point Lorentz) silver model
+C3  BEM DPCM / IEF on an icosphere R=10 nm (5120 triangles) and a capped rod
+C4  96 seeded graphene flakes, 128 w in 0.1-1.0 eV
+"""
+import math
+
+import numpy as np
+
+from . import geometry, permittivity
+from .model import R0_BOHR, TOBOHR, AU_TO_EV, AtomSystem, AtomType, Mesh
+
+SIGMA_SI_TO_AU = 0.00000021739
+TAU_FS_TO_AU = 41.3413733351702
+
+
+def ev(x):
+    return np.asarray(x, dtype=np.float64) / AU_TO_EV
+
+
+def silver_type():
+    tau = 1633.608
+    sigma0 = 6.5313e7 * SIGMA_SI_TO_AU
+    return AtomType(name="Ag", eta=0.4829838, a_ij=9.61, tau=tau, density=sigma0 / tau, fermi_d=12.0, fermi_s=1.1,
+                    r_q=3.3775793621738925, r_mu=2.368922250085755, alpha=49.9843, permittivity="silver etchegoin")
+
+
+def gold_type():
+    tau = 318.018
+    sigma0 = 11834849.81 * SIGMA_SI_TO_AU
+    return AtomType(name="Au", eta=0.52685, a_ij=9.61, tau=tau, density=sigma0 / tau, fermi_d=12.0, fermi_s=1.1,
+                    r_q=3.4996572071713126, r_mu=3.0710227258450864, alpha=39.5297, permittivity="gold etchegoin")
+
+
+def c1_sodium_sphere():
+    """C1: tau = 1.5e-14 s, sigma0 from w_p = 5.9 eV, Gaussian R_q = 2.5 bohr,
+    eta = sqrt(2/pi)/R_q, A_ij Na default, Fermi d = 12, s r0 = 1.1 l_nn with
+    r0 = RNa0 (parameters_module.f90:53)."""
+    xyz = geometry.fcc_sphere(4.23, 20.0)
+    tau = 1.5e-14 * TAU_FS_TO_AU * 1e15
+    wp = 5.9 / AU_TO_EV
+    sigma0 = wp ** 2 * tau / (4.0 * math.pi)
+    l_nn = 4.23 / math.sqrt(2.0) * TOBOHR
+    s = 1.1 * l_nn / R0_BOHR["Na"]
+    rq = 2.5
+    t = AtomType(name="Na", eta=math.sqrt(2.0 / math.pi) / rq, a_ij=12.07910025, tau=tau, density=sigma0 / tau,
+                 fermi_d=12.0, fermi_s=s, r_q=rq)
+    sysm = AtomSystem(model="wfq", kernel="gaussian", symbols=["Na"] * len(xyz), coord_bohr=xyz * TOBOHR, types=[t],
+                      fermi_d=np.array([[12.0]]), fermi_s=np.array([[s]]), r0=np.array([[R0_BOHR["Na"]]]))
+    freq = ev(np.linspace(2.5, 4.0, 61))
+    return sysm, freq
+
+
+def c2_silver_sphere(radius=40.0, n_freq=201):
+    xyz = geometry.fcc_sphere(4.09, radius)
+    t = silver_type()
+    sysm = AtomSystem(model="wfqfmu", kernel="gaussian", symbols=["Ag"] * len(xyz), coord_bohr=xyz * TOBOHR,
+                      types=[t], fermi_d=np.array([[12.0]]), fermi_s=np.array([[1.1]]), r0=np.array([[R0_BOHR["Ag"]]]))
+    freq = ev(np.linspace(2.8, 4.2, n_freq))
+    return sysm, freq
+
+
+def core_shell_sphere(r_core, r_shell, a=4.08):
+    sym, xyz = geometry.core_shell(a, r_core, r_shell)
+    types = [gold_type(), silver_type()]
+    fd = np.full((2, 2), 12.0)
+    fs = np.array([[1.1, 0.95], [0.95, 1.1]])
+    r0 = np.full((2, 2), R0_BOHR["Ag"])
+    return AtomSystem(model="wfqfmu", kernel="gaussian", symbols=sym, coord_bohr=xyz * TOBOHR, types=types,
+                      fermi_d=fd, fermi_s=fs, r0=r0)
+
+
+def graphene_flake(xyz, fermi_energy_ev, tau_s=1e-13):
+    tau = tau_s * TAU_FS_TO_AU * 1e15
+    t = AtomType(name="C", eta=0.372124, a_ij=1.7424, tau=tau, density=fermi_energy_ev / (math.pi * AU_TO_EV),
+                 fermi_d=100.0, fermi_s=1.2, r_q=math.sqrt(2.0 / math.pi) / 0.372124)
+    return AtomSystem(model="wfq", kernel="gaussian", symbols=["C"] * len(xyz), coord_bohr=np.asarray(xyz) * TOBOHR,
+                      types=[t], fermi_d=np.array([[100.0]]), fermi_s=np.array([[1.2]]),
+                      r0=np.array([[R0_BOHR["C"]]]))
+
+
+def c3_meshes(variant="dpcm"):
+    sphere = Mesh(verts_bohr=geometry.icosphere(100.0, 4, seed=0) * TOBOHR, variant=variant, green="accurate")
+    rod = Mesh(verts_bohr=geometry.capped_rod(400.0, 100.0, n_ring=80, n_cap=20) * TOBOHR, variant=variant,
+               green="accurate")
+    freq = ev(np.linspace(3.0, 4.0, 400))
+    return sphere, rod, freq
+
+
+def c3_permittivity(freq):
+    bb = permittivity.BB_SILVER
+    return permittivity.brendel_bormann(freq, bb["eps_inf"], bb["wp_ev"], bb["f0"], bb["g0_ev"], bb["osc"])
