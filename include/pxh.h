@@ -148,8 +148,11 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
 
 /* Tuning knobs (no counterpart in the reference): "kbar_cutoff" (1 = drop
  * K-bar pairs whose Fermi factor 1-f is below 1e-18, the default; 0 = keep
- * every pair), "batch" (systems per batched solve), "inner_tol" (relative
- * tolerance of the FP32 inner solves), "tile_i", "splits". */
+ * every pair), "batch" (systems sharing each product), "inner_tol" (relative
+ * tolerance of the low-precision inner solves), "precond" (diagonal right
+ * preconditioning of the batched path), "mfma" (omegaFQFmu low-precision
+ * products on MFMA), "matvec_mixed" (pxh_matvec uses the low-precision
+ * product), "splits", "max_outer". */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
 /* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector

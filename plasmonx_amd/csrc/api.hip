@@ -232,7 +232,10 @@ int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *
             std::vector<pxh::SysParam> hp(n_vec, pxh::make_sysparam(c, freq_au, i_freq));
             pxh::DevBuf<pxh::SysParam> dp;
             dp.upload(hp.data(), n_vec, c->stream);
-            pxh::apply_atoms_fp64(c, n_vec, dl.p, dp.p, dx.p, dy.p, nv);
+            if (c->opt.matvec_mixed)
+                pxh::apply_atoms_fp32(c, n_vec, dl.p, dp.p, dx.p, dy.p, nv);
+            else
+                pxh::apply_atoms_fp64(c, n_vec, dl.p, dp.p, dx.p, dy.p, nv);
         }
         PXH_HIP(hipMemcpyAsync(y, dy.p, sizeof(double2) * nv * n_vec, hipMemcpyDeviceToHost, c->stream));
         PXH_HIP(hipStreamSynchronize(c->stream));
@@ -301,6 +304,10 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.inner_tol = value;
         } else if (k == "splits") {
             c->opt.splits = (int)value;
+        } else if (k == "matvec_mixed") {
+            c->opt.matvec_mixed = value != 0.0;
+        } else if (k == "mfma") {
+            c->opt.mfma = value != 0.0;
         } else if (k == "precond") {
             c->opt.precond = value != 0.0;
         } else if (k == "max_outer") {

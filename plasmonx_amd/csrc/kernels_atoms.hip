@@ -608,6 +608,8 @@ int choose_splits(const pxh_ctx *c, int n, int groups) {
     return std::min(s, max_s);
 }
 
+}  // namespace
+
 // Second stage of every product: t = sum of the split partials (+ eta x_q),
 // then y_q = K-bar t_q - z_q x_q and y_mu = t_mu + Lambda x_mu.
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
@@ -624,6 +626,8 @@ void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam 
                                                                           d_par, tt, nv, x, ld, y);
     PXH_HIP(hipGetLastError());
 }
+
+namespace {
 
 template <typename R, int KV>
 void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
@@ -695,6 +699,11 @@ void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
 void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld) {
     if (nsys <= 0) return;
+    const Atoms &A = c->atoms;
+    if (c->opt.mfma && A.model == PXH_WFQFMU && A.kernel == PXH_GAUSSIAN) {
+        apply_atoms_mfma(c, nsys, d_sys_list, d_par, x, y, ld);
+        return;
+    }
     if (nsys == 1)
         apply_atoms_impl<float, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
     else if (nsys == 2)

@@ -116,6 +116,8 @@ struct Options {
     int kv = 4;              // systems per pair-kernel pass (FP32)
     int max_outer = 30;
     int precond = 1;         // diagonal right preconditioning in the batched path
+    int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
+    int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
 };
 
 struct Stats {
@@ -140,6 +142,8 @@ struct pxh_ctx {
     pxh::Stats stats;
     std::string last_error;
     pxh::DevBuf<double2> work_part, work_t;  // scratch of the operator products
+    pxh::DevBuf<_Float16> work_xh, work_xl;   // fp16 hi / lo operand of the MFMA products
+    pxh::DevBuf<float> work_scale;            // per-column scales of that operand
     bool use_stored = false;                  // in-memory method: products use the stored T
 };
 
@@ -161,6 +165,8 @@ void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
                       double2 *y, int ld);
 void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                         double2 *y, int ld);
+void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                      double2 *y, int ld);
 void prepare_stored_t(pxh_ctx *c);
 void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag);
 void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs /* nv x 3 */);

@@ -168,3 +168,28 @@ def test_bem_matches_oracle_and_log(gpu_ctx, case):
     assert relerr(got["polar"], ref["polar"]) < (1e-10 if meth == 0 else 1e-5)
     exp = np.array(fx["expected"]["results"])
     assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5 if meth == 0 else 1e-4, 1e-6).all()
+
+
+@pytest.mark.parametrize("mfma", [0, 1])
+@pytest.mark.parametrize("nvec", [1, 5, 50])
+def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
+    """The low-precision product (FP32 VALU tiles, or fp16 hi/lo MFMA) against
+    the FP64 product on a 675-atom Ag sphere: relative error <= 2e-5 per vector
+    (the inner solves need ~1e-5; FP64 refinement fixes the final accuracy)."""
+    from plasmonx_amd.workloads import c2_silver_sphere
+    s, freq = c2_silver_sphere(radius=14.0, n_freq=3)
+    gpu_ctx.set_option("kbar_cutoff", 1)
+    gpu_ctx.set_atoms(s)
+    gpu_ctx.set_alpha(s.alpha_table(freq))
+    rng = np.random.default_rng(nvec)
+    x = rng.normal(size=(nvec, s.n_var)) + 1j * rng.normal(size=(nvec, s.n_var))
+    x[-1] *= 1e-3  # columns of different magnitude
+    ref = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("mfma", mfma)
+    gpu_ctx.set_option("matvec_mixed", 1)
+    got = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("matvec_mixed", 0)
+    gpu_ctx.set_option("mfma", 1)
+    for k in range(nvec):
+        err = np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k])
+        assert err < 2e-5, (k, err)
