@@ -89,13 +89,14 @@ struct SrcTile {
     Cx<R> m[KV][3][TB];
 };
 
-// T sweep: part[split][k][row] = sum_{j in split, j != i} T(i, j) x_k(j).
+// T sweep: part[split][row][k] = sum_{j in split, j != i} T(i, j) x_k(j).
 // KERN: 1 Gaussian, 2 Coulomb, 3 Ohno.  FMU: omegaFQFmu (4 unknowns/atom).
 template <typename R, int KV, bool FMU, int KERN>
 __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const double4 *__restrict__ pos,
                                                const PairConst *__restrict__ pc, int nsys,
                                                const int *__restrict__ sys_list, const double2 *__restrict__ x,
-                                               int ld, double2 *__restrict__ part, int nv, int j_chunk) {
+                                               int ld, double2 *__restrict__ part, int nv, int j_chunk,
+                                               double rfar2) {
     __shared__ SrcTile<R, KV> tile;
     __shared__ PairR<R> spc[kMaxTypes * kMaxTypes];
     const int tid = threadIdx.x;
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
         spc[e].inv_rmumu = (R)pc[e].inv_rmumu;
         spc[e].eta_bar = (R)pc[e].eta_bar;
     }
+    const R rfar2r = (R)rfar2;
     const bool active = i < n;
     double4 pi = active ? pos[i] : make_double4(0, 0, 0, 0);
     const int ti = (int)pi.w;
@@ -172,13 +174,41 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
                 const R r = r2 * inv_r;
                 const int tj = (int)pj.w;
                 const PairR<R> p = spc[ti + nt * tj];
-                R tqq;
-                if (KERN == 1)
-                    tqq = d_erf<R>(r * p.inv_rqq) * inv_r;
-                else if (KERN == 2)
+                const R inv_r2 = inv_r * inv_r;
+                const R inv_r3 = inv_r2 * inv_r;
+                R tqq, g1 = (R)0, g2 = (R)0, c0 = (R)0, c1 = (R)0;
+                if (KERN == 1 && r2 >= rfar2r) {
+                    // far field: the Gaussian damping is below the working
+                    // precision (erf -> 1, exp -> 0 to rounding)
                     tqq = inv_r;
-                else
-                    tqq = p.eta_bar * d_rsqrt<R>((R)1 + p.eta_bar * p.eta_bar * r2);
+                    g1 = g2 = c0 = inv_r3;
+                    c1 = (R)-3 * inv_r3 * inv_r2;
+                } else {
+                    if (KERN == 1)
+                        tqq = d_erf<R>(r * p.inv_rqq) * inv_r;
+                    else if (KERN == 2)
+                        tqq = inv_r;
+                    else
+                        tqq = p.eta_bar * d_rsqrt<R>((R)1 + p.eta_bar * p.eta_bar * r2);
+                    if (FMU) {
+                        // charge on i, dipole on j
+                        const R a1 = r * p.inv_rqmu;
+                        const R e1 = d_exp<R>(-a1 * a1);
+                        g1 = (d_erf<R>(a1) - (R)kTwoOverSqrtPi * a1 * e1) * inv_r3;
+                        g2 = g1;
+                        if (hetero) {  // charge on j, dipole on i
+                            const R a2 = r * spc[tj + nt * ti].inv_rqmu;
+                            const R e2 = d_exp<R>(-a2 * a2);
+                            g2 = (d_erf<R>(a2) - (R)kTwoOverSqrtPi * a2 * e2) * inv_r3;
+                        }
+                        const R a3 = r * p.inv_rmumu;
+                        const R e3 = d_exp<R>(-a3 * a3);
+                        const R fd1 = d_erf<R>(a3) - (R)kTwoOverSqrtPi * a3 * e3;
+                        const R fd2 = (R)kFourOverSqrtPi * p.inv_rmumu * p.inv_rmumu * p.inv_rmumu * e3;
+                        c0 = fd1 * inv_r3;
+                        c1 = fd2 * inv_r2 - (R)3 * fd1 * inv_r3 * inv_r2;
+                    }
+                }
                 if (!FMU) {
 #pragma unroll
                     for (int v = 0; v < KV; ++v) {
@@ -187,24 +217,6 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
                     }
                     continue;
                 }
-                const R inv_r2 = inv_r * inv_r;
-                const R inv_r3 = inv_r2 * inv_r;
-                // charge on i, dipole on j
-                const R a1 = r * p.inv_rqmu;
-                const R e1 = d_exp<R>(-a1 * a1);
-                const R g1 = (d_erf<R>(a1) - (R)kTwoOverSqrtPi * a1 * e1) * inv_r3;
-                R g2 = g1;
-                if (hetero) {  // charge on j, dipole on i
-                    const R a2 = r * spc[tj + nt * ti].inv_rqmu;
-                    const R e2 = d_exp<R>(-a2 * a2);
-                    g2 = (d_erf<R>(a2) - (R)kTwoOverSqrtPi * a2 * e2) * inv_r3;
-                }
-                const R a3 = r * p.inv_rmumu;
-                const R e3 = d_exp<R>(-a3 * a3);
-                const R fd1 = d_erf<R>(a3) - (R)kTwoOverSqrtPi * a3 * e3;
-                const R fd2 = (R)kFourOverSqrtPi * p.inv_rmumu * p.inv_rmumu * p.inv_rmumu * e3;
-                const R c0 = fd1 * inv_r3;
-                const R c1 = fd2 * inv_r2 - (R)3 * fd1 * inv_r3 * inv_r2;
 #pragma unroll
                 for (int v = 0; v < KV; ++v) {
                     const Cx<R> q = tile.q[v][jj];
@@ -236,85 +248,121 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
         }
     }
     if (!active) return;
+    // part[split][row][system]
+    double2 *out = part + (size_t)split * nv * nsys + g0;
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
         if (v >= nk) break;
-        double2 *out = part + ((size_t)split * nsys + g0 + v) * nv;
-        out[i] = make_double2(aq_re[v], aq_im[v]);
+        out[(size_t)i * nsys + v] = make_double2(aq_re[v], aq_im[v]);
         if (FMU) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) out[n + 3 * i + a] = make_double2(am_re[v][a], am_im[v][a]);
+            for (int a = 0; a < 3; ++a)
+                out[(size_t)(n + 3 * i + a) * nsys + v] = make_double2(am_re[v][a], am_im[v][a]);
         }
     }
 }
 
-// t[k][row] = sum_split part[split][k][row] + (q rows) eta_i x_q(i).
+// t[row][k] = sum_split part[split][row][k] + (q rows) eta_i x_q(i): the
+// product intermediates are row-major (all systems of a row adjacent), so
+// that the sparse K-bar stage gathers whole rows.
 __global__ void k_reduce_splits(int n, int nv, int nsys, int splits, const double2 *__restrict__ part,
                                 const int *__restrict__ sys_list, const double2 *__restrict__ x, int ld,
                                 const double4 *__restrict__ pos, const double *__restrict__ eta,
                                 double2 *__restrict__ t) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (row >= nv) return;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)nv * nsys) return;
+    const int row = (int)(g / nsys), k = (int)(g % nsys);
     double re = 0.0, im = 0.0;
     for (int s = 0; s < splits; ++s) {
-        double2 p = part[((size_t)s * nsys + k) * nv + row];
+        const double2 p = part[(size_t)s * nv * nsys + g];
         re += p.x;
         im += p.y;
     }
     if (row < n) {
-        double e = eta[(int)pos[row].w];
-        double2 xq = x[(size_t)sys_list[k] * ld + row];
+        const double e = eta[(int)pos[row].w];
+        const double2 xq = x[(size_t)sys_list[k] * ld + row];
         re += e * xq.x;
         im += e * xq.y;
     }
-    t[(size_t)k * nv + row] = make_double2(re, im);
+    t[g] = make_double2(re, im);
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// q rows: y_k(i) = sum_j c_ij (t_k(j) - t_k(i)) - z_q(t_i) x_k(i), one wavefront per row.
-// c_ij = K_ij, or 1/2 (K_ij + (w_tj / w_ti) K_ji) for alloys.
-__global__ __launch_bounds__(64) void k_kbar_q(int n, int nt, int hetero, const int *__restrict__ rowptr,
-                                               const int *__restrict__ col, const double *__restrict__ kij,
-                                               const double *__restrict__ kji, const double4 *__restrict__ pos,
-                                               int nsys, const int *__restrict__ sys_list,
-                                               const SysParam *__restrict__ par, const double2 *__restrict__ t,
-                                               int nv, const double2 *__restrict__ x, int ld,
-                                               double2 *__restrict__ y) {
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
+// value of v held by lane l (l wave-uniform)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// q rows: y_k(i) = sum_j c_ij (t_k(j) - t_k(i)) - z_q(t_i) x_k(i), with
+// c_ij = K_ij, or 1/2 (K_ij + (w_tj / w_ti) K_ji) for alloys.  One wavefront
+// per row and one lane per system: the CSR entries are wave-uniform loads
+// and each neighbour's systems are one contiguous row of t.  Workgroups
+// with the same blockIdx.x % 8 (one XCD) take one contiguous range of rows,
+// so the neighbour rows they gather stay in that XCD's L2.
+__global__ __launch_bounds__(256) void k_kbar_q(int n, int nt, int hetero, const int *__restrict__ rowptr,
+                                                const int *__restrict__ col, const double *__restrict__ kij,
+                                                const double *__restrict__ kji, const double4 *__restrict__ pos,
+                                                int nsys, const int *__restrict__ sys_list,
+                                                const SysParam *__restrict__ par, const double2 *__restrict__ t,
+                                                const double2 *__restrict__ x, int ld, double2 *__restrict__ y) {
+    const int nblk = (n + 3) / 4;
+    const int per = (nblk + 7) / 8;
+    const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int i = blk * 4 + threadIdx.x / 64;
+    if (blk >= nblk || i >= n) return;
+    const int lane = threadIdx.x % 64;
     const int ti = (int)pos[i].w;
     const int b = rowptr[i], e = rowptr[i + 1];
-    for (int k = 0; k < nsys; ++k) {
-        const int s = sys_list[k];
-        const double2 *tk = t + (size_t)k * nv;
-        const double2 tii = tk[i];
+    for (int k0 = 0; k0 < nsys; k0 += 64) {
+        const int k = k0 + lane;
+        const bool on = k < nsys;
+        const int kk = on ? k : 0;
+        const int s = sys_list[kk];
+        const double2 tii = t[(size_t)i * nsys + kk];
         double re = 0.0, im = 0.0;
-        for (int z = b + lane; z < e; z += 64) {
-            const int j = col[z];
-            const double2 tj = tk[j];
-            const double2 d = make_double2(tj.x - tii.x, tj.y - tii.y);
-            if (!hetero) {
-                re += kij[z] * d.x;
-                im += kij[z] * d.y;
-            } else {
-                const int tj_t = (int)pos[j].w;
-                const double2 wr = par[s].wratio[ti + nt * tj_t];
-                const double2 c = make_double2(0.5 * (kij[z] + wr.x * kji[z]), 0.5 * wr.y * kji[z]);
-                const double2 p = cmul(c, d);
-                re += p.x;
-                im += p.y;
+        // 64 CSR entries at a time into registers (one per lane), then
+        // broadcast with readlane so that 8 row gathers are in flight at once
+        for (int zb = b; zb < e; zb += 64) {
+            const int zc = min(64, e - zb);
+            const bool has = lane < zc;
+            const int cj = has ? col[zb + lane] : 0;
+            const double kv = has ? kij[zb + lane] : 0.0;
+            const double kw = has && hetero ? kji[zb + lane] : 0.0;
+            const int tjv = hetero ? (int)pos[cj].w : 0;
+            for (int q = 0; q < zc; q += 8) {
+                double2 tv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = __builtin_amdgcn_readlane(cj, q + u);
+                    tv[u] = t[(size_t)j * nsys + kk];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (q + u >= zc) break;
+                    const double kz = readlane_f64(kv, q + u);
+                    const double2 d = make_double2(tv[u].x - tii.x, tv[u].y - tii.y);
+                    if (!hetero) {
+                        re += kz * d.x;
+                        im += kz * d.y;
+                    } else {
+                        const double kz2 = readlane_f64(kw, q + u);
+                        const int tj = __builtin_amdgcn_readlane(tjv, q + u);
+                        const double2 wr = par[s].wratio[ti + nt * tj];
+                        const double2 c = make_double2(0.5 * (kz + wr.x * kz2), 0.5 * wr.y * kz2);
+                        const double2 p = cmul(c, d);
+                        re += p.x;
+                        im += p.y;
+                    }
+                }
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            re += __shfl_down(re, off, 64);
-            im += __shfl_down(im, off, 64);
-        }
-        if (lane == 0) {
+        if (on) {
             const double2 xi = x[(size_t)s * ld + i];
             const double2 zx = cmul(par[s].zq[ti], xi);
             y[(size_t)s * ld + i] = make_double2(re - zx.x, im - zx.y);
@@ -325,11 +373,11 @@ __global__ __launch_bounds__(64) void k_kbar_q(int n, int nt, int hetero, const 
 // mu rows: y_k = t_k + Lambda_i x_k, Lambda_i = sum_t nbfrac(t, i) / alpha_t.
 __global__ void k_mu_rows(int n, int nt, const double *__restrict__ nbfrac, int nsys,
                           const int *__restrict__ sys_list, const SysParam *__restrict__ par,
-                          const double2 *__restrict__ t, int nv, const double2 *__restrict__ x, int ld,
+                          const double2 *__restrict__ t, const double2 *__restrict__ x, int ld,
                           double2 *__restrict__ y) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (r >= 3 * n) return;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)3 * n * nsys) return;
+    const int r = (int)(g / nsys), k = (int)(g % nsys);
     const int s = sys_list[k];
     const int i = r / 3;
     double2 lam = make_double2(0, 0);
@@ -339,7 +387,7 @@ __global__ void k_mu_rows(int n, int nt, const double *__restrict__ nbfrac, int 
         lam.y += f * par[s].inv_alpha[tt].y;
     }
     const double2 xv = x[(size_t)s * ld + n + r];
-    const double2 tv = t[(size_t)k * nv + n + r];
+    const double2 tv = t[(size_t)n * nsys + g];
     const double2 lx = cmul(lam, xv);
     y[(size_t)s * ld + n + r] = make_double2(tv.x + lx.x, tv.y + lx.y);
 }
@@ -583,19 +631,19 @@ __global__ void k_copy_mu_rows(int n, int nv, const double *__restrict__ T, doub
 template <typename R, int KV, bool FMU>
 void launch_tsweep_k(int kern, dim3 grid, hipStream_t st, int n, int nt, int hetero, const double4 *pos,
                      const PairConst *pc, int nsys, const int *sys_list, const double2 *x, int ld, double2 *part,
-                     int nv, int j_chunk) {
+                     int nv, int j_chunk, double rfar2) {
     switch (kern) {
         case PXH_COULOMB:
             k_tsweep<R, KV, FMU, 2><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
-                                                         j_chunk);
+                                                         j_chunk, rfar2);
             break;
         case PXH_OHNO:
             k_tsweep<R, KV, FMU, 3><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
-                                                         j_chunk);
+                                                         j_chunk, rfar2);
             break;
         default:
             k_tsweep<R, KV, FMU, 1><<<grid, TB, 0, st>>>(n, nt, hetero, pos, pc, nsys, sys_list, x, ld, part, nv,
-                                                         j_chunk);
+                                                         j_chunk, rfar2);
     }
 }
 
@@ -613,18 +661,41 @@ int choose_splits(const pxh_ctx *c, int n, int groups) {
 // Second stage of every product: t = sum of the split partials (+ eta x_q),
 // then y_q = K-bar t_q - z_q x_q and y_mu = t_mu + Lambda x_mu.
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits) {
+                    int ld, int splits, bool fp32_class) {
     Atoms &A = c->atoms;
     const int n = A.n, nv = A.n_var();
     double2 *part = c->work_part.p, *tt = c->work_t.p;
-    k_reduce_splits<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(n, nv, nsys, splits, part, d_sys_list, x,
+    const size_t tot = (size_t)nv * nsys;
+    k_reduce_splits<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(n, nv, nsys, splits, part, d_sys_list, x,
                                                                         ld, A.pos.p, A.d_eta.p, tt);
-    k_kbar_q<<<n, 64, 0, c->stream>>>(n, A.nt, A.hetero ? 1 : 0, A.kb_rowptr.p, A.kb_col.p, A.kb_kij.p, A.kb_kji.p,
-                                      A.pos.p, nsys, d_sys_list, d_par, tt, nv, x, ld, y);
-    if (A.model == PXH_WFQFMU)
-        k_mu_rows<<<dim3((3 * n + 255) / 256, nsys), 256, 0, c->stream>>>(n, A.nt, A.d_nbfrac.p, nsys, d_sys_list,
-                                                                          d_par, tt, nv, x, ld, y);
+    const int nblk = (n + 3) / 4;
+    const bool loose = fp32_class && A.kbm_built;
+    k_kbar_q<<<8 * ((nblk + 7) / 8), 256, 0, c->stream>>>(
+        n, A.nt, A.hetero ? 1 : 0, loose ? A.kbm_rowptr.p : A.kb_rowptr.p, loose ? A.kbm_col.p : A.kb_col.p,
+        loose ? A.kbm_kij.p : A.kb_kij.p, loose ? A.kbm_kji.p : A.kb_kji.p, A.pos.p, nsys, d_sys_list, d_par, tt, x,
+        ld, y);
+    if (A.model == PXH_WFQFMU) {
+        const size_t tm = (size_t)3 * n * nsys;
+        k_mu_rows<<<(unsigned)((tm + 255) / 256), 256, 0, c->stream>>>(n, A.nt, A.d_nbfrac.p, nsys, d_sys_list,
+                                                                        d_par, tt, x, ld, y);
+    }
     PXH_HIP(hipGetLastError());
+}
+
+// Squared far-field radius of the Gaussian kernel: beyond it, for every type
+// pair, erfc(kappa r) and (kappa r)^3 exp(-(kappa r)^2) are below half an ulp
+// of the working precision, so the damped elements equal their Coulomb
+// limits to rounding.  kappa r >= 6 (FP64) / 4 (FP32) for T_qq and >= 7 / 5
+// for the charge-dipole and dipole-dipole terms.
+double far_field_r2(const Atoms &A, bool fp64) {
+    if (A.kernel != PXH_GAUSSIAN) return 0.0;
+    const double aq = fp64 ? 6.0 : 4.0, ad = fp64 ? 7.0 : 5.0;
+    double r = 0.0;
+    for (const PairConst &p : A.h_pair) {
+        r = std::max(r, aq / p.inv_rqq);
+        if (A.model == PXH_WFQFMU) r = std::max({r, ad / p.inv_rqmu, ad / p.inv_rmumu});
+    }
+    return r * r;
 }
 
 namespace {
@@ -643,16 +714,17 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     c->work_t.alloc((size_t)nsys * nv);
     double2 *part = c->work_part.p;
     dim3 grid((n + TB - 1) / TB, splits_eff, groups);
+    const double rfar2 = far_field_r2(A, sizeof(R) == 8);
     PXH_HIP(hipEventRecord(c->ev0, c->stream));
     if (fmu)
         launch_tsweep_k<R, KV, true>(A.kernel, grid, c->stream, n, A.nt, A.hetero ? 1 : 0, A.pos.p, A.pair.p, nsys,
-                                     d_sys_list, x, ld, part, nv, j_chunk);
+                                     d_sys_list, x, ld, part, nv, j_chunk, rfar2);
     else
         launch_tsweep_k<R, KV, false>(A.kernel, grid, c->stream, n, A.nt, 0, A.pos.p, A.pair.p, nsys, d_sys_list,
-                                      x, ld, part, nv, j_chunk);
+                                      x, ld, part, nv, j_chunk, rfar2);
     PXH_HIP(hipGetLastError());
     PXH_HIP(hipEventRecord(c->ev1, c->stream));
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, sizeof(R) == 4);
     // time of the pair kernel on the context's stream, and its algorithmic
     // flops: per ordered pair (i != j) kPairEvalFlops for the interaction
     // elements plus kPairVecFlops per complex vector for the contraction
@@ -669,7 +741,7 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     c->stats.v[6] += (double)n * (double)n * groups;
 }
 
-// part[0][k][row] = sum_c T[row, c] x_k[c] with the stored T (q-q diagonal zero).
+// part[row][k] = sum_c T[row, c] x_k[c] with the stored T (q-q diagonal zero).
 __global__ void k_stored_t(int nv, const double *__restrict__ T, int nsys, const int *__restrict__ sys_list,
                            const double2 *__restrict__ x, int ld, double2 *__restrict__ part) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -682,7 +754,7 @@ __global__ void k_stored_t(int nv, const double *__restrict__ T, int nsys, const
         re += t * xs[c].x;
         im += t * xs[c].y;
     }
-    part[(size_t)k * nv + row] = make_double2(re, im);
+    part[(size_t)row * nsys + k] = make_double2(re, im);
 }
 
 }  // namespace
@@ -770,7 +842,12 @@ void build_atoms_device(pxh_ctx *c) {
     build_kbar(c);
 }
 
-void build_kbar(pxh_ctx *c) {
+namespace {
+
+// CSR of K-bar keeping the pairs with 1 - f_ij >= cut for either direction
+// (cut = 0: every pair).  1 - f < cut beyond x = 1 + ln(1/cut)/d, x = r / (s r0).
+void build_kbar_csr(pxh_ctx *c, double cut, DevBuf<int> &rowptr, DevBuf<int> &col, DevBuf<double> &kij,
+                    DevBuf<double> &kji, size_t &nnz) {
     Atoms &A = c->atoms;
     FermiTab ft{};
     for (int t = 0; t < A.nt; ++t) ft.a[t] = A.aij[t];
@@ -778,10 +855,8 @@ void build_kbar(pxh_ctx *c) {
         ft.d[e] = A.fd[e];
         ft.s[e] = A.fs[e];
         ft.r0[e] = A.r0[e];
-        // 1 - f < 1e-18 beyond x = 1 + ln(1e18)/d, x = r / (s r0): those pairs add
-        // less than 1e-18 of the nearest-neighbour coupling and are dropped.
         double rc = 1e300;
-        if (c->opt.kbar_cutoff && A.fd[e] > 0.0) rc = (1.0 + std::log(1.0e18) / A.fd[e]) * A.fs[e] * A.r0[e];
+        if (cut > 0.0 && A.fd[e] > 0.0) rc = (1.0 + std::log(1.0 / cut) / A.fd[e]) * A.fs[e] * A.r0[e];
         ft.rcut2[e] = rc >= 1e150 ? 1e300 : rc * rc;
     }
     DevBuf<int> cnt;
@@ -793,13 +868,31 @@ void build_kbar(pxh_ctx *c) {
     PXH_HIP(hipMemcpyAsync(hc.data(), cnt.p, sizeof(int) * A.n, hipMemcpyDeviceToHost, c->stream));
     PXH_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < A.n; ++i) rp[i + 1] = rp[i] + hc[i];
-    A.kb_nnz = (size_t)rp[A.n];
-    A.kb_rowptr.upload(rp.data(), rp.size(), c->stream);
-    A.kb_col.alloc(std::max<size_t>(1, A.kb_nnz));
-    A.kb_kij.alloc(std::max<size_t>(1, A.kb_nnz));
-    A.kb_kji.alloc(std::max<size_t>(1, A.kb_nnz));
-    k_kbar_fill<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.pos.p, ft, A.kb_rowptr.p, A.kb_col.p,
-                                                              A.kb_kij.p, A.kb_kji.p);
+    nnz = (size_t)rp[A.n];
+    rowptr.upload(rp.data(), rp.size(), c->stream);
+    col.alloc(std::max<size_t>(1, nnz));
+    kij.alloc(std::max<size_t>(1, nnz));
+    kji.alloc(std::max<size_t>(1, nnz));
+    k_kbar_fill<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.pos.p, ft, rowptr.p, col.p, kij.p,
+                                                              kji.p);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+// K-bar for the FP64 products: the pairs with 1 - f < 1e-18 add less than
+// 1e-18 of the nearest-neighbour coupling and are dropped.  The FP32-class
+// products of the inner solves use a second CSR cut at 2^-30 (below their
+// own rounding), about a third of the entries.
+void build_kbar(pxh_ctx *c) {
+    Atoms &A = c->atoms;
+    const bool cut = c->opt.kbar_cutoff != 0;
+    build_kbar_csr(c, cut ? 1e-18 : 0.0, A.kb_rowptr, A.kb_col, A.kb_kij, A.kb_kji, A.kb_nnz);
+    if (cut)
+        build_kbar_csr(c, std::ldexp(1.0, -30), A.kbm_rowptr, A.kbm_col, A.kbm_kij, A.kbm_kji, A.kbm_nnz);
+    A.kbm_built = cut;
+    const int blk = 128;
     A.diag_s1.alloc(A.n);
     A.diag_s2.alloc((size_t)A.n * A.nt);
     k_diag_sums<<<(A.n + blk - 1) / blk, blk, 0, c->stream>>>(A.n, A.nt, A.kernel, A.kb_rowptr.p, A.kb_col.p,
@@ -873,7 +966,7 @@ void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPa
     k_stored_t<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(nv, A.Tstore.p, nsys, d_sys_list, x, ld,
                                                                     c->work_part.p);
     PXH_HIP(hipGetLastError());
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1, false);
 }
 
 void constant_matrix_atoms(pxh_ctx *c, double *d_m) {

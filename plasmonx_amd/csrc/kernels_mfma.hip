@@ -3,19 +3,29 @@
 // t_k = T x_k for up to 16*NCB systems at once.  T (4N x 4N, atom-interleaved
 // rows [q_i, mu_ix, mu_iy, mu_iz]) is the frequency-independent real
 // interaction matrix of src/matrix/matrix_module.f90 (T_qq :58-115,
-// T_qmu/T_muq :120-234, T_mumu :240-345); its 4 x 4 block for a pair (i, j)
-// is generated on the fly in FP32 from the pair's distance vector and
-// stored in LDS as fp16 hi + lo (value * 2^kTScale), then contracted with
-// the block X (4N x 32*NCB real columns: re / im of each system, also fp16
-// hi + lo with a power-of-two scale per column) by
+// T_qmu/T_muq :120-234, T_mumu :240-345).  Its 4 x 4 block for a pair (i, j)
+// is a function of five pair scalars (T_qq, the two charge-dipole radial
+// factors, the two dipole-dipole coefficients) and the distance vector d:
+//     row q:    [ T_qq,      g1 d_x,                g1 d_y,   g1 d_z ]
+//     row mu_a: [ -g2 d_a,   c1 d_a d_b + c0 delta_ab  (b = x, y, z) ]
+// The scalars are generated on the fly in FP32 (one work-item per pair,
+// stored in LDS), the elements are built in registers directly in the MFMA
+// A-operand layout, split into fp16 hi + lo (value * 2^tscale), and
+// contracted with the block X (4N x 32*NCB real columns: re / im of each
+// system, also fp16 hi + lo with a power-of-two scale per column) by
 // v_mfma_f32_32x32x16_f16: acc += Ahi Bhi + Ahi Blo + Alo Bhi, a product
-// with a 22-bit significand, accumulated in FP32 within a source split and
-// in FP64 across splits.
+// with a 22-bit significand, accumulated in FP32 for a few source steps and
+// in FP64 beyond.
+//
+// Beyond the far-field radius the Gaussian damping (erf -> 1, exp -> 0) is
+// below FP32 resolution and the scalars take their Coulomb limits, so the
+// erf / exp evaluations run only for the near pairs of a wavefront.
 //
 // Workgroup: 256 work-items (4 waves), 32 target atoms (128 rows); wave w
 // owns rows [32w, 32w+32) for all 32*NCB columns.  Each source step stages
-// 16 atoms (K = 64): every work-item evaluates 2 pair blocks into LDS, then
-// each wave runs 4 k-steps x NCB column blocks x 3 MFMAs.
+// 16 atoms (K = 64): every work-item evaluates the scalars of 2 pairs, then
+// each wave builds its A fragments and runs 4 k-steps x NCB column blocks x
+// 3 MFMAs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,10 +38,9 @@ namespace pxh {
 namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef float float8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kTScale = 12;        // T elements are stored as T * 2^12
 constexpr int kXTarget = 13;       // columns are scaled so that max |x| * s lies in [2^12, 2^13)
 constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 constexpr double kFourOverSqrtPi = 2.2567583341910252;
@@ -95,22 +104,62 @@ __global__ void k_prep_x(int n, int nsys, const int *__restrict__ sys_list, cons
     }
 }
 
+// Charge-dipole radial factor (erf(a) - 2a/sqrt(pi) exp(-a^2)) / r^3.
+__device__ __forceinline__ float qmu_factor(float a, float inv_r3) {
+    return (erff(a) - (float)kTwoOverSqrtPi * a * __expf(-a * a)) * inv_r3;
+}
+
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+// Split of a float pair into fp16 hi + lo: hi = RN16(v), lo = RN16(v - hi).
+// v - hi is exact in FP32; v_fma_mix_f32 reads hi as fp16 directly.
+__device__ __forceinline__ void split2(float v0, float v1, half2v &hi, half2v &lo) {
+    hi = __builtin_convertvector((float2v){v0, v1}, half2v);
+    float r0, r1;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r0) : "v"(hi), "v"(v0));
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r1) : "v"(hi), "v"(v1));
+    lo = __builtin_convertvector((float2v){r0, r1}, half2v);
+}
+
+__device__ __forceinline__ void split4(float e0, float e1, float e2, float e3, half4 &hi, half4 &lo) {
+    half2v h0, l0, h1, l1;
+    split2(e0, e1, h0, l0);
+    split2(e2, e3, h1, l1);
+    hi = (half4){h0[0], h0[1], h1[0], h1[1]};
+    lo = (half4){l0[0], l0[1], l1[0], l1[1]};
+}
+
+// Workgroup: 256 work-items (4 waves), 32 target atoms; wave a owns the 32
+// rows of component a (0: q, 1-3: mu_x, mu_y, mu_z) of those atoms, for all
+// 32*NCB columns.  Per source step (16 atoms, K = 64):
+//   1. every work-item evaluates the 4 x 4 blocks of 2 pairs, splits them
+//      into fp16 hi / lo and stores row a of each block in wave a's
+//      A-fragment layout in LDS: [a][source pair jp = jl / 2][target il][jl % 2][4];
+//   2. each wave runs 4 k-steps x NCB column blocks x 3 MFMAs with A and B
+//      fragments read straight from LDS.
+// Blocks are placed so that all workgroups of one source split share an XCD
+// (split = linear block id mod splits): with splits a multiple of 8 an XCD's
+// L2 holds the X fragments of its split.
 template <int NCB>
-__global__ __launch_bounds__(256) void k_tsweep_mfma(int n, int nt, int hetero, const double4 *__restrict__ pos,
-                                                     const PairConst *__restrict__ pc, int nsys,
-                                                     const float *__restrict__ scale,
-                                                     const _Float16 *__restrict__ xh, const _Float16 *__restrict__ xl,
-                                                     double2 *__restrict__ part, int nv, int jsteps_per_split,
-                                                     int nsys_total_in_part, int sys_offset) {
-    constexpr int NC = 32 * NCB;
-    __shared__ half8 sAh[4 * 4 * 64];  // [wave][k-step][lane] (16 KB)
-    __shared__ half8 sAl[4 * 4 * 64];
+__global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int hetero, float rfar2, float tsc,
+                                                        const double4 *__restrict__ pos,
+                                                        const PairConst *__restrict__ pc, int nsys,
+                                                        const float *__restrict__ scale,
+                                                        const _Float16 *__restrict__ xh,
+                                                        const _Float16 *__restrict__ xl, double2 *__restrict__ part,
+                                                        int nv, int splits, int jsteps_per_split,
+                                                        int nsys_total_in_part, int sys_offset) {
     __shared__ half8 sBh[4 * NCB * 64];  // [k-step][cb][lane]
     __shared__ half8 sBl[4 * NCB * 64];
+    __shared__ half8 sAh[4 * 8 * kAtomsI];  // [a][jp][il] (half8 = rows a of pairs 2jp, 2jp+1)
+    __shared__ half8 sAl[4 * 8 * kAtomsI];
     __shared__ float spc[kMaxTypes * kMaxTypes][4];
-    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const int i0 = blockIdx.x * kAtomsI;
-    const int split = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int split = blockIdx.x % splits;
+    const int i0 = (blockIdx.x / splits) * kAtomsI;
     const int jb0 = split * jsteps_per_split;
     const int jb_end = min((n + kAtomsJ - 1) / kAtomsJ, jb0 + jsteps_per_split);
     for (int e = tid; e < nt * nt; e += 256) {
@@ -119,7 +168,7 @@ __global__ __launch_bounds__(256) void k_tsweep_mfma(int n, int nt, int hetero, 
         spc[e][2] = (float)pc[e].inv_rmumu;
         spc[e][3] = (float)pc[e].eta_bar;
     }
-    // the pair blocks this work-item evaluates: target il, sources jl0 and jl0 + 8
+    // evaluation role: target il, sources jl0 and jl0 + 8
     const int il = tid % 32;
     const int jl0 = tid / 32;
     const int gi = i0 + il;
@@ -134,108 +183,117 @@ __global__ __launch_bounds__(256) void k_tsweep_mfma(int n, int nt, int hetero, 
             acc[cb][v] = 0.f;
             accd[cb][v] = 0.0;
         }
-    const float tsc = ldexpf(1.f, kTScale);
-    int since_flush = 0;
-    for (int jb = jb0; jb < jb_end; ++jb) {
-        __syncthreads();
-        // ---- stage X fragments of this source step (contiguous in HBM)
-        {
-            const size_t base = (size_t)jb * 4 * NCB * 64;
-            for (int e = tid; e < 4 * NCB * 64; e += 256) {
-                sBh[e] = *reinterpret_cast<const half8 *>(xh + (base + e) * 8);
-                sBl[e] = *reinterpret_cast<const half8 *>(xl + (base + e) * 8);
-            }
+    // operands of the next source step are fetched into registers one step
+    // ahead, so that their latency hides behind the MFMA block
+    half8 nbh[NCB], nbl[NCB];  // X fragments: elements tid + 256 r of the step
+    double4 npos[2];           // positions of sources jl0 and jl0 + 8
+    auto fetch = [&](int jb) {
+        const size_t base = (size_t)jb * 4 * NCB * 64 + tid;
+#pragma unroll
+        for (int r = 0; r < NCB; ++r) {
+            nbh[r] = *reinterpret_cast<const half8 *>(xh + (base + 256 * r) * 8);
+            nbl[r] = *reinterpret_cast<const half8 *>(xl + (base + 256 * r) * 8);
         }
-        // ---- evaluate two 4 x 4 pair blocks into the A fragments
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            const int jl = jl0 + 8 * p;
-            const int gj = jb * kAtomsJ + jl;
-            float blk[4][4];
+            const int gj = jb * kAtomsJ + jl0 + 8 * p;
+            npos[p] = gj < n ? pos[gj] : make_double4(0, 0, 0, 0);
+        }
+    };
+    if (jb0 < jb_end) fetch(jb0);
+    // source steps in blocks of kStepsPerFlush: FP32 accumulation inside a
+    // block (the accumulators stay in the MFMA registers), FP64 across blocks
+    for (int jbb = jb0; jbb < jb_end; jbb += kStepsPerFlush) {
+        const int jbe = min(jb_end, jbb + kStepsPerFlush);
+        for (int jb = jbb; jb < jbe; ++jb) {
+            __syncthreads();
+            // ---- X fragments of this source step to LDS (4 * NCB * 64 half8 = 256 NCB)
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int r = 0; r < NCB; ++r) {
+                sBh[tid + 256 * r] = nbh[r];
+                sBl[tid + 256 * r] = nbl[r];
+            }
+            // ---- 4 x 4 blocks of (il, jl0) and (il, jl0 + 8), fp16 hi / lo to LDS
 #pragma unroll
-                for (int b = 0; b < 4; ++b) blk[a][b] = 0.f;
-            if (gi < n && gj < n && gi != gj) {
-                const double4 pj = pos[gj];
-                const int tj = (int)pj.w;
-                const float dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float inv_r = rsqrtf(r2);
-                const float r = r2 * inv_r;
-                const float *pp = spc[ti + nt * tj];
-                const float tqq = erff(r * pp[0]) * inv_r;
-                const float inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
-                const float a1 = r * pp[1];
-                const float g1 = (erff(a1) - (float)kTwoOverSqrtPi * a1 * __expf(-a1 * a1)) * inv_r3;
-                float g2 = g1;
-                if (hetero) {
-                    const float a2 = r * spc[tj + nt * ti][1];
-                    g2 = (erff(a2) - (float)kTwoOverSqrtPi * a2 * __expf(-a2 * a2)) * inv_r3;
+            for (int p = 0; p < 2; ++p) {
+                const int jl = jl0 + 8 * p;
+                const int gj = jb * kAtomsJ + jl;
+                float tqq = 0.f, g1 = 0.f, g2 = 0.f, c0 = 0.f, c1 = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+                if (gi < n && gj < n && gi != gj) {
+                    const double4 pj = npos[p];
+                    dx = (float)(pi.x - pj.x);
+                    dy = (float)(pi.y - pj.y);
+                    dz = (float)(pi.z - pj.z);
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const float inv_r = rsqrtf(r2);
+                    const float inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
+                    tqq = inv_r;
+                    g1 = g2 = c0 = inv_r3;
+                    c1 = -3.f * inv_r3 * inv_r2;
+                    if (r2 < rfar2) {
+                        const int tj = (int)pj.w;
+                        const float r = r2 * inv_r;
+                        const float *pp = spc[ti + nt * tj];
+                        tqq = erff(r * pp[0]) * inv_r;
+                        g1 = qmu_factor(r * pp[1], inv_r3);
+                        g2 = hetero ? qmu_factor(r * spc[tj + nt * ti][1], inv_r3) : g1;
+                        const float a3 = r * pp[2];
+                        const float e3 = __expf(-a3 * a3);
+                        const float fd1 = erff(a3) - (float)kTwoOverSqrtPi * a3 * e3;
+                        const float fd2 = (float)kFourOverSqrtPi * pp[2] * pp[2] * pp[2] * e3;
+                        c0 = fd1 * inv_r3;
+                        c1 = fd2 * inv_r2 - 3.f * fd1 * inv_r3 * inv_r2;
+                    }
+                    tqq *= tsc;
+                    g1 *= tsc;
+                    g2 *= -tsc;
+                    c0 *= tsc;
+                    c1 *= tsc;
                 }
-                const float a3 = r * pp[2];
-                const float e3 = __expf(-a3 * a3);
-                const float fd1 = erff(a3) - (float)kTwoOverSqrtPi * a3 * e3;
-                const float fd2 = (float)kFourOverSqrtPi * pp[2] * pp[2] * pp[2] * e3;
-                const float c0 = fd1 * inv_r3;
-                const float c1 = fd2 * inv_r2 - 3.f * fd1 * inv_r3 * inv_r2;
-                const float d[3] = {dx, dy, dz};
-                blk[0][0] = tqq;
+                const float ux = c1 * dx, uy = c1 * dy, uz = c1 * dz;
+                const float xy = ux * dy, xz = ux * dz, yz = uy * dz;
+                half4 h[4], l[4];
+                split4(tqq, g1 * dx, g1 * dy, g1 * dz, h[0], l[0]);
+                split4(g2 * dx, fmaf(ux, dx, c0), xy, xz, h[1], l[1]);
+                split4(g2 * dy, xy, fmaf(uy, dy, c0), yz, h[2], l[2]);
+                split4(g2 * dz, xz, yz, fmaf(uz, dz, c0), h[3], l[3]);
 #pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    blk[0][1 + b] = g1 * d[b];   // T_qmu(i, j)
-                    blk[1 + b][0] = -g2 * d[b];  // T_muq(i, j) = T_qmu(j, i)
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) blk[1 + a][1 + b] = c1 * d[a] * d[b] + (a == b ? c0 : 0.f);
+                for (int a = 0; a < 4; ++a) {
+                    const int idx = (a * 8 + jl / 2) * kAtomsI + il;
+                    reinterpret_cast<half4 *>(&sAh[idx])[jl % 2] = h[a];
+                    reinterpret_cast<half4 *>(&sAl[idx])[jl % 2] = l[a];
                 }
             }
-            // rows 4 il + a, columns 4 jl + b of the 128 x 64 A tile
-            const int s = jl / 4, h0 = 4 * (jl % 4) % 8, khalf = (4 * (jl % 4)) / 8;
+            if (jb + 1 < jb_end) fetch(jb + 1);
+            __syncthreads();
+            // ---- 4 k-steps x NCB column blocks x 3 products; A row = target lane % 32,
+            // k = 8 (lane / 32) + h = component h % 4 of source 2 (lane / 32) + h / 4
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int r = 4 * il + a;
-                const int w = r / 32, lr = (khalf * 32) + (r % 32);
-                half4 hv, lv;
+            for (int s = 0; s < 4; ++s) {
+                const int ia = (wave * 8 + 2 * s + lane / 32) * kAtomsI + lane % 32;
+                const half8 ah = sAh[ia];
+                const half8 al = sAl[ia];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const float v = blk[a][b] * tsc;
-                    const _Float16 hi = (_Float16)v;
-                    hv[b] = hi;
-                    lv[b] = (_Float16)(v - (float)hi);
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const half8 bh = sBh[(s * NCB + cb) * 64 + lane];
+                    const half8 bl = sBl[(s * NCB + cb) * 64 + lane];
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[cb], 0, 0, 0);
+                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[cb], 0, 0, 0);
                 }
-                const int idx = (w * 4 + s) * 64 + lr;
-                *reinterpret_cast<half4 *>(reinterpret_cast<_Float16 *>(&sAh[idx]) + h0) = hv;
-                *reinterpret_cast<half4 *>(reinterpret_cast<_Float16 *>(&sAl[idx]) + h0) = lv;
             }
         }
-        __syncthreads();
-        // ---- contraction: 4 k-steps x NCB column blocks x 3 products
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const half8 ah = sAh[(wave * 4 + s) * 64 + lane];
-            const half8 al = sAl[(wave * 4 + s) * 64 + lane];
+        for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const half8 bh = sBh[(s * NCB + cb) * 64 + lane];
-                const half8 bl = sBl[(s * NCB + cb) * 64 + lane];
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[cb], 0, 0, 0);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[cb], 0, 0, 0);
-                acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[cb], 0, 0, 0);
+            for (int v = 0; v < 16; ++v) {
+                accd[cb][v] += (double)acc[cb][v];
+                acc[cb][v] = 0.f;
             }
-        }
-        if (++since_flush == kStepsPerFlush) {
-            since_flush = 0;
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb)
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    accd[cb][v] += (double)acc[cb][v];
-                    acc[cb][v] = 0.f;
-                }
-        }
     }
-    // ---- write partials: acc[cb][v] -> row m = (v/4)*8 + (lane/32)*4 + v%4 of the wave, column cb*32 + lane%32
-    const double tinv = ldexp(1.0, -kTScale);
+    // ---- write partials: acc[cb][v] -> target atom m = (v/4)*8 + (lane/32)*4 + v%4 of the
+    // workgroup, component = wave, column cb*32 + lane%32
+    const double tinv = 1.0 / (double)tsc;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
         const int col = cb * 32 + (lane % 32);
@@ -244,15 +302,12 @@ __global__ __launch_bounds__(256) void k_tsweep_mfma(int n, int nt, int hetero, 
         const double cs = scale[col] > 0.f ? tinv / (double)scale[col] : 0.0;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const int m = (v / 4) * 8 + (lane / 32) * 4 + (v % 4);
-            const int rr = wave * 32 + m;  // row in the 128-row tile
-            const int i = i0 + rr / 4, a = rr % 4;
+            const int i = i0 + (v / 4) * 8 + (lane / 32) * 4 + (v % 4);
             if (i >= n) continue;
-            const int row = a == 0 ? i : n + 3 * i + a - 1;
-            const double val = (accd[cb][v] + (double)acc[cb][v]) * cs;
+            const int row = wave == 0 ? i : n + 3 * i + wave - 1;
             double *dst = reinterpret_cast<double *>(
-                part + ((size_t)split * nsys_total_in_part + sys_offset + sys) * nv + row);
-            dst[col & 1] = val;
+                part + ((size_t)split * nv + row) * nsys_total_in_part + sys_offset + sys);
+            dst[col & 1] = accd[cb][v] * cs;
         }
     }
 }
@@ -260,29 +315,49 @@ __global__ __launch_bounds__(256) void k_tsweep_mfma(int n, int nt, int hetero, 
 }  // namespace
 
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits);
+                    int ld, int splits, bool fp32_class);
+
+// Power-of-two scale of the T elements: the largest Gaussian-damped element
+// of any type pair (T_qq <= 2 kappa_qq / sqrt(pi), |g d| <= 0.45 kappa_qmu^2,
+// |c1 d d + c0| <= kappa_mumu^3) times the scale stays below 2^14, so that
+// fp16 holds it with margin and small far-field elements stay normal.
+float mfma_t_scale(const Atoms &A) {
+    double bound = 1e-30;
+    for (const PairConst &p : A.h_pair)
+        bound = std::max({bound, 1.13 * p.inv_rqq, 0.45 * p.inv_rqmu * p.inv_rqmu,
+                          p.inv_rmumu * p.inv_rmumu * p.inv_rmumu});
+    const int e = std::min(20, std::max(0, (int)std::floor(std::log2(16384.0 / bound))));
+    return std::ldexp(1.f, e);
+}
 
 // MFMA path of the FP32-class product for omegaFQFmu (Gaussian kernel).
 void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld) {
-    constexpr int NCB = 3;  // 48 systems per pass; LDS 57 KB per workgroup
+    constexpr int NCB = 3;  // 48 systems per pass
     constexpr int SYS_PER_GROUP = 16 * NCB;
     Atoms &A = c->atoms;
     const int n = A.n, nv = A.n_var();
     const int jsteps = (n + kAtomsJ - 1) / kAtomsJ;
     const int iblocks = (n + kAtomsI - 1) / kAtomsI;
     const int groups = (nsys + SYS_PER_GROUP - 1) / SYS_PER_GROUP;
-    // splits so that a launch has >= ~1500 workgroups, each split >= 16 source steps
-    int splits = c->opt.splits > 0 ? c->opt.splits : std::max(1, 1536 / std::max(1, iblocks));
-    splits = std::min(splits, std::max(1, jsteps / 16));
+    // source splits: a multiple of 8 (one XCD per split residue) once every
+    // split still spans >= 16 source steps, and >= ~2048 workgroups
+    int splits = c->opt.splits;
+    if (splits <= 0) {
+        splits = std::max(1, (2048 + iblocks - 1) / iblocks);
+        if (jsteps >= 8 * 16) splits = std::min((splits + 7) / 8 * 8, jsteps / 16 / 8 * 8);
+        splits = std::max(1, std::min(splits, jsteps / 16));
+    }
+    splits = std::min(splits, jsteps);
     const int per = (jsteps + splits - 1) / splits;
-    splits = (jsteps + per - 1) / per;
     c->work_part.alloc((size_t)splits * nsys * nv);
     c->work_t.alloc((size_t)nsys * nv);
     const size_t xelems = (size_t)jsteps * 4 * NCB * 64 * 8;
     c->work_xh.alloc(xelems);
     c->work_xl.alloc(xelems);
     c->work_scale.alloc(32 * NCB);
+    const float tsc = mfma_t_scale(A);
+    const float rfar2 = (float)far_field_r2(A, false);
     PXH_HIP(hipEventRecord(c->ev0, c->stream));
     for (int g = 0; g < groups; ++g) {
         const int ns = std::min(SYS_PER_GROUP, nsys - g * SYS_PER_GROUP);
@@ -290,13 +365,13 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         k_col_scale<<<32 * NCB, 256, 0, c->stream>>>(nv, ns, lst, x, ld, c->work_scale.p, 32 * NCB);
         k_prep_x<NCB><<<jsteps, 256, 0, c->stream>>>(n, ns, lst, x, ld, c->work_scale.p, c->work_xh.p,
                                                      c->work_xl.p);
-        k_tsweep_mfma<NCB><<<dim3(iblocks, splits), 256, 0, c->stream>>>(
-            n, A.nt, A.hetero ? 1 : 0, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p, c->work_xl.p,
-            c->work_part.p, nv, per, nsys, g * SYS_PER_GROUP);
+        k_tsweep_mfma<NCB><<<iblocks * splits, 256, 0, c->stream>>>(
+            n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p,
+            c->work_xl.p, c->work_part.p, nv, splits, per, nsys, g * SYS_PER_GROUP);
     }
     PXH_HIP(hipGetLastError());
     PXH_HIP(hipEventRecord(c->ev1, c->stream));
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true);
     PXH_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     PXH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));

@@ -87,6 +87,11 @@ struct Atoms {
     DevBuf<int> kb_rowptr, kb_col;
     DevBuf<double> kb_kij, kb_kji;      // K_ij = (1-f_ij) A_i / r, K_ji = (1-f_ji) A_j / r
     size_t kb_nnz = 0;
+    // looser K-bar CSR (1 - f >= 2^-30) for the FP32-class products
+    DevBuf<int> kbm_rowptr, kbm_col;
+    DevBuf<double> kbm_kij, kbm_kji;
+    size_t kbm_nnz = 0;
+    bool kbm_built = false;
     DevBuf<double> Tstore;              // stored T for the in-memory method (n_var^2)
     // diagonal of K-bar T_qq: S1_i = sum_j K_ij (T_qq(j,i) - eta_i), and for
     // alloys S2_i[t] = sum_{j of type t} K_ji (T_qq(j,i) - eta_i)
@@ -168,6 +173,9 @@ void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPa
 void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld);
 void prepare_stored_t(pxh_ctx *c);
+// squared radius beyond which the Gaussian damping is below FP64 / FP32 rounding
+double far_field_r2(const Atoms &A, bool fp64);
+float mfma_t_scale(const Atoms &A);
 void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag);
 void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs /* nv x 3 */);
 void constant_matrix_atoms(pxh_ctx *c, double *d_m);

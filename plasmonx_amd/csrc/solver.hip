@@ -5,7 +5,8 @@
 //
 // Batched layout in HBM: slot-major.  Slot s owns
 //   X[s], B[s], R[s], W[s]            n_var complex each,
-//   V[s][0..m]                        the Krylov basis (FP64),
+//   V[s][0..m]                        the Krylov basis (FP64; complex FP32 in
+//                                     the inner solves of the mixed path),
 //   H[s] (m+1) x m, g[s] (m+1), cs[s], sn[s] (m)   Hessenberg and rotations.
 // Dots and norms are reduced in a fixed order (one workgroup per dot), so
 // a system's result does not depend on which other systems share the batch.
@@ -16,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "pxh_internal.h"
 
@@ -30,6 +32,21 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {  // conj(a) * b
     return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Krylov basis element: FP64 (double2), or complex FP32 (float2) for the
+// inner solves of the mixed path, whose operator is FP32-class anyway.
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+template <typename VT>
+__device__ __forceinline__ VT from_d2(double2 v);
+template <>
+__device__ __forceinline__ double2 from_d2<double2>(double2 v) {
+    return v;
+}
+template <>
+__device__ __forceinline__ float2 from_d2<float2>(double2 v) {
+    return make_float2((float)v.x, (float)v.y);
 }
 
 __device__ __forceinline__ double2 block_sum2(double2 v) {
@@ -73,19 +90,110 @@ __global__ __launch_bounds__(RB) void k_dots(int nv, const int *__restrict__ slo
     if (threadIdx.x == 0) h[(size_t)s * ldh + i] = acc;
 }
 
-// W[s] -= sum_{i < jcount[s]} V[s][i] h[s][i]  (and optionally hacc[s][i] += h[s][i]).
+// Batched dots in two stages: one wavefront per (slot, chunk of kDotRows rows)
+// computes the chunk's partial sums for every basis vector, k_dots_sum adds
+// the chunks in a fixed order (deterministic), W is read once per chunk.
+constexpr int kDotRows = 512;
+constexpr int kDotGroup = 8;  // basis vectors per pass over the chunk
+
+// part[(s ldh + i) nch + ch] = sum_{r in chunk ch} conj(V[s][i][r]) W[s][r], i < jcount[s]
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dots_part(int nv, const int *__restrict__ slots,
+                                                   const int *__restrict__ jcount, const VT *__restrict__ V,
+                                                   size_t ldv_slot, const double2 *__restrict__ W,
+                                                   double2 *__restrict__ part, int ldh, int nch) {
+    constexpr int U = kDotRows / 64;
+    const int ch = blockIdx.x * 4 + threadIdx.x / 64;
+    const int s = slots[blockIdx.y];
+    const int lane = threadIdx.x % 64;
+    if (ch >= nch) return;
+    const int jc = jcount[s];
+    const int r0 = ch * kDotRows + lane;
+    double2 w[U];
+    const double2 *ws = W + (size_t)nv * s;
+#pragma unroll
+    for (int u = 0; u < U; ++u) w[u] = r0 + 64 * u < nv ? ws[r0 + 64 * u] : make_double2(0, 0);
+    {
+        const int i0 = blockIdx.z * kDotGroup;  // one group of basis vectors per workgroup layer
+        if (i0 >= jc) return;
+        double2 acc[kDotGroup];
+#pragma unroll
+        for (int g = 0; g < kDotGroup; ++g) {
+            acc[g] = make_double2(0, 0);
+            if (i0 + g < jc) {
+                const VT *v = V + ldv_slot * s + (size_t)nv * (i0 + g);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (r0 + 64 * u < nv) {
+                        const double2 p = cconjmul(to_d2(v[r0 + 64 * u]), w[u]);
+                        acc[g].x += p.x;
+                        acc[g].y += p.y;
+                    }
+                }
+            }
+        }
+        // transposing reduction of the 8 sums over the wave: each exchange
+        // halves the values a lane holds (10 shuffles instead of 48); lane l
+        // ends with the sum of g = 4 b5 + 2 b4 + b3 (bits of l) when l % 8 == 0
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool up = lane & 32;
+            const double2 snd = up ? acc[q] : acc[q + 4];
+            const double2 keep = up ? acc[q + 4] : acc[q];
+            acc[q] = make_double2(keep.x + __shfl_xor(snd.x, 32, 64), keep.y + __shfl_xor(snd.y, 32, 64));
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bool up = lane & 16;
+            const double2 snd = up ? acc[q] : acc[q + 2];
+            const double2 keep = up ? acc[q + 2] : acc[q];
+            acc[q] = make_double2(keep.x + __shfl_xor(snd.x, 16, 64), keep.y + __shfl_xor(snd.y, 16, 64));
+        }
+        {
+            const bool up = lane & 8;
+            const double2 snd = up ? acc[0] : acc[1];
+            const double2 keep = up ? acc[1] : acc[0];
+            acc[0] = make_double2(keep.x + __shfl_xor(snd.x, 8, 64), keep.y + __shfl_xor(snd.y, 8, 64));
+        }
+#pragma unroll
+        for (int off = 4; off > 0; off >>= 1) {
+            acc[0].x += __shfl_xor(acc[0].x, off, 64);
+            acc[0].y += __shfl_xor(acc[0].y, off, 64);
+        }
+        const int g = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+        if ((lane & 7) == 0 && i0 + g < jc) part[((size_t)s * ldh + i0 + g) * nch + ch] = acc[0];
+    }
+}
+
+// h[s][i] = sum_ch part[s][i][ch]
+__global__ void k_dots_sum(const int *__restrict__ slots, const int *__restrict__ jcount,
+                           const double2 *__restrict__ part, int ldh, int nch, double2 *__restrict__ h) {
+    const int s = slots[blockIdx.x];
+    for (int i = threadIdx.x; i < jcount[s]; i += blockDim.x) {
+        const double2 *p = part + ((size_t)s * ldh + i) * nch;
+        double2 acc = make_double2(0, 0);
+        for (int ch = 0; ch < nch; ++ch) {
+            acc.x += p[ch].x;
+            acc.y += p[ch].y;
+        }
+        h[(size_t)s * ldh + i] = acc;
+    }
+}
+
+// W[s] -= sum_{i < jcount[s]} V[s][i] h[s][i]
+template <typename VT>
 __global__ void k_gs_update(int nv, const int *__restrict__ slots, const int *__restrict__ jcount,
-                            const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ h, int ldh,
+                            const VT *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ h, int ldh,
                             double2 *__restrict__ W) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
     if (r >= nv) return;
     const int jc = jcount[s];
-    const double2 *v = V + ldv_slot * s + r;
+    const VT *v = V + ldv_slot * s + r;
     const double2 *hs = h + (size_t)s * ldh;
     double2 acc = make_double2(0, 0);
     for (int i = 0; i < jc; ++i) {
-        const double2 p = cmul(v[(size_t)nv * i], hs[i]);
+        const double2 p = cmul(to_d2(v[(size_t)nv * i]), hs[i]);
         acc.x += p.x;
         acc.y += p.y;
     }
@@ -119,16 +227,17 @@ __global__ __launch_bounds__(RB) void k_norm(int nv, const int *__restrict__ slo
 }
 
 // V[s][j] = W[s] / nrm[s]   (nothing when nrm == 0, as gmres_module.f90:148-151)
+template <typename VT>
 __global__ void k_scale_into(int nv, const int *__restrict__ slots, const int *__restrict__ jcol,
                              const double2 *__restrict__ W, size_t ldw, const double *__restrict__ nrm,
-                             double2 *__restrict__ V, size_t ldv_slot) {
+                             VT *__restrict__ V, size_t ldv_slot) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
     if (r >= nv) return;
     const double d = nrm[s];
     if (d == 0.0) return;
     const double2 w = W[ldw * s + r];
-    V[ldv_slot * s + (size_t)nv * jcol[s] + r] = make_double2(w.x / d, w.y / d);
+    V[ldv_slot * s + (size_t)nv * jcol[s] + r] = from_d2<VT>(make_double2(w.x / d, w.y / d));
 }
 
 // R[s] = B[s] - W[s]
@@ -227,18 +336,19 @@ __global__ __launch_bounds__(64) void k_trsv(const int *__restrict__ slots, cons
 }
 
 // X[s] (+)= V[s][0:jlen] y[s]
+template <typename VT>
 __global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__restrict__ jlen,
-                           const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ y, int ldy,
+                           const VT *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ y, int ldy,
                            const double2 *__restrict__ dinv, double2 *__restrict__ X, int accumulate) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
     if (r >= nv) return;
     const int n = jlen[s];
-    const double2 *v = V + ldv_slot * s + r;
+    const VT *v = V + ldv_slot * s + r;
     const double2 *ys = y + (size_t)s * ldy;
     double2 acc = make_double2(0, 0);
     for (int i = 0; i < n; ++i) {
-        const double2 p = cmul(v[(size_t)nv * i], ys[i]);
+        const double2 p = cmul(to_d2(v[(size_t)nv * i]), ys[i]);
         acc.x += p.x;
         acc.y += p.y;
     }
@@ -253,20 +363,22 @@ __global__ void k_update_x(int nv, const int *__restrict__ slots, const int *__r
 }
 
 // Operator input of each slot: stage[s] = V[s][j_s] (times D^-1 when preconditioned).
+template <typename VT>
 __global__ void k_stage(int nv, const int *__restrict__ slots, const int *__restrict__ jcol,
-                        const double2 *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ dinv,
+                        const VT *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ dinv,
                         double2 *__restrict__ stage) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
     if (r >= nv) return;
-    double2 v = V[ldv_slot * s + (size_t)nv * jcol[s] + r];
+    double2 v = to_d2(V[ldv_slot * s + (size_t)nv * jcol[s] + r]);
     if (dinv) v = cmul(dinv[(size_t)nv * s + r], v);
     stage[(size_t)nv * s + r] = v;
 }
 
 // Start a Krylov cycle: V[s][0] = R[s] / beta[s], g[s] = (beta, 0, ...).
+template <typename VT>
 __global__ void k_start_cycle(int nv, const int *__restrict__ slots, const double2 *__restrict__ R,
-                              const double *__restrict__ beta, double2 *__restrict__ V, size_t ldv_slot,
+                              const double *__restrict__ beta, VT *__restrict__ V, size_t ldv_slot,
                               double2 *__restrict__ g, int ldg) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
@@ -274,7 +386,7 @@ __global__ void k_start_cycle(int nv, const int *__restrict__ slots, const doubl
     if (r < ldg) g[(size_t)s * ldg + r] = make_double2(r == 0 ? b : 0.0, 0.0);
     if (r >= nv) return;
     const double2 v = R[(size_t)nv * s + r];
-    V[ldv_slot * s + r] = make_double2(v.x / b, v.y / b);
+    V[ldv_slot * s + r] = from_d2<VT>(make_double2(v.x / b, v.y / b));
 }
 
 // Copy a right-hand side (column k of rhs, nv x 3) into slot s of B and R, zero X.
@@ -298,7 +410,7 @@ struct Batched {
     pxh_ctx *c;
     int nv, m, B;
     bool mixed, plain;
-    DevBuf<double2> X, Bv, R, W, V, H, g, hbuf, hacc, y, sn;
+    DevBuf<double2> X, Bv, R, W, H, g, hbuf, hacc, y, sn, dpart;
     DevBuf<double> cs, nrm, resid;
     DevBuf<int> slots_d, jcount_d, jp1_d, jl_d;
     DevBuf<double2> stage;
@@ -350,11 +462,14 @@ static void apply_op(Batched &S, int nsl, bool fp64, const double2 *x, double2 *
 }
 
 // Solve all (ifreq, pol) systems.  rhs_all: device, [n_freq][nv x 3].
-static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
-                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
-                       double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
-                       std::vector<double> &tres_out) {
+// VT: element type of the Krylov basis.
+template <typename VT>
+static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
+                         int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
+                         double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
+                         std::vector<double> &tres_out) {
     Batched S;
+    DevBuf<VT> V;
     S.c = c;
     S.nv = nv;
     S.mixed = precision == PXH_MIXED;
@@ -372,7 +487,9 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
     S.Bv.alloc((size_t)nv * B);
     S.R.alloc((size_t)nv * B);
     S.W.alloc((size_t)nv * B);
-    S.V.alloc(S.ldv_slot * B);
+    V.alloc(S.ldv_slot * B);
+    const int nch = (nv + kDotRows - 1) / kDotRows;
+    S.dpart.alloc((size_t)B * S.ldh * nch);
     S.H.alloc(S.ldH_slot * B);
     S.g.alloc((size_t)S.ldg * B);
     S.hbuf.alloc((size_t)S.ldh * B);
@@ -501,8 +618,8 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
             }
             if (!restart.empty()) {
                 upload_list(S, restart);
-                k_start_cycle<<<dim3((std::max(nv, S.ldg) + 255) / 256, (int)restart.size()), 256, 0, st>>>(
-                    nv, S.slots_d.p, S.R.p, S.nrm.p, S.V.p, S.ldv_slot, S.g.p, S.ldg);
+                k_start_cycle<VT><<<dim3((std::max(nv, S.ldg) + 255) / 256, (int)restart.size()), 256, 0, st>>>(
+                    nv, S.slots_d.p, S.R.p, S.nrm.p, V.p, S.ldv_slot, S.g.p, S.ldg);
                 for (int s : restart) {
                     phase[s] = INNER;
                     S.jcount[s] = 0;
@@ -523,8 +640,8 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
         // W = A V_j: V_j of each slot is staged at stage + nv * slot (the operator's layout)
         S.h_jc = S.jcount;
         PXH_HIP(hipMemcpyAsync(S.jcount_d.p, S.h_jc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
-        k_stage<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, S.V.p, S.ldv_slot,
-                                              S.precond ? S.dinv.p : nullptr, S.stage.p);
+        k_stage<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot,
+                                                  S.precond ? S.dinv.p : nullptr, S.stage.p);
         apply_op(S, na, !S.mixed, S.stage.p, S.W.p);
         if (S.mixed)
             c->stats.v[1] += 1;
@@ -536,29 +653,35 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
         PXH_HIP(hipMemcpyAsync(S.jp1_d.p, S.h_jp1.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
         int *jp1_dp = S.jp1_d.p;
         const bool mgs = plain && (jmax + 1 < 25);
-        if (mgs) {
-            // modified Gram-Schmidt (gmres_module.f90:123-127); plain path has one slot
-            for (int i = 0; i <= jmax; ++i) {
-                k_dots<<<dim3(1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, i, S.V.p, S.ldv_slot, S.W.p,
-                                                   S.hbuf.p, S.ldh);
-                k_axpy_col<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, i, S.V.p, S.ldv_slot, S.hbuf.p, S.ldh,
-                                                         S.W.p);
+        if constexpr (std::is_same<VT, double2>::value) {
+            if (mgs) {
+                // modified Gram-Schmidt (gmres_module.f90:123-127); plain path has one slot
+                for (int i = 0; i <= jmax; ++i) {
+                    k_dots<<<dim3(1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, i, V.p, S.ldv_slot, S.W.p,
+                                                       S.hbuf.p, S.ldh);
+                    k_axpy_col<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, i, V.p, S.ldv_slot, S.hbuf.p,
+                                                             S.ldh, S.W.p);
+                }
             }
-        } else {
+        }
+        if (!mgs) {
             // classical Gram-Schmidt with one re-orthogonalisation (:128-142)
-            k_dots<<<dim3(jmax + 1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, 0, S.V.p, S.ldv_slot, S.W.p,
-                                                      S.hbuf.p, S.ldh);
-            k_gs_update<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.V.p, S.ldv_slot, S.hbuf.p,
-                                                      S.ldh, S.W.p);
-            k_dots<<<dim3(jmax + 1, na), RB, 0, st>>>(nv, S.slots_d.p, jp1_dp, 0, S.V.p, S.ldv_slot, S.W.p,
-                                                      S.hacc.p, S.ldh);
-            k_gs_update<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.V.p, S.ldv_slot, S.hacc.p,
-                                                      S.ldh, S.W.p);
+            const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDotGroup - 1) / kDotGroup);
+            k_dots_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
+                                                S.ldh, nch);
+            k_dots_sum<<<na, 128, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hbuf.p);
+            k_gs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.hbuf.p,
+                                                          S.ldh, S.W.p);
+            k_dots_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
+                                                S.ldh, nch);
+            k_dots_sum<<<na, 128, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hacc.p);
+            k_gs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.hacc.p,
+                                                          S.ldh, S.W.p);
             k_add_h<<<na, 64, 0, st>>>(S.slots_d.p, jp1_dp, S.hbuf.p, S.hacc.p, S.ldh);
         }
         k_norm<<<na, RB, 0, st>>>(nv, S.slots_d.p, S.W.p, (size_t)nv, S.nrm.p);
-        k_scale_into<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p, S.V.p,
-                                                   S.ldv_slot);
+        k_scale_into<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p, V.p,
+                                                       S.ldv_slot);
         k_givens<<<(na + 63) / 64, 64, 0, st>>>(na, S.slots_d.p, S.jcount_d.p, S.hbuf.p, S.ldh, S.nrm.p, S.H.p,
                                                 S.ldH, S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg, S.resid.p);
         PXH_HIP(hipGetLastError());
@@ -578,7 +701,7 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
             int *jl_dp = S.jl_d.p;
             k_trsv<<<(int)fin.size(), 64, 0, st>>>(S.slots_d.p, jl_dp, S.H.p, S.ldH, S.ldH_slot, S.g.p, S.ldg,
                                                    S.y.p, S.ldh);
-            k_update_x<<<grid_rows((int)fin.size()), 256, 0, st>>>(nv, S.slots_d.p, jl_dp, S.V.p, S.ldv_slot,
+            k_update_x<VT><<<grid_rows((int)fin.size()), 256, 0, st>>>(nv, S.slots_d.p, jl_dp, V.p, S.ldv_slot,
                                                                   S.y.p, S.ldh, S.precond ? S.dinv.p : nullptr,
                                                                   S.X.p, 1);
             PXH_HIP(hipGetLastError());
@@ -587,6 +710,16 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
     }
     PXH_HIP(hipStreamSynchronize(st));
     return status;
+}
+
+static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
+                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
+                       double2 *d_sol, std::vector<int> &iters_out, std::vector<double> &tres_out) {
+    if (precision == PXH_MIXED && !plain)
+        return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
+                                     iters_out, tres_out);
+    return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
+                                  iters_out, tres_out);
 }
 
 // polar[a + 3 b] = sum_i var_b(i) x_a(i) / E0 (+ dipoles); BEM: area-weighted for DPCM.
