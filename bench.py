@@ -29,7 +29,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+FP16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense fp16 MFMA peak (the pair kernel's arithmetic unit)
 
 
 def log(*a):
@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--check", type=int, default=3, help="frequencies checked against FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--traffic", action="store_true",
+                    help="run one 48-system pair-kernel launch and a 256 MiB device copy (for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE), then exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -69,6 +72,9 @@ def main():
     ctx.set_atoms(s)
     ctx.set_alpha(aw)
     ctx.set_option("batch", a.batch)
+    if a.traffic:
+        traffic_run(ctx, s, freq)
+        return
 
     def solve(fr_idx, precision=_lib.MIXED, tol_abs=tol, pol=7, kdim=a.krylov):
         ctx.set_alpha(aw[:, fr_idx])
@@ -115,14 +121,21 @@ def main():
                  "iters_mean": float(pr["iters"].mean())}
         log(f"[bench] plain FP64 GPU path: {plain['value']:.3f} freq-points/s ({tpl:.1f}s), iters {pr['iters'].tolist()}")
 
-    # roofline of the dominant kernel: the FP32 pair sweep
-    ms = st["sweep_ms_fp32"]
-    launches = st["sweep_launches_fp32"]
+    # roofline of the dominant kernel: the MFMA pair sweep.  Algorithmic flops
+    # per launch: the real 4N x 4N T applied to 2 real columns per system (2 flops
+    # per element and column, 64 N(N-1) per system) plus 42 flops per ordered
+    # pair for the elements; divided by the kernel's own time (HIP events on
+    # the library's stream around each launch, summed over the timed step).
+    ms = st["mfma_kernel_ms"]
+    launches = st["mfma_launches"]
     achieved = st["sweep_flops_fp32"] / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "k_tsweep<float,4,FMU,gaussian>", "launches": int(launches),
-            "avg_launch_ms": round(ms / max(launches, 1), 3)}
+    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "kernel": "k_tsweep_mfma<3> (fp16 hi/lo split, 3 MFMA products per element pair)",
+            "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 3)}
+    latency = {"p50_ms": round(st["latency_p50_ms"], 1), "p99_ms": round(st["latency_p99_ms"], 1),
+               "definition": "per frequency point: admission of its first field to convergence of its last, "
+                             "201 points queued at t=0 into 48 solver slots"}
 
     cpu = None
     if not a.no_cpu:
@@ -142,11 +155,41 @@ def main():
         "plain_fp64_gpu": plain,
         "speedup_vs_plain_fp64": round(value / plain["value"], 2) if plain else None,
         "iters_mean": float(out["iters"].mean()),
+        "latency": latency,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+# HBM bytes per launch of the pair kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3
+# counters on the 48-system product, scaled by the same counters on a copy of
+# known size); None until measured.
+TRAFFIC_BYTES_PER_LAUNCH = 697.8e6
+
+
+def traffic_run(ctx, s, freq):
+    """Calibration copy of known size (a 256 MiB device-to-device hipMemcpy:
+    reads and writes 256 MiB), then one mixed product of 48 systems: exactly
+    one launch of the MFMA pair kernel."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the HIP runtime libpxh is linked against
+    size = 256 * 1024 * 1024
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(size)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(size), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(48, s.n_var)) + 1j * rng.normal(size=(48, s.n_var))
+    ctx.set_option("matvec_mixed", 1)
+    ctx.matvec(freq[len(freq) // 2], len(freq) // 2, x)
+    ctx.set_option("matvec_mixed", 0)
+    log("[bench] traffic run: 256 MiB copy + one 48-system product done")
 
 
 def cpu_baseline(s, freq, aw, iters_per_system):

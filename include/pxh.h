@@ -159,7 +159,10 @@ int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
  * products, [1] FP32 block products, [2] FP64 block products, [3] / [4]
  * pair-kernel ms (FP32 / FP64, HIP events on the context's stream),
  * [5] wall ms, [6] pair evaluations, [7] outer refinements, [8] / [9]
- * pair-kernel launches (FP32 / FP64), [10] / [11] their algorithmic flops. */
+ * pair-kernel launches (FP32 / FP64), [10] / [11] their algorithmic flops,
+ * [12] / [13] ms and launches of the MFMA pair kernel alone, [14] / [15]
+ * p50 / p99 service time of a frequency point in ms (admission of its first
+ * field direction to convergence of its last; iterative methods). */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
 #ifdef __cplusplus

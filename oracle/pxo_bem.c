@@ -5,17 +5,16 @@
  *
  * TEST INFRASTRUCTURE ONLY (see pxo.h).  Not linked into the product.
  *
- * What pins these functions (tests/test_oracle_bem.py):
+ * What pins these functions (tests/test_oracle_golden.py,
+ * tests/test_oracle_invariants.py):
  *   - bem_dpcm_accurate_inversion and bem_ief_*_inversion reference logs:
  *     RHS (E10.3), D + lambda I real/imag (E11.4), charges, polarizabilities
- *     and cross sections (E14.6) on the 284-tessera matlab.msh mesh;
- *   - Legendre-Gauss-Lobatto nodes and weights against their closed forms
- *     (roots of (1-x^2) P'_{N}(x); weights 2/(N(N+1) P_N(x)^2));
- *   - the 28-point triangle rule integrating polynomials up to degree 11
- *     exactly, the polar rule integrating a constant to the tessera area;
+ *     and cross sections (E14.6) on the 284-tessera matlab.msh mesh; the
+ *     quadrature (Legendre-Gauss-Lobatto polar rule, 28-point Strang-Fix
+ *     near-field rule) enters the printed D and is pinned through it;
  *   - the accurate-mode sum rule sum_k A_k D_ki = 2 pi A_i (exact by
  *     construction of the diagonal) and the quasi-static sphere limit
- *     alpha -> R^3 (eps - eps_s)/(eps + 2 eps_s) on fine icospheres.
+ *     alpha -> R^3 (eps - 1)/(eps + 2) on an icosphere (within 2 %).
  */
 #include "pxo.h"
 

@@ -182,4 +182,5 @@ class Context:
         return {"matvecs": s[0], "fp32_products": s[1], "fp64_products": s[2], "sweep_ms_fp32": s[3],
                 "sweep_ms_fp64": s[4], "wall_ms": s[5], "pair_evaluations": s[6], "outer": s[7],
                 "sweep_launches_fp32": s[8], "sweep_launches_fp64": s[9], "sweep_flops_fp32": s[10],
-                "sweep_flops_fp64": s[11]}
+                "sweep_flops_fp64": s[11], "mfma_kernel_ms": s[12], "mfma_launches": s[13],
+                "latency_p50_ms": s[14], "latency_p99_ms": s[15]}

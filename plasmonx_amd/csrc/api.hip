@@ -56,6 +56,8 @@ int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out) {
         PXH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         PXH_HIP(hipEventCreate(&c->ev0));
         PXH_HIP(hipEventCreate(&c->ev1));
+        PXH_HIP(hipEventCreate(&c->ev2));
+        PXH_HIP(hipEventCreate(&c->ev3));
     });
     if (st != PXH_OK) {
         delete c;
@@ -71,6 +73,8 @@ int pxh_ctx_destroy(pxh_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PXH_OK;

@@ -135,7 +135,7 @@ struct pxh_ctx {
     int device = 0;
     size_t budget = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     pxh::Atoms atoms;
     pxh::Mesh mesh;
     int target = 0;                       // PXH_WFQ / PXH_WFQFMU / PXH_BEM

@@ -467,9 +467,11 @@ template <typename VT>
 static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
                          int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
                          double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
-                         std::vector<double> &tres_out) {
+                         std::vector<double> &tres_out, std::vector<double> &t_admit, std::vector<double> &t_done) {
     Batched S;
     DevBuf<VT> V;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
     S.c = c;
     S.nv = nv;
     S.mixed = precision == PXH_MIXED;
@@ -526,6 +528,8 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     size_t next = 0, done = 0;
     iters_out.assign(systems.size(), 0);
     tres_out.assign(systems.size(), 0.0);
+    t_admit.assign(systems.size(), 0.0);
+    t_done.assign(systems.size(), 0.0);
     const double inner_rel = S.mixed ? c->opt.inner_tol : 0.0;
     auto grid_rows = [&](int nsl) { return dim3((nv + 255) / 256, nsl); };
     std::vector<double> hbeta(B);
@@ -533,6 +537,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
 
     auto admit = [&](int s) {
         const System &sy = systems[next];
+        t_admit[next] = now_s();
         sys_of[s] = (int)next++;
         if (c->target == PXH_BEM) {
             const double *e = c->eps.data() + 2 * (size_t)sy.ifreq;
@@ -596,6 +601,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                     // converged (or zero RHS: the solution is zero, gmres_module.f90:84-85)
                     tres_out[id] = rn;
                     iters_out[id] = total_it[s];
+                    t_done[id] = now_s();
                     const System &sy = systems[id];
                     PXH_HIP(hipMemcpyAsync(d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
                                            S.X.p + (size_t)nv * s, sizeof(double2) * nv, hipMemcpyDeviceToDevice,
@@ -607,6 +613,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                 if (outer[s] >= c->opt.max_outer) {
                     tres_out[id] = rn;
                     iters_out[id] = total_it[s];
+                    t_done[id] = now_s();
                     status = PXH_ENOCONV;
                     phase[s] = FREE;
                     ++done;
@@ -714,12 +721,13 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
 
 static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
                        int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
-                       double2 *d_sol, std::vector<int> &iters_out, std::vector<double> &tres_out) {
+                       double2 *d_sol, std::vector<int> &iters_out, std::vector<double> &tres_out,
+                       std::vector<double> &t_admit, std::vector<double> &t_done) {
     if (precision == PXH_MIXED && !plain)
         return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
-                                     iters_out, tres_out);
+                                     iters_out, tres_out, t_admit, t_done);
     return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
-                                  iters_out, tres_out);
+                                  iters_out, tres_out, t_admit, t_done);
 }
 
 // polar[a + 3 b] = sum_i var_b(i) x_a(i) / E0 (+ dipoles); BEM: area-weighted for DPCM.
@@ -833,7 +841,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         }
     }
     std::vector<int> it_sys;
-    std::vector<double> tr_sys;
+    std::vector<double> tr_sys, t_adm, t_fin;
     std::vector<System> systems;
     for (int f = 0; f < n_freq; ++f)
         for (int k = 0; k < 3; ++k)
@@ -852,7 +860,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         try {
             status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, c->use_stored ? PXH_FP64
                                                                                                         : precision,
-                                 plain, sol.p, it_sys, tr_sys);
+                                 plain, sol.p, it_sys, tr_sys, t_adm, t_fin);
         } catch (...) {
             c->use_stored = false;
             c->atoms.Tstore.release();
@@ -860,6 +868,23 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         }
         c->use_stored = false;
         c->atoms.Tstore.release();
+    }
+    // service time of each frequency point: admission of its first field to
+    // convergence of its last (iterative paths), p50 / p99 in ms
+    if (!t_fin.empty()) {
+        std::vector<double> a(n_freq, 1e300), d(n_freq, 0.0), lat;
+        for (size_t k = 0; k < systems.size(); ++k) {
+            a[systems[k].ifreq] = std::min(a[systems[k].ifreq], t_adm[k]);
+            d[systems[k].ifreq] = std::max(d[systems[k].ifreq], t_fin[k]);
+        }
+        for (int f = 0; f < n_freq; ++f)
+            if (a[f] < 1e299) lat.push_back(1e3 * (d[f] - a[f]));
+        std::sort(lat.begin(), lat.end());
+        if (!lat.empty()) {
+            auto pct = [&](double q) { return lat[std::min(lat.size() - 1, (size_t)std::floor(q * (lat.size() - 1) + 0.5))]; };
+            c->stats.v[14] = pct(0.50);
+            c->stats.v[15] = pct(0.99);
+        }
     }
     // polarizabilities and results
     const int nb = 64;

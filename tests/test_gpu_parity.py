@@ -193,3 +193,22 @@ def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
     for k in range(nvec):
         err = np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k])
         assert err < 2e-5, (k, err)
+
+
+@pytest.mark.parametrize("case", ["bem_ief_accurate_inversion", "bem_ief_approximate_inversion"])
+def test_bem_ief_iterative_agrees_with_inversion(gpu_ctx, case):
+    """The iterative IEF applies the full operator -(D + lambda I) S A^-1, so
+    GMRES reaches the inversion's answer (the reference's iterative IEF omits
+    S A^-1, algorithm_module.f90:356-358)."""
+    fx = load_fixture(case)
+    m = mesh_from_fixture(fx)
+    freq = np.array(fx["freq_au"])[:4]
+    eps = np.array([complex(a, b) for a, b in fx["eps_w"]])[:4] if "eps_w" in fx else pm.silver_etchegoin(freq)
+    gpu_ctx.set_mesh(m)
+    gpu_ctx.set_permittivity(eps)
+    gpu_ctx.set_option("batch", 1)
+    tol = 1e-11 * np.sqrt(m.n)
+    got = gpu_ctx.solve_spectrum(m.n, freq, fx["e0"], method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=600)
+    ref = pxo.bem_solve_spectrum(m, freq, eps, fx["e0"], method=0)
+    assert np.all(got["true_residual"] < tol)
+    assert relerr(got["polar"], ref["polar"]) < 1e-6

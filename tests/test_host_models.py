@@ -1,0 +1,43 @@
+"""Host-side models and workload generators: closed forms and library
+routines pin them (no GPU)."""
+import numpy as np
+import pytest
+from scipy.special import wofz
+
+from plasmonx_amd import geometry, permittivity as pm
+from plasmonx_amd.workloads import c1_sodium_sphere, c2_silver_sphere
+
+
+def test_faddeeva_matches_scipy_wofz():
+    rng = np.random.default_rng(11)
+    z = rng.uniform(-8, 8, 400) + 1j * rng.uniform(1e-3, 6, 400)
+    assert np.abs(pm.faddeeva(z) - wofz(z)).max() < 1e-10 * np.abs(wofz(z)).max()
+
+
+def test_brendel_bormann_reduces_to_lorentz_as_broadening_vanishes():
+    """With the Gaussian width s -> 0 a Brendel-Bormann oscillator becomes the
+    Lorentz term f wp^2 / (w0^2 - w^2 - i g w); the Drude term is
+    -f0 wp^2 / (w (w + i g0))."""
+    ev = np.linspace(1.5, 2.5, 21)  # away from w0 = 3 eV, where the limit is smooth
+    freq = ev / 27.211386245981
+    f, w0, g, wp, f0, g0 = 0.5, 3.0, 0.2, 9.0, 0.8, 0.05
+    eps = pm.brendel_bormann(freq, 1.0, wp, f0, g0, [(f, w0, g, 1e-4)])
+    ref = 1.0 - f0 * wp ** 2 / (ev * (ev + 1j * g0)) + f * wp ** 2 / (w0 ** 2 - ev ** 2 - 1j * g * ev)
+    assert np.abs(eps - ref).max() < 1e-6 * np.abs(ref).max()
+
+
+def test_workload_sizes():
+    assert len(c1_sodium_sphere()[0].symbols) == 1745
+    s, f = c2_silver_sphere(n_freq=201)
+    assert s.n_atoms == 15683 and s.n_var == 62732 and len(f) == 201
+    assert geometry.icosphere(100.0, 4, seed=0).shape == (5120, 3, 3)
+
+
+def test_icosphere_is_closed_and_outward():
+    v = geometry.icosphere(10.0, 3, seed=0)
+    c = v.mean(axis=1)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    assert np.all((n * c).sum(axis=1) > 0)
+    # divergence theorem: sum of (area vector . centroid) / 3 = enclosed volume
+    vol = (n * c).sum() / 6.0
+    assert vol == pytest.approx(4 / 3 * np.pi * 10.0 ** 3, rel=0.02)
