@@ -152,7 +152,9 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * tolerance of the low-precision inner solves), "precond" (diagonal right
  * preconditioning of the batched path), "mfma" (omegaFQFmu low-precision
  * products on MFMA), "matvec_mixed" (pxh_matvec uses the low-precision
- * product), "splits", "max_outer". */
+ * product), "splits", "max_outer", "reorth_kappa" (batched classical
+ * Gram-Schmidt: second pass only when the first leaves less than kappa of
+ * the norm; default 1/sqrt(2), >= 1 = always). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
 /* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector

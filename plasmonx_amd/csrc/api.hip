@@ -316,6 +316,9 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.precond = value != 0.0;
         } else if (k == "max_outer") {
             c->opt.max_outer = std::max(1, (int)value);
+        } else if (k == "reorth_kappa") {
+            PXH_CHECK(value >= 0, PXH_EINVAL, "reorth_kappa must be >= 0");
+            c->opt.reorth_kappa = value;
         } else {
             throw Error{PXH_EINVAL, "unknown option: " + k};
         }

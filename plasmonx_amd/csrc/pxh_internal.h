@@ -123,6 +123,7 @@ struct Options {
     int precond = 1;         // diagonal right preconditioning in the batched path
     int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
     int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
+    double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
 };
 
 struct Stats {

@@ -101,7 +101,8 @@ template <typename VT>
 __global__ __launch_bounds__(256) void k_dots_part(int nv, const int *__restrict__ slots,
                                                    const int *__restrict__ jcount, const VT *__restrict__ V,
                                                    size_t ldv_slot, const double2 *__restrict__ W,
-                                                   double2 *__restrict__ part, int ldh, int nch) {
+                                                   double2 *__restrict__ part, int ldh, int nch,
+                                                   double *__restrict__ wn_part) {
     constexpr int U = kDotRows / 64;
     const int ch = blockIdx.x * 4 + threadIdx.x / 64;
     const int s = slots[blockIdx.y];
@@ -113,6 +114,14 @@ __global__ __launch_bounds__(256) void k_dots_part(int nv, const int *__restrict
     const double2 *ws = W + (size_t)nv * s;
 #pragma unroll
     for (int u = 0; u < U; ++u) w[u] = r0 + 64 * u < nv ? ws[r0 + 64 * u] : make_double2(0, 0);
+    if (wn_part && blockIdx.z == 0) {  // chunk partial of ||W[s]||^2
+        double q = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) q += w[u].x * w[u].x + w[u].y * w[u].y;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+        if (lane == 0) wn_part[(size_t)s * nch + ch] = q;
+    }
     {
         const int i0 = blockIdx.z * kDotGroup;  // one group of basis vectors per workgroup layer
         if (i0 >= jc) return;
@@ -165,41 +174,94 @@ __global__ __launch_bounds__(256) void k_dots_part(int nv, const int *__restrict
     }
 }
 
-// h[s][i] = sum_ch part[s][i][ch]
-__global__ void k_dots_sum(const int *__restrict__ slots, const int *__restrict__ jcount,
-                           const double2 *__restrict__ part, int ldh, int nch, double2 *__restrict__ h) {
+// h[s][i] = sum_ch part[s][i][ch]; one wavefront per (i, slot), fixed reduction tree
+__global__ __launch_bounds__(256) void k_dots_sum(const int *__restrict__ slots, const int *__restrict__ jcount,
+                                                  const double2 *__restrict__ part, int ldh, int nch,
+                                                  double2 *__restrict__ h) {
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
+    const int s = slots[blockIdx.y];
+    const int lane = threadIdx.x % 64;
+    if (i >= jcount[s]) return;
+    const double2 *p = part + ((size_t)s * ldh + i) * nch;
+    double2 acc = make_double2(0, 0);
+    for (int ch = lane; ch < nch; ch += 64) {
+        acc.x += p[ch].x;
+        acc.y += p[ch].y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, 64);
+        acc.y += __shfl_xor(acc.y, off, 64);
+    }
+    if (lane == 0) h[(size_t)s * ldh + i] = acc;
+}
+
+// out[s] = sum_b part[s][b] (sqrt when root); one wavefront per slot, only
+// slots with gate[s] != 0 when gate is given
+__global__ __launch_bounds__(64) void k_sum_slot(const int *__restrict__ slots, const double *__restrict__ part,
+                                                 int nb, const int *__restrict__ gate, int root,
+                                                 double *__restrict__ out) {
     const int s = slots[blockIdx.x];
-    for (int i = threadIdx.x; i < jcount[s]; i += blockDim.x) {
-        const double2 *p = part + ((size_t)s * ldh + i) * nch;
-        double2 acc = make_double2(0, 0);
-        for (int ch = 0; ch < nch; ++ch) {
-            acc.x += p[ch].x;
-            acc.y += p[ch].y;
-        }
-        h[(size_t)s * ldh + i] = acc;
+    if (gate && gate[s] == 0) return;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 64) acc += part[(size_t)s * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) out[s] = root ? sqrt(acc) : acc;
+}
+
+// After the first Gram-Schmidt pass: ||w'|| from the update's partials, and
+// the second pass only where the first removed more than (1 - kappa) of the
+// norm (||w'|| < kappa ||w||, "twice is enough"); kappa >= 1: always.
+__global__ __launch_bounds__(64) void k_reorth_decide(const int *__restrict__ slots, const int *__restrict__ jcount,
+                                                      const double *__restrict__ npart, int nb,
+                                                      const double *__restrict__ wn2, double kappa,
+                                                      int *__restrict__ jc2, double *__restrict__ nrm) {
+    const int s = slots[blockIdx.x];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 64) acc += npart[(size_t)s * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) {
+        const bool again = kappa >= 1.0 || acc < kappa * kappa * wn2[s];
+        jc2[s] = again ? jcount[s] : 0;
+        nrm[s] = sqrt(acc);
     }
 }
 
-// W[s] -= sum_{i < jcount[s]} V[s][i] h[s][i]
+// W[s] -= sum_{i < jcount[s]} V[s][i] h[s][i]; with npart, also the
+// per-workgroup partial sums of ||W[s]||^2 after the update.
 template <typename VT>
-__global__ void k_gs_update(int nv, const int *__restrict__ slots, const int *__restrict__ jcount,
-                            const VT *__restrict__ V, size_t ldv_slot, const double2 *__restrict__ h, int ldh,
-                            double2 *__restrict__ W) {
+__global__ __launch_bounds__(256) void k_gs_update(int nv, const int *__restrict__ slots,
+                                                   const int *__restrict__ jcount, const VT *__restrict__ V,
+                                                   size_t ldv_slot, const double2 *__restrict__ h, int ldh,
+                                                   double2 *__restrict__ W, double *__restrict__ npart) {
+    __shared__ double sq[4];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slots[blockIdx.y];
-    if (r >= nv) return;
     const int jc = jcount[s];
-    const VT *v = V + ldv_slot * s + r;
-    const double2 *hs = h + (size_t)s * ldh;
-    double2 acc = make_double2(0, 0);
-    for (int i = 0; i < jc; ++i) {
-        const double2 p = cmul(to_d2(v[(size_t)nv * i]), hs[i]);
-        acc.x += p.x;
-        acc.y += p.y;
+    if (jc == 0) return;
+    double q = 0.0;
+    if (r < nv) {
+        const VT *v = V + ldv_slot * s + r;
+        const double2 *hs = h + (size_t)s * ldh;
+        double2 acc = make_double2(0, 0);
+        for (int i = 0; i < jc; ++i) {
+            const double2 p = cmul(to_d2(v[(size_t)nv * i]), hs[i]);
+            acc.x += p.x;
+            acc.y += p.y;
+        }
+        double2 &wv = W[(size_t)nv * s + r];
+        wv.x -= acc.x;
+        wv.y -= acc.y;
+        q = wv.x * wv.x + wv.y * wv.y;
     }
-    double2 &wv = W[(size_t)nv * s + r];
-    wv.x -= acc.x;
-    wv.y -= acc.y;
+    if (!npart) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+    if (threadIdx.x % 64 == 0) sq[threadIdx.x / 64] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[(size_t)s * gridDim.x + blockIdx.x] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
 }
 
 // Single-vector MGS step: W[s] -= h V[s][i] with h = hbuf[s][i].
@@ -279,30 +341,50 @@ __device__ void zrot_d(double2 &x, double2 &y, double c, double2 s) {
     x = t;
 }
 
-// Givens update of column j of H[s] (gmres_module.f90:153-167), one thread per slot.
+// Givens update of column j of H[s] (gmres_module.f90:153-167), one
+// workgroup per slot: the column and the previous rotations are staged in LDS
+// by all lanes, lane 0 applies the rotations in order, all lanes write back.
 // hcol holds the Gram-Schmidt coefficients h_0..h_j, hn the new norm.
-__global__ void k_givens(int nsl, const int *__restrict__ slots, const int *__restrict__ jcol,
-                         const double2 *__restrict__ hcol, int ldh, const double *__restrict__ hn,
-                         double2 *__restrict__ H, int ldH, size_t ldH_slot, double2 *__restrict__ g, int ldg,
-                         double *__restrict__ cs, double2 *__restrict__ sn, int ldcs, double *__restrict__ resid) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsl) return;
-    const int s = slots[k];
+__global__ __launch_bounds__(64) void k_givens(const int *__restrict__ slots, const int *__restrict__ jcol,
+                                               const double2 *__restrict__ hcol, int ldh,
+                                               const double *__restrict__ hn, double2 *__restrict__ H, int ldH,
+                                               size_t ldH_slot, double2 *__restrict__ g, int ldg,
+                                               double *__restrict__ cs, double2 *__restrict__ sn, int ldcs,
+                                               double *__restrict__ resid) {
+    extern __shared__ double2 sh_g[];  // col[ldH] | sn[ldcs] | cs[ldcs] (doubles)
+    double2 *col = sh_g;
+    double2 *snl = sh_g + ldH;
+    double *csl = reinterpret_cast<double *>(sh_g + ldH + ldcs);
+    const int s = slots[blockIdx.x];
     const int j = jcol[s];
+    const int t = threadIdx.x;
     double2 *Hc = H + ldH_slot * s + (size_t)ldH * j;
     const double2 *hs = hcol + (size_t)s * ldh;
-    for (int i = 0; i <= j; ++i) Hc[i] = hs[i];
-    Hc[j + 1] = make_double2(hn[s], 0.0);
     double *css = cs + (size_t)s * ldcs;
     double2 *sns = sn + (size_t)s * ldcs;
-    for (int i = 0; i < j; ++i) zrot_d(Hc[i], Hc[i + 1], css[i], sns[i]);
-    double2 a = Hc[j];
-    zrotg_d(a, Hc[j + 1], css[j], sns[j]);
-    double2 *gs = g + (size_t)s * ldg;
-    zrot_d(gs[j], gs[j + 1], css[j], sns[j]);
-    zrot_d(Hc[j], Hc[j + 1], css[j], sns[j]);
-    Hc[j + 1] = make_double2(0, 0);
-    resid[s] = hypot(gs[j + 1].x, gs[j + 1].y);
+    for (int i = t; i <= j; i += 64) col[i] = hs[i];
+    for (int i = t; i < j; i += 64) {
+        csl[i] = css[i];
+        snl[i] = sns[i];
+    }
+    if (t == 0) col[j + 1] = make_double2(hn[s], 0.0);
+    __syncthreads();
+    if (t == 0) {
+        for (int i = 0; i < j; ++i) zrot_d(col[i], col[i + 1], csl[i], snl[i]);
+        double2 a = col[j];
+        double c;
+        double2 sv;
+        zrotg_d(a, col[j + 1], c, sv);
+        css[j] = c;
+        sns[j] = sv;
+        double2 *gs = g + (size_t)s * ldg;
+        zrot_d(gs[j], gs[j + 1], c, sv);
+        zrot_d(col[j], col[j + 1], c, sv);
+        col[j + 1] = make_double2(0, 0);
+        resid[s] = hypot(gs[j + 1].x, gs[j + 1].y);
+    }
+    __syncthreads();
+    for (int i = t; i <= j + 1; i += 64) Hc[i] = col[i];
 }
 
 // Back substitution H(0:j,0:j) y = g(0:j) for each slot (one workgroup per slot).
@@ -411,6 +493,8 @@ struct Batched {
     int nv, m, B;
     bool mixed, plain;
     DevBuf<double2> X, Bv, R, W, H, g, hbuf, hacc, y, sn, dpart;
+    DevBuf<double> wpart, npart, wn2;  // norm partials of the Gram-Schmidt passes
+    DevBuf<int> jc2_d;                 // columns of the second pass per slot (0 = skipped)
     DevBuf<double> cs, nrm, resid;
     DevBuf<int> slots_d, jcount_d, jp1_d, jl_d;
     DevBuf<double2> stage;
@@ -492,6 +576,10 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     V.alloc(S.ldv_slot * B);
     const int nch = (nv + kDotRows - 1) / kDotRows;
     S.dpart.alloc((size_t)B * S.ldh * nch);
+    S.wpart.alloc((size_t)B * nch);
+    S.npart.alloc((size_t)B * ((nv + 255) / 256));
+    S.wn2.alloc(B);
+    S.jc2_d.alloc(B);
     S.H.alloc(S.ldH_slot * B);
     S.g.alloc((size_t)S.ldg * B);
     S.hbuf.alloc((size_t)S.ldh * B);
@@ -672,24 +760,34 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             }
         }
         if (!mgs) {
-            // classical Gram-Schmidt with one re-orthogonalisation (:128-142)
+            // classical Gram-Schmidt (:128-142); the second pass runs for every
+            // slot on the plain path (the reference's CGS2) and where the first
+            // pass lost more than 1 - 1/sqrt(2) of the norm on the batched path
             const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDotGroup - 1) / kDotGroup);
+            const dim3 gs((jmax + 1 + 3) / 4, na);
+            const int nb = (nv + 255) / 256;
             k_dots_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
-                                                S.ldh, nch);
-            k_dots_sum<<<na, 128, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hbuf.p);
+                                                S.ldh, nch, S.wpart.p);
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hbuf.p);
+            k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.wpart.p, nch, nullptr, 0, S.wn2.p);
             k_gs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.hbuf.p,
-                                                          S.ldh, S.W.p);
-            k_dots_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
-                                                S.ldh, nch);
-            k_dots_sum<<<na, 128, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hacc.p);
-            k_gs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.hacc.p,
-                                                          S.ldh, S.W.p);
-            k_add_h<<<na, 64, 0, st>>>(S.slots_d.p, jp1_dp, S.hbuf.p, S.hacc.p, S.ldh);
+                                                          S.ldh, S.W.p, S.npart.p);
+            k_reorth_decide<<<na, 64, 0, st>>>(S.slots_d.p, jp1_dp, S.npart.p, nb, S.wn2.p,
+                                               plain ? 2.0 : c->opt.reorth_kappa, S.jc2_d.p, S.nrm.p);
+            k_dots_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, S.jc2_d.p, V.p, S.ldv_slot, S.W.p, S.dpart.p,
+                                                S.ldh, nch, nullptr);
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, S.jc2_d.p, S.dpart.p, S.ldh, nch, S.hacc.p);
+            k_gs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jc2_d.p, V.p, S.ldv_slot,
+                                                          S.hacc.p, S.ldh, S.W.p, S.npart.p);
+            k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.npart.p, nb, S.jc2_d.p, 1, S.nrm.p);
+            k_add_h<<<na, 64, 0, st>>>(S.slots_d.p, S.jc2_d.p, S.hbuf.p, S.hacc.p, S.ldh);
+        } else {
+            k_norm<<<na, RB, 0, st>>>(nv, S.slots_d.p, S.W.p, (size_t)nv, S.nrm.p);
         }
-        k_norm<<<na, RB, 0, st>>>(nv, S.slots_d.p, S.W.p, (size_t)nv, S.nrm.p);
         k_scale_into<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p, V.p,
                                                        S.ldv_slot);
-        k_givens<<<(na + 63) / 64, 64, 0, st>>>(na, S.slots_d.p, S.jcount_d.p, S.hbuf.p, S.ldh, S.nrm.p, S.H.p,
+        k_givens<<<na, 64, sizeof(double2) * (S.ldH + S.ldg) + sizeof(double) * S.ldg, st>>>(
+            S.slots_d.p, S.jcount_d.p, S.hbuf.p, S.ldh, S.nrm.p, S.H.p,
                                                 S.ldH, S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg, S.resid.p);
         PXH_HIP(hipGetLastError());
         std::vector<double> hres(B);
