@@ -167,6 +167,39 @@ int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
  * field direction to convergence of its last; iterative methods). */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
+/* ------------------------------------------------------------------ serving
+ * A request loop on one device (no counterpart in the reference, which
+ * solves one structure per process; admission by HBM footprint is the check
+ * of memory_manager_module.f90:394-397 applied per request).  A request is
+ * one pxh_solve_spectrum on a context the caller has prepared with
+ * pxh_set_atoms / pxh_set_mesh (+ alpha / permittivity tables).  Requests are
+ * admitted in arrival order as their footprint fits the budget and their
+ * context is idle (a smaller later request may pass a larger waiting one);
+ * up to max_inflight run at once, each on its own context's stream, so their
+ * work overlaps on the device.  A context must not be used by the caller
+ * while a request on it is outstanding.  A result is identical alone and in
+ * the mix. */
+typedef struct pxh_server pxh_server;
+
+/* hbm_budget_bytes = 0: 90 % of the HBM free at creation. */
+int pxh_server_create(int device, size_t hbm_budget_bytes, int max_inflight, pxh_server **out);
+
+/* Queue a solve (arguments as pxh_solve_spectrum; outputs come from
+ * pxh_server_wait).  The frequency list is copied. */
+int pxh_server_submit(pxh_server *srv, pxh_ctx *ctx, int n_freq, const double *freq_au, double e0, int rhs_form,
+                      int pol_mask, int method, double tol_abs, int krylov_dim, int precision, int64_t *request_id);
+
+/* Block until the request is done, copy its outputs (each may be NULL;
+ * layouts as pxh_solve_spectrum) and forget it.  times_ms[0] = queueing
+ * time, times_ms[1] = submission to completion.  Returns the solve's status. */
+int pxh_server_wait(pxh_server *srv, int64_t request_id, double *polar, double *results, int32_t *iters,
+                    double *true_residual, double *times_ms);
+
+int pxh_server_last_error(pxh_server *srv, char *buf, int len);
+
+/* Waits for every submitted request, then stops the loop. */
+int pxh_server_destroy(pxh_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ EXPORTS = [
     "pxh_ctx_create", "pxh_ctx_destroy", "pxh_set_atoms", "pxh_set_alpha", "pxh_set_mesh",
     "pxh_set_permittivity", "pxh_solve_spectrum", "pxh_matvec", "pxh_constant_matrix",
     "pxh_dynamic_rhs", "pxh_last_error", "pxh_set_option", "pxh_get_stats",
+    "pxh_server_create", "pxh_server_submit", "pxh_server_wait", "pxh_server_last_error", "pxh_server_destroy",
 ]
 
 
@@ -62,6 +63,11 @@ def load():
     lib.pxh_last_error.argtypes = [P, ctypes.c_char_p, I]
     lib.pxh_set_option.argtypes = [P, ctypes.c_char_p, D]
     lib.pxh_get_stats.argtypes = [P, P, I]
+    lib.pxh_server_create.argtypes = [I, ctypes.c_size_t, I, ctypes.POINTER(P)]
+    lib.pxh_server_submit.argtypes = [P, P, I, P, D, I, I, I, D, I, I, ctypes.POINTER(ctypes.c_int64)]
+    lib.pxh_server_wait.argtypes = [P, ctypes.c_int64, P, P, P, P, P]
+    lib.pxh_server_last_error.argtypes = [P, ctypes.c_char_p, I]
+    lib.pxh_server_destroy.argtypes = [P]
     for name in EXPORTS:
         getattr(lib, name).restype = I
     _lib = lib
@@ -121,6 +127,7 @@ class Context:
                                     _ptr(fd), _ptr(fs), _ptr(r0), _ptr(arrs[7]), _ptr(arrs[8]), _ptr(nbc),
                                     _ptr(nbt))
         self._check(st)
+        self.n_var = sysm.n_var
 
     def set_alpha(self, alpha_w):
         """alpha_w: complex (n_types, n_freq)."""
@@ -132,6 +139,7 @@ class Context:
         self._check(self.lib.pxh_set_mesh(self._h, mesh.variant_id, mesh.green_id, mesh.n, _ptr(v),
                                           float(mesh.normal_factor), float(mesh.sphere_r_bohr),
                                           float(mesh.eps_solvent), int(mesh.charge_constraint)))
+        self.n_var = mesh.n
 
     def set_permittivity(self, eps_w):
         e = np.ascontiguousarray(np.asarray(eps_w, dtype=np.complex128))
@@ -184,3 +192,54 @@ class Context:
                 "sweep_launches_fp32": s[8], "sweep_launches_fp64": s[9], "sweep_flops_fp32": s[10],
                 "sweep_flops_fp64": s[11], "mfma_kernel_ms": s[12], "mfma_launches": s[13],
                 "latency_p50_ms": s[14], "latency_p99_ms": s[15]}
+
+
+class Server:
+    """The request loop of libpxh (pxh_server_*): solves on prepared contexts,
+    admitted by HBM footprint and run concurrently on one device."""
+
+    def __init__(self, device=0, hbm_budget=0, max_inflight=4):
+        self.lib = load()
+        self._h = ctypes.c_void_p()
+        st = self.lib.pxh_server_create(int(device), int(hbm_budget), int(max_inflight), ctypes.byref(self._h))
+        if st != PXH_OK:
+            raise PxhError(st, "pxh_server_create failed")
+        self._pending = {}
+
+    def submit(self, ctx, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7, method=ITERATIVE_OTF, tol_abs=1e-8,
+               krylov_dim=1000, precision=FP64):
+        freq = np.ascontiguousarray(freq_au, dtype=np.float64)
+        rid = ctypes.c_int64()
+        st = self.lib.pxh_server_submit(self._h, ctx._h, freq.shape[0], _ptr(freq), float(e0), rhs_form, pol_mask,
+                                        method, float(tol_abs), int(krylov_dim), precision, ctypes.byref(rid))
+        if st != PXH_OK:
+            raise PxhError(st, "pxh_server_submit failed")
+        self._pending[rid.value] = freq.shape[0]
+        return rid.value
+
+    def wait(self, rid):
+        nf = self._pending.pop(rid)
+        polar = np.zeros((nf, 3, 3), dtype=np.complex128)
+        res = np.zeros((20, nf), dtype=np.float64)
+        iters = np.zeros((nf, 3), dtype=np.int32)
+        tres = np.zeros((nf, 3), dtype=np.float64)
+        times = np.zeros(2, dtype=np.float64)
+        st = self.lib.pxh_server_wait(self._h, ctypes.c_int64(rid), _ptr(polar), _ptr(res), _ptr(iters), _ptr(tres),
+                                      _ptr(times))
+        if st != PXH_OK:
+            buf = ctypes.create_string_buffer(512)
+            self.lib.pxh_server_last_error(self._h, buf, 512)
+            raise PxhError(st, buf.value.decode(errors="replace"))
+        return {"polar": np.transpose(polar, (0, 2, 1)), "results": res.T.copy(), "iters": iters,
+                "true_residual": tres, "queue_ms": times[0], "latency_ms": times[1]}
+
+    def close(self):
+        if self._h:
+            self.lib.pxh_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

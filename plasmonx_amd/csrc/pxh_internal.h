@@ -32,6 +32,8 @@ struct Error {
 
 // Maximum number of atom types handled by the pair kernels.
 constexpr int kMaxTypes = 8;
+// Frequencies whose dense matrices are factorised together (inversion).
+constexpr size_t kMaxFreqChunk = 16;
 
 // Per type-pair constants of the interaction kernels (index ti + nt*tj).
 struct PairConst {
@@ -193,4 +195,7 @@ void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
 int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
                    double tol, int kdim, int precision, double *variables, double *polar, double *results,
                    int32_t *iters, double *true_res);
+// HBM that one pxh_solve_spectrum call allocates beyond the context's
+// resident data (upper estimate).
+size_t estimate_solve_bytes(const pxh_ctx *c, int n_freq, int pol_mask, int method, int kdim, int precision);
 }  // namespace pxh

@@ -1044,8 +1044,34 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             if (true_res && k < tr_sys.size()) true_res[o] = tr_sys[k];
         }
     }
+    // the product scratch is per call (a context holds only its target between calls)
+    c->work_part.release();
+    c->work_t.release();
+    c->work_xh.release();
+    c->work_xl.release();
     c->stats.v[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return status;
+}
+
+size_t estimate_solve_bytes(const pxh_ctx *c, int n_freq, int pol_mask, int method, int kdim, int precision) {
+    const bool bem = c->target == PXH_BEM;
+    const size_t nv = bem ? (size_t)c->mesh.n : (size_t)c->atoms.n_var();
+    const size_t nsys = (size_t)n_freq * (size_t)__builtin_popcount(pol_mask & 7);
+    size_t b = 2 * nv * 3 * (size_t)n_freq * sizeof(double2);  // right-hand sides and solutions
+    if (method == PXH_INVERSION) {
+        const size_t k = std::min<size_t>(n_freq, kMaxFreqChunk);
+        b += k * nv * nv * 2 * sizeof(double2) + 2 * nv * nv * sizeof(double);
+    } else {
+        const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
+        const size_t B = plain ? 1 : std::max<size_t>(1, std::min<size_t>(c->opt.batch, nsys));
+        const size_t m = std::min<size_t>(std::max(kdim, 1), nv);
+        const bool vf = precision == PXH_MIXED && !plain;
+        b += B * (m + 1) * nv * (vf ? sizeof(float2) : sizeof(double2));  // Krylov basis
+        b += 8 * B * nv * sizeof(double2);                                  // slot vectors
+        b += 9 * B * nv * sizeof(double2);                                  // product partials and t
+        if (!bem && method == PXH_ITERATIVE) b += nv * nv * sizeof(double);  // stored T
+    }
+    return b + b / 10;
 }
 
 }  // namespace pxh

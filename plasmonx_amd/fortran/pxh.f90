@@ -126,6 +126,46 @@ module pxh
          real(c_double), intent(out) :: stats(*)
          integer(c_int), value :: n
       end function
+
+      integer(c_int) function pxh_server_create(device, hbm_budget_bytes, max_inflight, srv) &
+         bind(C, name="pxh_server_create")
+         import :: c_int, c_size_t, c_ptr
+         integer(c_int), value :: device, max_inflight
+         integer(c_size_t), value :: hbm_budget_bytes
+         type(c_ptr) :: srv
+      end function
+
+      integer(c_int) function pxh_server_submit(srv, ctx, n_freq, freq_au, e0, rhs_form, pol_mask, method, tol_abs, &
+                                                krylov_dim, precision, request_id) bind(C, name="pxh_server_submit")
+         import :: c_int, c_int64_t, c_double, c_ptr
+         type(c_ptr), value :: srv, ctx
+         integer(c_int), value :: n_freq, rhs_form, pol_mask, method, krylov_dim, precision
+         real(c_double), intent(in) :: freq_au(*)
+         real(c_double), value :: e0, tol_abs
+         integer(c_int64_t) :: request_id
+      end function
+
+      integer(c_int) function pxh_server_wait(srv, request_id, polar, results, iters, true_residual, times_ms) &
+         bind(C, name="pxh_server_wait")
+         import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr
+         type(c_ptr), value :: srv
+         integer(c_int64_t), value :: request_id
+         complex(c_double_complex) :: polar(3, 3, *)
+         real(c_double) :: results(*), true_residual(3, *), times_ms(2)
+         integer(c_int32_t) :: iters(3, *)
+      end function
+
+      integer(c_int) function pxh_server_last_error(srv, buf, len) bind(C, name="pxh_server_last_error")
+         import :: c_int, c_char, c_ptr
+         type(c_ptr), value :: srv
+         character(kind=c_char) :: buf(*)
+         integer(c_int), value :: len
+      end function
+
+      integer(c_int) function pxh_server_destroy(srv) bind(C, name="pxh_server_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: srv
+      end function
    end interface
 
 contains

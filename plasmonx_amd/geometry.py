@@ -157,3 +157,22 @@ def capped_rod(length, radius, n_ring=40, n_len=None, n_cap=10):
     flip = np.einsum("ij,ij->i", n, cen - axis_pt) < 0
     T[flip] = T[flip][:, [0, 2, 1]]
     return T
+
+
+def fcc_rod(a, length, radius):
+    """fcc lattice (constant a, a site at the origin) inside a cylinder of
+    total length `length` along z with hemispherical caps of radius
+    `radius` (all in the units of a)."""
+    m_xy = int(math.ceil(radius / a)) + 1
+    m_z = int(math.ceil(length / 2 / a)) + 1
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    gx = np.arange(-m_xy, m_xy + 1)
+    gz = np.arange(-m_z, m_z + 1)
+    cells = np.stack(np.meshgrid(gx, gx, gz, indexing="ij"), axis=-1).reshape(-1, 3)
+    pts = (cells[:, None, :] + base[None, :, :]).reshape(-1, 3) * a
+    half = max(0.0, length / 2 - radius)
+    zc = np.clip(pts[:, 2], -half, half)
+    d = np.sqrt(pts[:, 0] ** 2 + pts[:, 1] ** 2 + (pts[:, 2] - zc) ** 2)
+    pts = pts[d <= radius + 1e-9]
+    order = np.lexsort((pts[:, 2], pts[:, 1], pts[:, 0]))
+    return pts[order]

@@ -8,6 +8,8 @@ C2  omegaFQFmu Ag fcc sphere a=4.09 A, r=4 nm (15683 atoms, 62732 unknowns),
     (Drude + critical-point Lorentz) silver model
 C3  BEM DPCM / IEF on an icosphere R=10 nm (5120 triangles) and a capped rod
 C4  96 seeded graphene flakes, 128 w in 0.1-1.0 eV
+C5  a seeded Poisson stream of spectrum requests (omegaFQ Na rods, omegaFQFmu
+    Au@Ag core-shell spheres, BEM rods)
 """
 import math
 
@@ -96,3 +98,59 @@ def c3_meshes(variant="dpcm"):
 def c3_permittivity(freq):
     bb = permittivity.BB_SILVER
     return permittivity.brendel_bormann(freq, bb["eps_inf"], bb["wp_ev"], bb["f0"], bb["g0_ev"], bb["osc"])
+
+
+def c4_flakes(n_flakes=96, seed=20261203, lo=500, hi=12000):
+    """C4: the seeded flakes as omegaFQ systems (E_F per flake), 128 w in
+    0.1-1.0 eV, in-plane fields x and y."""
+    out = []
+    for f in geometry.graphene_batch(n_flakes, seed, lo, hi):
+        out.append(graphene_flake(f["xyz"], f["fermi_energy"]))
+    return out, ev(np.linspace(0.1, 1.0, 128))
+
+
+def sodium_rod(length_ang, radius_ang):
+    """omegaFQ Na rod with the C1 parameters."""
+    base, _ = c1_sodium_sphere()
+    xyz = geometry.fcc_rod(4.23, length_ang, radius_ang)
+    return AtomSystem(model="wfq", kernel="gaussian", symbols=["Na"] * len(xyz), coord_bohr=xyz * TOBOHR,
+                      types=base.types, fermi_d=base.fermi_d, fermi_s=base.fermi_s, r0=base.r0)
+
+
+def c5_requests(n_requests=24, seed=7, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100)):
+    """C5: a seeded Poisson stream.  Each request is a dict with its arrival
+    time (s), kind, a builder of its target and its frequency list (eV ranges
+    as the C1-C3 configurations).  Sizes are drawn log-uniform in the given
+    ranges (the full configuration is 2k-60k atoms, 2k-20k tesserae, 50-400
+    frequencies; the bounds are parameters)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        kind = ["wfq_rod", "wfqfmu_coreshell", "bem_rod"][int(rng.integers(0, 3))]
+        nf = int(rng.integers(n_freq[0], n_freq[1] + 1))
+        if kind == "bem_rod":
+            nt = float(np.exp(rng.uniform(np.log(tesserae[0]), np.log(tesserae[1]))))
+            aspect = float(rng.uniform(2.0, 4.0))
+            radius = 60.0
+            # tesserae ~ 2 n_ring^2 (aspect - 1) / pi + 4 n_ring n_cap with n_cap = n_ring / 4
+            n_ring = max(16, int(round(np.sqrt(nt / (2 * (aspect - 1) / np.pi + 1.0)))))
+            n_cap = max(4, n_ring // 4)
+            verts = geometry.capped_rod(aspect * 2 * radius, radius, n_ring=n_ring, n_cap=n_cap) * TOBOHR
+            build = (lambda v=verts: Mesh(verts_bohr=v, variant="dpcm", green="accurate"))
+            freq = ev(np.linspace(3.0, 4.0, nf))
+        elif kind == "wfq_rod":
+            na = float(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1]))))
+            aspect = float(rng.uniform(2.0, 4.0))
+            # fcc Na: 4 atoms per 4.23^3 A^3; volume of a capped rod pi r^2 (L - 2r/3)
+            radius = (na * 4.23 ** 3 / 4 / (np.pi * (2 * aspect - 2.0 / 3.0))) ** (1.0 / 3.0)
+            build = (lambda L=2 * aspect * radius, r=radius: sodium_rod(L, r))
+            freq = ev(np.linspace(2.5, 4.0, nf))
+        else:
+            na = float(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1]))))
+            r_shell = (na * 4.08 ** 3 / 4 / (4.0 / 3.0 * np.pi)) ** (1.0 / 3.0)
+            build = (lambda rs=r_shell: core_shell_sphere(0.6 * rs, rs))
+            freq = ev(np.linspace(2.8, 4.2, nf))
+        out.append({"arrival": t, "kind": kind, "build": build, "freq": freq})
+    return out
