@@ -52,11 +52,20 @@ def main():
     ap.add_argument("--traffic", action="store_true",
                     help="run one 48-system pair-kernel launch and a 256 MiB device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE), then exit")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): the headline omegaFQFmu spectrum; c1 dense LU omegaFQ Na; c3 BEM meshes; "
+                         "c4 graphene flake batch; c5 request stream")
+    ap.add_argument("--requests", type=int, default=24, help="c5: requests in the stream")
+    ap.add_argument("--flakes", type=int, default=16, help="c4: flakes (the full batch is 96)")
+    ap.add_argument("--flake-max", type=int, default=4000, help="c4: largest flake (the full batch reaches 12000)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
-                          "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: only --gpus 1 is measured", "n_gpus": a.gpus}))
         sys.exit(2)
+
+    if a.workload != "c2":
+        {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[a.workload](a)
+        return
 
     from plasmonx_amd import Context, _lib
     from plasmonx_amd.workloads import c2_silver_sphere
@@ -209,6 +218,214 @@ def cpu_baseline(s, freq, aw, iters_per_system):
     return {"value": round(1.0 / per_point, 6), "unit": "freq-point solves/s", "cores": cores, "kind": "port",
             "sample": f"one FP64 on-the-fly product at full size ({dt:.1f}s) x {iters_per_system:.0f} GMRES "
                       f"iterations x 3 fields per frequency point"}
+
+
+def emit(metric, value, unit, ms, a, config, **extra):
+    line = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic", "config": config}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def run_c1(a):
+    """C1: omegaFQ Na fcc sphere (1745 atoms), 61 w in 2.5-4.0 eV, x field,
+    dense complex LU per frequency (the reference's 'inversion')."""
+    from plasmonx_amd import Context, _lib
+    from plasmonx_amd.workloads import c1_sodium_sphere
+    s, freq = c1_sodium_sphere()
+    ctx = Context(0)
+    ctx.set_atoms(s)
+    e0 = 1e-4
+    for _ in range(a.warmup):
+        ctx.solve_spectrum(s.n_var, freq[:4], e0, pol_mask=1, method=_lib.INVERSION)
+    times = []
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        out = ctx.solve_spectrum(s.n_var, freq, e0, pol_mask=1, method=_lib.INVERSION)
+        times.append(time.perf_counter() - t)
+    dt = sum(times) / len(times)
+    n = s.n_var
+    flops = len(freq) * (32.0 / 3.0) * n ** 3  # complex LU = 8/3 n^3 complex multiply-adds = 32/3 n^3 real flop
+    achieved = flops / dt / 1e12
+    cpu = None
+    if not a.no_cpu:
+        from oracle import pxo
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        t = time.perf_counter()
+        pxo.solve_spectrum(s, freq[:2], e0, pol_mask=1, method=0)
+        dc = time.perf_counter() - t
+        cpu = {"value": round(2 / dc, 4), "unit": "freq-point solves/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 oracle, dense LU, 2 frequencies ({dc:.1f}s)"}
+    log(f"[bench] C1 inversion {len(freq) / dt:.2f} fp/s")
+    emit("freq-point solves/s (C1 omegaFQ Na 1745 atoms, dense LU)", len(freq) / dt, "freq-point solves/s",
+         1e3 * dt, a,
+         {"workload": "omegaFQ Na fcc sphere a=4.23A r=2nm (1745 atoms), 61 w 2.5-4.0 eV, x field, inversion",
+          "n_var": n, "n_freq": len(freq)},
+         dtype="fp64",
+         roofline={"bound": "fp64 valu", "achieved": round(achieved, 2), "peak": 78.6, "unit": "TFLOP/s",
+                   "frac": round(achieved / 78.6, 4), "traffic": None,
+                   "kernel": "whole inversion path (assembly + batched LU + solve), 32/3 n^3 flop per frequency"},
+         cpu_baseline=cpu)
+
+
+def run_c3(a):
+    """C3: BEM DPCM / IEF on the icosphere (5120) and DPCM on the capped rod,
+    Brendel-Bormann silver, 400 w x 3 fields, mixed GMRES on the stored D."""
+    from plasmonx_amd import Context, _lib
+    from plasmonx_amd.workloads import c3_meshes, c3_permittivity
+    res = {}
+    total_pts, total_t = 0, 0.0
+    for label, variant, which in [("sphere_dpcm", "dpcm", 0), ("sphere_ief", "iefpcm", 0), ("rod_dpcm", "dpcm", 1)]:
+        meshes = c3_meshes(variant)
+        m, freq = meshes[which], meshes[2]
+        eps = c3_permittivity(freq)
+        ctx = Context(0)
+        t = time.perf_counter()
+        ctx.set_mesh(m)
+        ctx.set_permittivity(eps)
+        t_setup = time.perf_counter() - t
+        tol = 1e-12 * np.sqrt(m.n)
+        ctx.set_option("batch", 48)
+        ctx.solve_spectrum(m.n, freq[:8], 1e-4, method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=200,
+                           precision=_lib.MIXED)
+        t = time.perf_counter()
+        out = ctx.solve_spectrum(m.n, freq, 1e-4, method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=200,
+                                 precision=_lib.MIXED)
+        dt = time.perf_counter() - t
+        chk = np.array([0, len(freq) // 2, len(freq) - 1])
+        ctx.set_option("batch", 1)
+        ctx.set_permittivity(eps[chk])  # the table is indexed by position in the frequency list
+        ref = ctx.solve_spectrum(m.n, freq[chk], 1e-4, method=_lib.ITERATIVE, tol_abs=1e-2 * tol, krylov_dim=600,
+                                 precision=_lib.FP64)
+        err = max(float(np.linalg.norm(out["polar"][f] - ref["polar"][k]) / np.linalg.norm(ref["polar"][k]))
+                  for k, f in enumerate(chk))
+        res[label] = {"n_tess": m.n, "value": round(len(freq) / dt, 3), "seconds": round(dt, 2),
+                      "setup_s": round(t_setup, 2), "iters_mean": float(out["iters"].mean()),
+                      "max_rel_err_vs_fp64": err}
+        log(f"[bench] C3 {label}: n={m.n}, {len(freq) / dt:.2f} fp/s, err {err:.1e}")
+        total_pts += len(freq)
+        total_t += dt
+        ctx.close()
+    emit("freq-point solves/s (C3 BEM, 400 w x 3 fields)", total_pts / total_t, "freq-point solves/s",
+         1e3 * total_t, a,
+         {"workload": "BEM on icosphere R=10nm (5120, DPCM and IEF) and capped rod 40x10nm (DPCM), "
+                      "Brendel-Bormann Ag, 400 w 3.0-4.0 eV x 3 fields, GMRES on stored D (FP32) + FP64 refinement"},
+         dtype="fp32 operator + fp64 refinement", cases=res, roofline=None, cpu_baseline=None)
+
+
+def run_c4(a):
+    """C4: seeded graphene flakes, 128 w in 0.1-1.0 eV, x and y fields; the
+    flakes are served concurrently by the request loop."""
+    from plasmonx_amd import Context, _lib
+    from plasmonx_amd.workloads import c4_flakes
+    flakes, freq = c4_flakes(a.flakes, hi=a.flake_max)
+    log(f"[bench] C4: {len(flakes)} flakes, {sum(f.n_atoms for f in flakes)} atoms, {len(freq)} w x 2 fields")
+    srv = _lib.Server(0, max_inflight=4)
+    ctxs = []
+    t = time.perf_counter()
+    ids = []
+    for s in flakes:
+        c = Context(0)
+        c.set_atoms(s)
+        c.set_option("batch", 64)
+        ctxs.append(c)
+        # dense LU per frequency: graphene's weakly damped plasmons make GMRES
+        # need thousands of iterations (3624-atom flake: 1.9 s by LU against
+        # 27 s by GMRES(2000) for the 128-point spectrum)
+        ids.append(srv.submit(c, freq, 1e-4, pol_mask=3, method=_lib.INVERSION))
+    outs = []
+    for k, i in enumerate(ids):
+        outs.append(srv.wait(i))
+        log(f"[bench] C4 flake {k}: {flakes[k].n_atoms} atoms done at {time.perf_counter() - t:.1f}s")
+    dt = time.perf_counter() - t
+    srv.close()
+    # check: the same flake alone (no request loop) at one frequency
+    errs = []
+    for k in (0, len(flakes) // 2):
+        c = ctxs[k]
+        r = c.solve_spectrum(flakes[k].n_var, freq[64:65], 1e-4, pol_mask=3, method=_lib.INVERSION)
+        errs.append(float(np.linalg.norm(outs[k]["polar"][64] - r["polar"][0]) / np.linalg.norm(r["polar"][0])))
+    for c in ctxs:
+        c.close()
+    pts = len(flakes) * len(freq)
+    lat = np.array([o["latency_ms"] for o in outs])
+    log(f"[bench] C4 {len(flakes)} flakes x {len(freq)} w: {pts / dt:.1f} fp/s, err {max(errs):.1e}")
+    emit("flake-frequency solves/s (C4 graphene batch)", pts / dt, "freq-point solves/s", 1e3 * dt, a,
+         {"workload": f"{len(flakes)} seeded graphene flakes (500-{a.flake_max} atoms; the full batch is 96 flakes of "
+                     "500-12k atoms), 128 w 0.1-1.0 eV, x and y fields, dense LU, 4 flakes in flight",
+          "atoms_total": int(sum(s.n_atoms for s in flakes))},
+         dtype="fp64", max_rel_err_vs_alone=max(errs),
+         latency={"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))},
+         roofline=None, cpu_baseline=None)
+
+
+def run_c5(a):
+    """C5: a seeded Poisson stream (200 requests/s) of spectrum requests --
+    omegaFQ Na rods, omegaFQFmu Au@Ag core-shell spheres, BEM rods -- served
+    by the request loop; latency from arrival (target setup included) to
+    result."""
+    from plasmonx_amd import Context, _lib
+    from plasmonx_amd.workloads import c3_permittivity, c5_requests
+    reqs = c5_requests(a.requests)
+
+    def c_model(r):
+        return getattr(r["target"], "model", "bem")
+
+    srv = _lib.Server(0, max_inflight=4)
+    t0 = time.perf_counter()
+    live = []
+    for r in reqs:
+        wait = r["arrival"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        tgt = r["build"]()
+        r["target"] = tgt
+        c = Context(0)
+        if r["kind"] == "bem_rod":
+            c.set_mesh(tgt)
+            c.set_permittivity(c3_permittivity(r["freq"]))
+            n_var, method = tgt.n, _lib.ITERATIVE
+        else:
+            c.set_atoms(tgt)
+            if tgt.model == "wfqfmu":
+                c.set_alpha(tgt.alpha_table(r["freq"]))
+            n_var, method = tgt.n_var, _lib.ITERATIVE_OTF
+        rid = srv.submit(c, r["freq"], 1e-4, pol_mask=7, method=method, tol_abs=1e-12 * np.sqrt(n_var),
+                         krylov_dim=200, precision=_lib.MIXED)
+        live.append((r, c, rid, n_var, method))
+    done = []
+    for r, c, rid, n_var, method in live:
+        o = srv.wait(rid)
+        done.append((r, c, o, n_var, method, time.perf_counter() - t0 - r["arrival"]))
+    makespan = time.perf_counter() - t0
+    srv.close()
+    # per-request error at one frequency against FP64 (3 fields)
+    errs = []
+    for r, c, o, n_var, method, lat in done:
+        c.set_option("batch", 3)
+        k = len(r["freq"]) // 2
+        fk = r["freq"][k:k + 1]
+        if r["kind"] == "bem_rod":
+            c.set_permittivity(c3_permittivity(fk))
+        elif method == _lib.ITERATIVE_OTF and c_model(r) == "wfqfmu":
+            c.set_alpha(r["target"].alpha_table(fk))
+        ref = c.solve_spectrum(n_var, r["freq"][k:k + 1], 1e-4, method=method, tol_abs=1e-14 * np.sqrt(n_var),
+                               krylov_dim=1000, precision=_lib.FP64)
+        errs.append(float(np.linalg.norm(o["polar"][k] - ref["polar"][0]) / np.linalg.norm(ref["polar"][0])))
+        c.close()
+    lat = np.array([d[5] for d in done]) * 1e3
+    pts = sum(len(r["freq"]) for r in reqs)
+    log(f"[bench] C5 {len(reqs)} requests, {pts} freq points in {makespan:.1f}s: {pts / makespan:.1f} fp/s, "
+        f"p50 {np.percentile(lat, 50):.0f} ms, p99 {np.percentile(lat, 99):.0f} ms, max err {max(errs):.1e}")
+    emit("freq-point solves/s (C5 request stream)", pts / makespan, "freq-point solves/s", 1e3 * makespan, a,
+         {"workload": f"{len(reqs)} Poisson arrivals at 200/s (seed 7): omegaFQ Na rods, omegaFQFmu Au@Ag "
+                      "core-shell spheres (2k-12k atoms), BEM rods (2k-8k tesserae), 50-100 w each, 3 fields",
+          "max_inflight": 4},
+         dtype="fp32 tiles + fp64 refinement",
+         latency={"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                  "definition": "arrival to result, target setup included"},
+         per_request_max_rel_err_vs_fp64=max(errs), per_request_err=errs, roofline=None, cpu_baseline=None)
 
 
 if __name__ == "__main__":

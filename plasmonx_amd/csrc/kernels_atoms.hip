@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 #include "pxh_internal.h"
 
@@ -156,10 +157,16 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
         }
         __syncthreads();
         const int cnt = min(TB, j_end - jt);
-        R tq_re[KV], tq_im[KV], tm_re[KV][3], tm_im[KV][3];
+        // omegaFQ accumulates the charge rows in FP64 even with FP32 elements:
+        // K-bar takes differences of neighbouring potentials, which would
+        // amplify an FP32 sum's rounding (FP32 and FP64 FMAs issue at the same
+        // rate here, packed FP32 being off)
+        using QAcc = typename std::conditional<FMU, R, double>::type;
+        QAcc tq_re[KV], tq_im[KV];
+        R tm_re[KV][3], tm_im[KV][3];
 #pragma unroll
         for (int v = 0; v < KV; ++v) {
-            tq_re[v] = tq_im[v] = (R)0;
+            tq_re[v] = tq_im[v] = (QAcc)0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) tm_re[v][a] = tm_im[v][a] = (R)0;
         }
@@ -212,8 +219,8 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
                 if (!FMU) {
 #pragma unroll
                     for (int v = 0; v < KV; ++v) {
-                        tq_re[v] += tqq * tile.q[v][jj].re;
-                        tq_im[v] += tqq * tile.q[v][jj].im;
+                        tq_re[v] += (QAcc)tqq * (QAcc)tile.q[v][jj].re;
+                        tq_im[v] += (QAcc)tqq * (QAcc)tile.q[v][jj].im;
                     }
                     continue;
                 }
@@ -764,6 +771,8 @@ void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     if (nsys <= 0) return;
     if (nsys == 1)
         apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
+    else if (c->atoms.model == PXH_WFQ && nsys >= 8)  // charges only: room for 8 systems per pass
+        apply_atoms_impl<double, 8>(c, nsys, d_sys_list, d_par, x, y, ld);
     else
         apply_atoms_impl<double, 2>(c, nsys, d_sys_list, d_par, x, y, ld);
 }
@@ -776,6 +785,7 @@ void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         apply_atoms_mfma(c, nsys, d_sys_list, d_par, x, y, ld);
         return;
     }
+
     if (nsys == 1)
         apply_atoms_impl<float, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
     else if (nsys == 2)

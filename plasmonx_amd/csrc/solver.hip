@@ -595,7 +595,10 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     S.stage.alloc((size_t)nv * B);
     // diagonal right preconditioning (x = D^-1 u) for the batched path; the
     // plain path is the reference's unpreconditioned GMRES
-    S.precond = !plain && c->opt.precond && !(c->target == PXH_BEM && c->mesh.variant == PXH_IEFPCM);
+    // (measured: it cuts the omegaFQFmu and BEM DPCM iteration counts, and
+    // makes weakly damped omegaFQ spectra stagnate, so omegaFQ runs without)
+    S.precond = !plain && c->opt.precond &&
+                (c->target == PXH_BEM ? c->mesh.variant == PXH_DPCM : c->atoms.model == PXH_WFQFMU);
     if (S.precond) S.dinv.alloc((size_t)nv * B);
     S.h_jp1.assign(B, 0);
     S.h_jl.assign(B, 0);
@@ -954,6 +957,12 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         // HBM; 'iterative on the fly' recomputes T in every product.
         c->use_stored = method == PXH_ITERATIVE && !bem;
         if (c->use_stored) prepare_stored_t(c);
+        // omegaFQ (charges only): the FP64 pair kernel issues at the FP32 rate
+        // here, and the restarts of the refinement cost more iterations than
+        // the low-precision products save (1745-atom Na sphere, 61 w: 326
+        // iterations per system in FP64, 804 refined with FP64 products, 891
+        // with FP32 products), so a mixed request runs the batched FP64 solve
+        if (!bem && c->atoms.model == PXH_WFQ && precision == PXH_MIXED) precision = PXH_FP64;
         const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
         try {
             status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, c->use_stored ? PXH_FP64
@@ -1062,6 +1071,12 @@ size_t estimate_solve_bytes(const pxh_ctx *c, int n_freq, int pol_mask, int meth
         const size_t k = std::min<size_t>(n_freq, kMaxFreqChunk);
         b += k * nv * nv * 2 * sizeof(double2) + 2 * nv * nv * sizeof(double);
     } else {
+        // omegaFQ (charges only): the FP64 pair kernel issues at the FP32 rate
+        // here, and the restarts of the refinement cost more iterations than
+        // the low-precision products save (1745-atom Na sphere, 61 w: 326
+        // iterations per system in FP64, 804 refined with FP64 products, 891
+        // with FP32 products), so a mixed request runs the batched FP64 solve
+        if (!bem && c->atoms.model == PXH_WFQ && precision == PXH_MIXED) precision = PXH_FP64;
         const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
         const size_t B = plain ? 1 : std::max<size_t>(1, std::min<size_t>(c->opt.batch, nsys));
         const size_t m = std::min<size_t>(std::max(kdim, 1), nv);
