@@ -34,6 +34,49 @@ int guarded(pxh_ctx *c, F &&f) {
 
 }  // namespace
 
+namespace pxh {
+
+namespace {
+void timer_collect(pxh_ctx *c, int t) {
+    TimerRing &R = c->timers;
+    PXH_HIP(hipEventSynchronize(R.end[t]));
+    float ms = 0.f;
+    PXH_HIP(hipEventElapsedTime(&ms, R.beg[t], R.end[t]));
+    c->stats.v[R.stat[t]] += ms;
+    R.pending[t] = false;
+}
+}  // namespace
+
+int timer_begin(pxh_ctx *c) {
+    TimerRing &R = c->timers;
+    const int t = R.head;
+    if (R.open[t]) return -1;  // as many timers open as the ring holds
+    if (R.pending[t]) timer_collect(c, t);
+    PXH_HIP(hipEventRecord(R.beg[t], c->stream));
+    R.open[t] = true;
+    R.head = (t + 1) % TimerRing::kN;
+    return t;
+}
+
+void timer_end(pxh_ctx *c, int t, int stat_index) {
+    if (t < 0) return;
+    TimerRing &R = c->timers;
+    PXH_HIP(hipEventRecord(R.end[t], c->stream));
+    R.stat[t] = stat_index;
+    R.open[t] = false;
+    R.pending[t] = true;
+}
+
+void timer_drain(pxh_ctx *c) {
+    TimerRing &R = c->timers;
+    for (int k = 0; k < TimerRing::kN; ++k) {
+        const int t = (R.head + k) % TimerRing::kN;  // oldest first
+        if (R.pending[t]) timer_collect(c, t);
+    }
+}
+
+}  // namespace pxh
+
 extern "C" {
 
 int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out) {
@@ -54,10 +97,10 @@ int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out) {
     c->budget = hbm_budget_bytes;
     const int st = guarded(c, [&] {
         PXH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        PXH_HIP(hipEventCreate(&c->ev0));
-        PXH_HIP(hipEventCreate(&c->ev1));
-        PXH_HIP(hipEventCreate(&c->ev2));
-        PXH_HIP(hipEventCreate(&c->ev3));
+        for (int k = 0; k < pxh::TimerRing::kN; ++k) {
+            PXH_HIP(hipEventCreate(&c->timers.beg[k]));
+            PXH_HIP(hipEventCreate(&c->timers.end[k]));
+        }
     });
     if (st != PXH_OK) {
         delete c;
@@ -71,10 +114,10 @@ int pxh_ctx_destroy(pxh_ctx *c) {
     if (!c) return PXH_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->ev2) (void)hipEventDestroy(c->ev2);
-    if (c->ev3) (void)hipEventDestroy(c->ev3);
+    for (int k = 0; k < pxh::TimerRing::kN; ++k) {
+        if (c->timers.beg[k]) (void)hipEventDestroy(c->timers.beg[k]);
+        if (c->timers.end[k]) (void)hipEventDestroy(c->timers.end[k]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PXH_OK;
@@ -231,7 +274,7 @@ int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *
             std::vector<double2> hl(n_vec, l);
             pxh::DevBuf<double2> dlam;
             dlam.upload(hl.data(), n_vec, c->stream);
-            pxh::apply_bem(c, n_vec, dl.p, dlam.p, dx.p, dy.p, nv, false);
+            pxh::apply_bem(c, n_vec, dl.p, dlam.p, dx.p, dy.p, nv, c->opt.matvec_mixed != 0);
         } else {
             std::vector<pxh::SysParam> hp(n_vec, pxh::make_sysparam(c, freq_au, i_freq));
             pxh::DevBuf<pxh::SysParam> dp;
@@ -327,6 +370,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
 
 int pxh_get_stats(pxh_ctx *c, double *stats, int n) {
     if (!c || !stats) return PXH_EINVAL;
+    const int st = guarded(c, [&] { pxh::timer_drain(c); });
+    if (st != PXH_OK) return st;
     for (int k = 0; k < n && k < 16; ++k) stats[k] = c->stats.v[k];
     return PXH_OK;
 }

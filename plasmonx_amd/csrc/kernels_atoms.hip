@@ -722,7 +722,8 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     double2 *part = c->work_part.p;
     dim3 grid((n + TB - 1) / TB, splits_eff, groups);
     const double rfar2 = far_field_r2(A, sizeof(R) == 8);
-    PXH_HIP(hipEventRecord(c->ev0, c->stream));
+    const bool f32 = sizeof(R) == 4;
+    const int timer = timer_begin(c);
     if (fmu)
         launch_tsweep_k<R, KV, true>(A.kernel, grid, c->stream, n, A.nt, A.hetero ? 1 : 0, A.pos.p, A.pair.p, nsys,
                                      d_sys_list, x, ld, part, nv, j_chunk, rfar2);
@@ -730,19 +731,14 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         launch_tsweep_k<R, KV, false>(A.kernel, grid, c->stream, n, A.nt, 0, A.pos.p, A.pair.p, nsys, d_sys_list,
                                       x, ld, part, nv, j_chunk, rfar2);
     PXH_HIP(hipGetLastError());
-    PXH_HIP(hipEventRecord(c->ev1, c->stream));
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, sizeof(R) == 4);
     // time of the pair kernel on the context's stream, and its algorithmic
     // flops: per ordered pair (i != j) kPairEvalFlops for the interaction
     // elements plus kPairVecFlops per complex vector for the contraction
-    PXH_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    PXH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    const bool f32 = sizeof(R) == 4;
+    timer_end(c, timer, f32 ? 3 : 4);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, f32);
     const double pairs = (double)n * (double)(n - 1);
     const double fl = pairs * ((double)(fmu ? kPairEvalFlopsFmu : kPairEvalFlopsQ) * groups +
                                (double)(fmu ? kPairVecFlopsFmu : kPairVecFlopsQ) * nsys);
-    c->stats.v[f32 ? 3 : 4] += ms;
     c->stats.v[f32 ? 8 : 9] += 1;
     c->stats.v[f32 ? 10 : 11] += fl;
     c->stats.v[6] += (double)n * (double)n * groups;

@@ -241,39 +241,135 @@ __global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restri
     if (i < n) b[i] = (float)a[i];
 }
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Operator products Y = Mat X on the matrix cores.  Mat is a real n x n
+// column-major matrix (FP64, or FP32 widened on load); X holds nsys complex
+// vectors, i.e. 2 nsys real columns.  The arithmetic is FP64 throughout
+// (v_mfma_f64_16x16x4_f64, FP64 accumulation).
+//
+// X is first packed by k_gemm_pack into tiles of 16 systems (32 real
+// columns): Xp[tile][c][j][h] with real column h*16 + j, so one 16-byte load
+// gives a lane both of its B operands.  Rows c >= n are zero up to cpad.
+// A workgroup computes a 32-row x 32-column tile of Y; its four waves split
+// the columns of Mat (K) in equal ranges and sum through LDS at the end.
+// Each wave holds 2 x 2 16x16 accumulators: row fragment h covers tile rows
+// 2i + h (so a lane reads two adjacent rows of one matrix column), column
+// fragment h covers real columns h*16 + j.
+constexpr int kGemmRows = 32;
+constexpr int kGemmSys = 16;
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__global__ void k_gemm_pack(int n, int cpad, int nsys, const int *__restrict__ slots, const double2 *__restrict__ x,
+                            int ld, int x_by_k, int ntiles, double *__restrict__ Xp) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)ntiles * cpad * kGemmSys) return;
+    // one work-item per (tile, row c, system within the tile)
+    const int q = (int)(e % kGemmSys);
+    const size_t tc = e / kGemmSys;
+    const int c = (int)(tc % cpad);
+    const int t = (int)(tc / cpad);
+    const int k = t * kGemmSys + q;
+    double2 v = make_double2(0.0, 0.0);
+    if (k < nsys && c < n) v = x[(size_t)ld * (x_by_k ? k : slots[k]) + c];
+    // real columns 2q (re) and 2q+1 (im) -> (j, h) = (col % 16, col / 16)
+    double *row = Xp + ((size_t)t * cpad + c) * 32;
+    const int c0 = 2 * q, c1 = 2 * q + 1;
+    row[(c0 % 16) * 2 + c0 / 16] = v.x;
+    row[(c1 % 16) * 2 + c1 / 16] = v.y;
 }
 
-// y_s = Mat x_s (real n x n matrix, complex vectors), optional FP32 matrix.
+struct GemmOut {
+    const double2 *lam;  // per slot; null: no shift
+    const double2 *u;    // shifted vectors (lam u), ldu, by k or by slot
+    int ldu, u_by_k;
+    double sign;
+    double2 *y;
+    int ldy, y_by_k;
+};
+
+// Y tile = sign * (Mat X + lam u) for the listed systems.  Grid: 8 * per
+// workgroups, per = ceil(tiles / 8); workgroup b takes tile
+// (b % 8) * per + b / 8, so each XCD (b % 8) works on a contiguous range of
+// row tiles and reads their matrix rows through its own L2 for every column
+// tile.
 template <typename T>
-__global__ void k_real_matvec(int n, const T *__restrict__ Mat, const int *__restrict__ slots,
-                              const double2 *__restrict__ x, int ld, double2 *__restrict__ y, int ldy, int y_by_k) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    const int s = slots[k];
-    if (r >= n) return;
-    const double2 *xs = x + (size_t)ld * s;
-    double re = 0.0, im = 0.0;
-    for (int c = 0; c < n; ++c) {
-        const double a = (double)Mat[r + (size_t)n * c];
-        re += a * xs[c].x;
-        im += a * xs[c].y;
+__global__ __launch_bounds__(256) void k_bem_gemm(int n, int cpad, int steps_per_wave, int rowtiles, int coltiles,
+                                                  const T *__restrict__ Mat, const double *__restrict__ Xp,
+                                                  const int *__restrict__ slots, int nsys, GemmOut o) {
+    __shared__ double red[4][16 * 64];
+    const int total = rowtiles * coltiles;
+    const int per = (total + 7) / 8;
+    const int L = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (L >= total) return;
+    const int rt = L / coltiles, ct = L % coltiles;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int r0 = rt * kGemmRows;
+    const int ra = min(r0 + 2 * i, n - 1), rb = min(r0 + 2 * i + 1, n - 1);
+    const int s0 = w * steps_per_wave;
+    const double *xb = Xp + (size_t)ct * cpad * 32 + 2 * i;
+    d4v acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    // four K steps (16 matrix columns) per iteration, the next four loaded
+    // while the current ones are multiplied
+    double a0[4], a1[4];
+    double2 bv[4];
+    auto load = [&](int s, double *pa0, double *pa1, double2 *pb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = 4 * (s + u) + kk;
+            const T *mc = Mat + (size_t)n * min(c, n - 1);
+            pa0[u] = (double)mc[ra];
+            pa1[u] = (double)mc[rb];
+            pb[u] = *reinterpret_cast<const double2 *>(xb + (size_t)c * 32);
+        }
+    };
+    load(s0, a0, a1, bv);
+    for (int s = s0; s < s0 + steps_per_wave; s += 4) {
+        double na0[4], na1[4];
+        double2 nb[4];
+        if (s + 4 < s0 + steps_per_wave) load(s + 4, na0, na1, nb);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], bv[u].x, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], bv[u].y, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], bv[u].x, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], bv[u].y, acc11, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a0[u] = na0[u];
+            a1[u] = na1[u];
+            bv[u] = nb[u];
+        }
     }
-    y[(size_t)ldy * (y_by_k ? k : s) + r] = make_double2(re, im);
-}
-
-// DPCM: y = Dx + lambda x ;  IEF (second stage): y = -(D u + lambda u), u = S A^-1 x
-__global__ void k_bem_shift(int n, const int *__restrict__ slots, const double2 *__restrict__ lam,
-                            const double2 *__restrict__ dx, int lddx, int dx_by_k, const double2 *__restrict__ u,
-                            int ldu, int u_by_k, double sign, double2 *__restrict__ y, int ld) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    const int s = slots[k];
-    if (r >= n) return;
-    const double2 a = dx[(size_t)lddx * (dx_by_k ? k : s) + r];
-    const double2 b = cmul(lam[s], u[(size_t)ldu * (u_by_k ? k : s) + r]);
-    y[(size_t)ld * s + r] = make_double2(sign * (a.x + b.x), sign * (a.y + b.y));
+    // accumulator element v of a lane (f64 16x16 layout): fragment row
+    // lane / 16 + 4 v, fragment column lane % 16
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        red[w][(0 * 4 + v) * 64 + lane] = acc00[v];
+        red[w][(1 * 4 + v) * 64 + lane] = acc01[v];
+        red[w][(2 * 4 + v) * 64 + lane] = acc10[v];
+        red[w][(3 * 4 + v) * 64 + lane] = acc11[v];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        const double sum = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+        const int l = e & 63, v = (e >> 6) & 3, fc = (e >> 8) & 1, fr = e >> 9;
+        const int r = r0 + 2 * ((l >> 4) + 4 * v) + fr;
+        const int col = fc * 16 + (l & 15);
+        const int k = ct * kGemmSys + col / 2, part = col & 1;
+        if (r >= n || k >= nsys) continue;
+        const int slot = slots[k];
+        double val = sum;
+        if (o.lam) {
+            const double2 l2 = o.lam[slot];
+            const double2 uu = o.u[(size_t)o.ldu * (o.u_by_k ? k : slot) + r];
+            val += part ? l2.x * uu.y + l2.y * uu.x : l2.x * uu.x - l2.y * uu.y;
+        }
+        reinterpret_cast<double *>(o.y)[2 * ((size_t)o.ldy * (o.y_by_k ? k : slot) + r) + part] = o.sign * val;
+    }
 }
 
 __global__ void k_bem_rhs_dpcm(int n, const double *__restrict__ normal, double e0, double2 *__restrict__ rhs) {
@@ -388,32 +484,35 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
     Mesh &M = c->mesh;
     const int n = M.n;
     hipStream_t st = c->stream;
-    c->work_part.alloc((size_t)nsys * n);
     c->work_t.alloc((size_t)nsys * n);
-    dim3 g((n + 255) / 256, nsys);
-    if (M.variant == PXH_DPCM) {
+    // K steps of 4 columns, split over the 4 waves of a workgroup in
+    // multiples of 4; the packed X is zero-padded to cover every wave's range
+    const int ksteps = (n + 3) / 4;
+    const int spw = (ksteps + 15) / 16 * 4;
+    const int cpad = 16 * spw;
+    const int coltiles = (nsys + kGemmSys - 1) / kGemmSys;
+    const int rowtiles = (n + kGemmRows - 1) / kGemmRows;
+    c->work_pack.alloc((size_t)coltiles * cpad * 32);
+    const size_t pack_items = (size_t)coltiles * cpad * kGemmSys;
+    const unsigned pack_blocks = (unsigned)((pack_items + 255) / 256);
+    const unsigned gemm_blocks = 8u * (unsigned)((rowtiles * coltiles + 7) / 8);
+    auto gemm = [&](const double *Md, const float *Mf, const double2 *xin, int x_by_k, const GemmOut &o) {
+        k_gemm_pack<<<pack_blocks, 256, 0, st>>>(n, cpad, nsys, d_sys_list, xin, x_by_k ? n : ld, x_by_k, coltiles,
+                                                 c->work_pack.p);
         if (fp32)
-            k_real_matvec<float><<<g, 256, 0, st>>>(n, M.Df.p, d_sys_list, x, ld, c->work_part.p, n, 1);
+            k_bem_gemm<float><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, Mf, c->work_pack.p,
+                                                           d_sys_list, nsys, o);
         else
-            k_real_matvec<double><<<g, 256, 0, st>>>(n, M.D.p, d_sys_list, x, ld, c->work_part.p, n, 1);
-        k_bem_shift<<<g, 256, 0, st>>>(n, d_sys_list, d_lambda, c->work_part.p, n, 1, x, ld, 0, 1.0, y, ld);
+            k_bem_gemm<double><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, Md, c->work_pack.p,
+                                                            d_sys_list, nsys, o);
+    };
+    if (M.variant == PXH_DPCM) {
+        // y = D x + lambda x
+        gemm(M.D.p, M.Df.p, x, 0, GemmOut{d_lambda, x, ld, 0, 1.0, y, ld, 0});
     } else {
         // u = S A^-1 x, then y = -(D u + lambda u)
-        if (fp32)
-            k_real_matvec<float><<<g, 256, 0, st>>>(n, M.SAf.p, d_sys_list, x, ld, c->work_t.p, n, 1);
-        else
-            k_real_matvec<double><<<g, 256, 0, st>>>(n, M.SA.p, d_sys_list, x, ld, c->work_t.p, n, 1);
-        // D u: u is indexed by k (position in the list); use an identity list
-        std::vector<int> ident(nsys);
-        for (int k = 0; k < nsys; ++k) ident[k] = k;
-        DevBuf<int> id;
-        id.upload(ident.data(), nsys, st);
-        if (fp32)
-            k_real_matvec<float><<<g, 256, 0, st>>>(n, M.Df.p, id.p, c->work_t.p, n, c->work_part.p, n, 1);
-        else
-            k_real_matvec<double><<<g, 256, 0, st>>>(n, M.D.p, id.p, c->work_t.p, n, c->work_part.p, n, 1);
-        k_bem_shift<<<g, 256, 0, st>>>(n, d_sys_list, d_lambda, c->work_part.p, n, 1, c->work_t.p, n, 1, -1.0, y, ld);
-        PXH_HIP(hipStreamSynchronize(st));
+        gemm(M.SA.p, M.SAf.p, x, 0, GemmOut{nullptr, nullptr, 0, 0, 1.0, c->work_t.p, n, 1});
+        gemm(M.D.p, M.Df.p, c->work_t.p, 1, GemmOut{d_lambda, c->work_t.p, n, 1, -1.0, y, ld, 0});
     }
     PXH_HIP(hipGetLastError());
     c->stats.v[6] += (double)n * n * nsys;

@@ -358,35 +358,27 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     c->work_scale.alloc(32 * NCB);
     const float tsc = mfma_t_scale(A);
     const float rfar2 = (float)far_field_r2(A, false);
-    PXH_HIP(hipEventRecord(c->ev0, c->stream));
+    const int t_product = timer_begin(c);
     for (int g = 0; g < groups; ++g) {
         const int ns = std::min(SYS_PER_GROUP, nsys - g * SYS_PER_GROUP);
         const int *lst = d_sys_list + g * SYS_PER_GROUP;
         k_col_scale<<<32 * NCB, 256, 0, c->stream>>>(nv, ns, lst, x, ld, c->work_scale.p, 32 * NCB);
         k_prep_x<NCB><<<jsteps, 256, 0, c->stream>>>(n, ns, lst, x, ld, c->work_scale.p, c->work_xh.p,
                                                      c->work_xl.p);
-        PXH_HIP(hipEventRecord(c->ev2, c->stream));
+        // the pair kernel alone, timed on the library's stream
+        const int t_kernel = timer_begin(c);
         k_tsweep_mfma<NCB><<<iblocks * splits, 256, 0, c->stream>>>(
             n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p,
             c->work_xl.p, c->work_part.p, nv, splits, per, nsys, g * SYS_PER_GROUP);
-        PXH_HIP(hipEventRecord(c->ev3, c->stream));
-        // the pair kernel alone, timed on the library's stream
-        PXH_HIP(hipEventSynchronize(c->ev3));
-        float kms = 0.f;
-        PXH_HIP(hipEventElapsedTime(&kms, c->ev2, c->ev3));
-        c->stats.v[12] += kms;
+        timer_end(c, t_kernel, 12);
         c->stats.v[13] += 1;
     }
     PXH_HIP(hipGetLastError());
-    PXH_HIP(hipEventRecord(c->ev1, c->stream));
+    timer_end(c, t_product, 3);
     finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true);
-    PXH_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    PXH_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     // algorithmic flops: the contraction of the 4N x 4N real T with 2 nsys real
     // columns (2 flops per element and column) plus the pair evaluation
     const double nn = (double)n * (double)(n - 1);
-    c->stats.v[3] += ms;
     c->stats.v[8] += groups;
     c->stats.v[10] += nn * 16.0 * 2.0 * 2.0 * nsys + nn * 42.0 * groups;
     c->stats.v[6] += (double)n * n * groups;

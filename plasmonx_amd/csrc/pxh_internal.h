@@ -132,13 +132,25 @@ struct Stats {
     double v[16] = {};
 };
 
+// Kernel timers that do not stall the host: a ring of event pairs recorded
+// on the context's stream.  An elapsed time is added into its stats entry
+// once the pair is known complete: when the ring comes round to it again, or
+// at drain (end of a solve, reading the stats).
+struct TimerRing {
+    static constexpr int kN = 64;
+    hipEvent_t beg[kN] = {}, end[kN] = {};
+    int stat[kN] = {};
+    bool open[kN] = {}, pending[kN] = {};
+    int head = 0;
+};
+
 }  // namespace pxh
 
 struct pxh_ctx {
     int device = 0;
     size_t budget = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    pxh::TimerRing timers;
     pxh::Atoms atoms;
     pxh::Mesh mesh;
     int target = 0;                       // PXH_WFQ / PXH_WFQFMU / PXH_BEM
@@ -152,10 +164,16 @@ struct pxh_ctx {
     pxh::DevBuf<double2> work_part, work_t;  // scratch of the operator products
     pxh::DevBuf<_Float16> work_xh, work_xl;   // fp16 hi / lo operand of the MFMA products
     pxh::DevBuf<float> work_scale;            // per-column scales of that operand
+    pxh::DevBuf<double> work_pack;            // packed operand of the BEM matrix products
     bool use_stored = false;                  // in-memory method: products use the stored T
 };
 
 namespace pxh {
+// api.hip: timers on the context's stream (TimerRing); timer_begin returns -1
+// when the ring has no free pair, and timer_end(-1) does nothing.
+int timer_begin(pxh_ctx *c);
+void timer_end(pxh_ctx *c, int t, int stat_index);
+void timer_drain(pxh_ctx *c);
 // kernels_atoms.hip
 void build_atoms_device(pxh_ctx *c);
 void build_kbar(pxh_ctx *c);

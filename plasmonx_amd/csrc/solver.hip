@@ -488,6 +488,28 @@ struct System {
     int ifreq, pol;
 };
 
+struct HostRing {
+    char *base = nullptr;
+    size_t chunk = 0;
+    int count = 0, next = 0;
+    HostRing() = default;
+    HostRing(const HostRing &) = delete;
+    HostRing &operator=(const HostRing &) = delete;
+    ~HostRing() {
+        if (base) (void)hipHostFree(base);
+    }
+    void init(size_t chunk_bytes, int n) {
+        chunk = (chunk_bytes + 255) / 256 * 256;
+        count = n;
+        PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&base), chunk * (size_t)n, hipHostMallocDefault));
+    }
+    void *take() {
+        void *p = base + chunk * (size_t)next;
+        next = (next + 1) % count;
+        return p;
+    }
+};
+
 struct Batched {
     pxh_ctx *c;
     int nv, m, B;
@@ -508,16 +530,27 @@ struct Batched {
     std::vector<int> jcount;
     size_t ldv_slot = 0, ldH_slot = 0;
     int ldH = 0, ldg = 0, ldh = 0;
+    HostRing ring;                 // pinned staging of the uploads
+    double *pin_res = nullptr;     // pinned readback of per-slot reals (B)
+    ~Batched() {
+        if (pin_res) (void)hipHostFree(pin_res);
+    }
 };
 
-// The host copies stay alive in S until the next synchronisation point.
-void upload_list(Batched &S, const std::vector<int> &list) {
-    S.h_list = list;
-    S.slots_d.alloc(std::max<size_t>(1, (size_t)S.B));
-    PXH_HIP(hipMemcpyAsync(S.slots_d.p, S.h_list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
-                           S.c->stream));
-    PXH_HIP(hipStreamSynchronize(S.c->stream));
+// Small host-to-device uploads go through a ring of pinned chunks, so that
+// the copies are asynchronous and need no synchronisation of their own: the
+// loop synchronises once per step (to read the residual estimates), and a
+// step uploads far fewer chunks than the ring holds.
+template <typename T>
+void upload(Batched &S, T *d_dst, const T *src, size_t n) {
+    const size_t bytes = sizeof(T) * n;
+    PXH_CHECK(bytes <= S.ring.chunk, PXH_EINVAL, "upload larger than a staging chunk");
+    void *p = S.ring.take();
+    std::memcpy(p, src, bytes);
+    PXH_HIP(hipMemcpyAsync(d_dst, p, bytes, hipMemcpyHostToDevice, S.c->stream));
 }
+
+void upload_list(Batched &S, const std::vector<int> &list) { upload(S, S.slots_d.p, list.data(), list.size()); }
 
 // hbuf[s][i] += hacc[s][i] for i <= j_s
 __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ jcount, double2 *__restrict__ h,
@@ -611,6 +644,9 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         S.h_par.assign(B, SysParam{});
     }
     S.jcount.assign(B, 0);
+    S.slots_d.alloc(std::max<size_t>(1, (size_t)B));
+    S.ring.init(std::max(sizeof(SysParam), sizeof(int) * (size_t)B), 4 * B + 64);
+    PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * (size_t)B, hipHostMallocDefault));
 
     enum Phase { FREE, INNER, NEED_RES };
     std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0);
@@ -637,10 +673,10 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
             const double den = dr * dr + di * di;
             S.h_lam[s] = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
-            PXH_HIP(hipMemcpyAsync(S.lam.p + s, &S.h_lam[s], sizeof(double2), hipMemcpyHostToDevice, st));
+            upload(S, S.lam.p + s, &S.h_lam[s], 1);
         } else {
             S.h_par[s] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
-            PXH_HIP(hipMemcpyAsync(S.par.p + s, &S.h_par[s], sizeof(SysParam), hipMemcpyHostToDevice, st));
+            upload(S, S.par.p + s, &S.h_par[s], 1);
         }
         k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, s, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
                                                     S.Bv.p, S.R.p, S.X.p);
@@ -681,8 +717,9 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             }
             upload_list(S, need);
             k_norm<<<(int)need.size(), RB, 0, st>>>(nv, S.slots_d.p, S.R.p, (size_t)nv, S.nrm.p);
-            PXH_HIP(hipMemcpyAsync(hbeta.data(), S.nrm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipMemcpyAsync(S.pin_res, S.nrm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
             PXH_HIP(hipStreamSynchronize(st));
+            std::memcpy(hbeta.data(), S.pin_res, sizeof(double) * B);
             std::vector<int> restart;
             for (int s : need) {
                 const bool first = std::find(res_first.begin(), res_first.end(), s) != res_first.end();
@@ -736,8 +773,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         for (int s : act) jmax = std::max(jmax, S.jcount[s]);
         upload_list(S, act);
         // W = A V_j: V_j of each slot is staged at stage + nv * slot (the operator's layout)
-        S.h_jc = S.jcount;
-        PXH_HIP(hipMemcpyAsync(S.jcount_d.p, S.h_jc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+        upload(S, S.jcount_d.p, S.jcount.data(), (size_t)B);
         k_stage<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot,
                                                   S.precond ? S.dinv.p : nullptr, S.stage.p);
         apply_op(S, na, !S.mixed, S.stage.p, S.W.p);
@@ -748,7 +784,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         c->stats.v[0] += na;
         // jcount_d holds j; the GS uses j+1 columns
         for (int s : act) S.h_jp1[s] = S.jcount[s] + 1;
-        PXH_HIP(hipMemcpyAsync(S.jp1_d.p, S.h_jp1.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+        upload(S, S.jp1_d.p, S.h_jp1.data(), (size_t)B);
         int *jp1_dp = S.jp1_d.p;
         const bool mgs = plain && (jmax + 1 < 25);
         if constexpr (std::is_same<VT, double2>::value) {
@@ -794,8 +830,9 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                                                 S.ldH, S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg, S.resid.p);
         PXH_HIP(hipGetLastError());
         std::vector<double> hres(B);
-        PXH_HIP(hipMemcpyAsync(hres.data(), S.resid.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipMemcpyAsync(S.pin_res, S.resid.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
         PXH_HIP(hipStreamSynchronize(st));
+        std::memcpy(hres.data(), S.pin_res, sizeof(double) * B);
         std::vector<int> fin;
         for (int s : act) {
             S.jcount[s] += 1;
@@ -805,7 +842,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         if (!fin.empty()) {
             upload_list(S, fin);
             for (int s : fin) S.h_jl[s] = S.jcount[s];
-            PXH_HIP(hipMemcpyAsync(S.jl_d.p, S.h_jl.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+            upload(S, S.jl_d.p, S.h_jl.data(), (size_t)B);
             int *jl_dp = S.jl_d.p;
             k_trsv<<<(int)fin.size(), 64, 0, st>>>(S.slots_d.p, jl_dp, S.H.p, S.ldH, S.ldH_slot, S.g.p, S.ldg,
                                                    S.y.p, S.ldh);
@@ -1024,6 +1061,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     if (variables)
         PXH_HIP(hipMemcpyAsync(variables, sol.p, sizeof(double2) * stride * n_freq, hipMemcpyDeviceToHost, st));
     PXH_HIP(hipStreamSynchronize(st));
+    timer_drain(c);
     for (int f = 0; f < n_freq; ++f) {
         double pol[18];
         for (int k = 0; k < 9; ++k) {

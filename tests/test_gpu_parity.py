@@ -170,6 +170,43 @@ def test_bem_matches_oracle_and_log(gpu_ctx, case):
     assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5 if meth == 0 else 1e-4, 1e-6).all()
 
 
+@pytest.mark.parametrize("case", ["bem_dpcm_accurate_inversion", "bem_ief_accurate_inversion",
+                                  "bem_ief_approximate_inversion"])
+@pytest.mark.parametrize("drop", [0, 1])
+@pytest.mark.parametrize("nvec", [1, 17, 40])
+def test_bem_matvec_matches_oracle(gpu_ctx, case, drop, nvec):
+    """The BEM operator products (matrix-core FP64 GEMM) against the oracle's
+    D and S A^-1: DPCM y = (D + lambda I) x, IEF y = -(D + lambda I) S A^-1 x,
+    with lambda = 2 pi (eps_s + eps) / (eps_s - eps).  284 tesserae (a
+    partial row tile) or 283 (odd), 1..40 vectors (one, two and three column
+    tiles); the FP32-matrix product within 1e-6."""
+    import dataclasses
+    fx = load_fixture(case)
+    m = mesh_from_fixture(fx)
+    if drop:
+        m = dataclasses.replace(m, verts_bohr=m.verts_bohr[:m.n - drop])
+    freq = np.array([3.2, 3.6]) / 27.211386245981
+    eps = pm.silver_etchegoin(freq)
+    gpu_ctx.set_mesh(m)
+    gpu_ctx.set_permittivity(eps)
+    D, Di, SA = pxo.bem_d(m)
+    lam = 2 * np.pi * (m.eps_solvent + eps[1]) / (m.eps_solvent - eps[1])
+    A = D + lam * np.eye(m.n)
+    if m.variant == "iefpcm":
+        A = -A @ SA
+    rng = np.random.default_rng(nvec + 100 * drop)
+    x = rng.normal(size=(nvec, m.n)) + 1j * rng.normal(size=(nvec, m.n))
+    ref = x @ A.T
+    got = gpu_ctx.matvec(freq[1], 1, x)
+    for k in range(nvec):
+        assert relerr(got[k], ref[k]) < 1e-12, k
+    gpu_ctx.set_option("matvec_mixed", 1)
+    got = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("matvec_mixed", 0)
+    for k in range(nvec):
+        assert relerr(got[k], ref[k]) < 1e-6, k
+
+
 @pytest.mark.parametrize("mfma", [0, 1])
 @pytest.mark.parametrize("nvec", [1, 5, 50])
 def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
