@@ -266,12 +266,7 @@ int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *
         dl.upload(list.data(), n_vec, c->stream);
         if (bem) {
             PXH_CHECK(i_freq >= 0 && i_freq < c->eps_nf, PXH_EINVAL, "i_freq outside the permittivity table");
-            const double *e = c->eps.data() + 2 * (size_t)i_freq;
-            const double es = c->mesh.eps_solvent;
-            const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
-            const double den = dr * dr + di * di;
-            const double2 l = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
-            std::vector<double2> hl(n_vec, l);
+            std::vector<double2> hl(n_vec, pxh::bem_lambda(c, i_freq));
             pxh::DevBuf<double2> dlam;
             dlam.upload(hl.data(), n_vec, c->stream);
             pxh::apply_bem(c, n_vec, dl.p, dlam.p, dx.p, dy.p, nv, c->opt.matvec_mixed != 0);
@@ -351,6 +346,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.inner_tol = value;
         } else if (k == "splits") {
             c->opt.splits = (int)value;
+        } else if (k == "multishift") {
+            c->opt.multishift = value != 0.0;
         } else if (k == "matvec_mixed") {
             c->opt.matvec_mixed = value != 0.0;
         } else if (k == "mfma") {

@@ -126,6 +126,7 @@ struct Options {
     int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
     int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
     double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
+    int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies
 };
 
 struct Stats {
@@ -201,6 +202,8 @@ void dense_t_atoms(pxh_ctx *c, double *d_T, bool zero_qq_diag);
 void dynamic_rhs_atoms(pxh_ctx *c, double freq, double e0, int rhs_form, double2 *d_rhs /* nv x 3 */);
 void constant_matrix_atoms(pxh_ctx *c, double *d_m);
 SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq);
+// BEM shift lambda(omega) of entry i_freq of the permittivity table (solver.hip)
+double2 bem_lambda(const pxh_ctx *c, int i_freq);
 // dinv = 1 / diag(A(omega)) of one system (n_var complex, device)
 void precond_diag_atoms(pxh_ctx *c, const SysParam *d_par_one, double2 *dinv);
 void precond_diag_bem(pxh_ctx *c, const double2 *d_lambda_one, double2 *dinv);

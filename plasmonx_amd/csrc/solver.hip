@@ -562,6 +562,165 @@ __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ j
     }
 }
 
+// ------------------------------------------------------------ multi-shift
+// Shifted systems (A0 + sigma_s I) x_s = b, x_s(0) = 0, share one Krylov
+// space K_m(A0, b): a single Arnoldi process per right-hand side serves every
+// shift.  The Hessenberg matrix of shift s is H + sigma_s [I; 0]; each shift
+// keeps its own Givens rotations and rotated right-hand side g, whose
+// |g_{j+1}| is that shift's GMRES residual norm.  The state of (shift s,
+// right-hand side f) sits at q = f nsh + s, ldg entries each.
+
+// Column j of the Arnoldi matrix of each listed right-hand side:
+// Hs[f][j][0..j] = h, Hs[f][j][j+1] = ||w||.
+__global__ void k_ms_store_col(const int *__restrict__ slots, const int *__restrict__ jcol,
+                               const double2 *__restrict__ hbuf, int ldh, const double *__restrict__ nrm,
+                               double2 *__restrict__ Hs, int ldH, size_t ldH_f) {
+    const int f = slots[blockIdx.x];
+    const int j = jcol[f];
+    double2 *Hc = Hs + ldH_f * f + (size_t)ldH * j;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) Hc[i] = hbuf[(size_t)f * ldh + i];
+    if (threadIdx.x == 0) Hc[j + 1] = make_double2(nrm[f], 0.0);
+}
+
+// g[q] = (beta_f, 0, ...) for every shift of the listed right-hand sides.
+__global__ void k_ms_init(int nsh, const int *__restrict__ slots, const double *__restrict__ beta,
+                          double2 *__restrict__ g, int ldg) {
+    const int f = slots[blockIdx.y];
+    double2 *gq = g + ((size_t)f * nsh + blockIdx.x) * ldg;
+    for (int i = threadIdx.x; i < ldg; i += blockDim.x) gq[i] = make_double2(i == 0 ? beta[f] : 0.0, 0.0);
+}
+
+// Step j for every shift (grid nsh x listed right-hand sides): the stored
+// rotations 0..j-1 applied to column j of H + sigma_s I, then the rotation
+// that zeroes its subdiagonal, applied to g as well; resid[q] = |g_{j+1}|.
+// Lane 0 runs the rotation chain; the rotated column itself is not kept.
+__global__ __launch_bounds__(64) void k_ms_givens(int nsh, const int *__restrict__ slots,
+                                                  const int *__restrict__ jcol, const double2 *__restrict__ Hs,
+                                                  int ldH, size_t ldH_f, const double2 *__restrict__ sigma,
+                                                  double *__restrict__ cs, double2 *__restrict__ sn,
+                                                  double2 *__restrict__ g, int ldg, double *__restrict__ resid) {
+    extern __shared__ double2 sh_ms[];  // col[ldH] | sn[ldg] | cs[ldg] (doubles)
+    double2 *col = sh_ms;
+    double2 *snl = sh_ms + ldH;
+    double *csl = reinterpret_cast<double *>(sh_ms + ldH + ldg);
+    const int s = blockIdx.x;
+    const int f = slots[blockIdx.y];
+    const int j = jcol[f];
+    const size_t q = (size_t)f * nsh + s;
+    const double2 *Hc = Hs + ldH_f * f + (size_t)ldH * j;
+    for (int i = threadIdx.x; i <= j + 1; i += 64) col[i] = Hc[i];
+    for (int i = threadIdx.x; i < j; i += 64) {
+        csl[i] = cs[q * ldg + i];
+        snl[i] = sn[q * ldg + i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double2 sg = sigma[s];
+    col[j].x += sg.x;
+    col[j].y += sg.y;
+    double2 carry = col[0];
+    for (int i = 0; i < j; ++i) {
+        double2 nx = col[i + 1];
+        zrot_d(carry, nx, csl[i], snl[i]);
+        carry = nx;
+    }
+    double c;
+    double2 sv;
+    zrotg_d(carry, col[j + 1], c, sv);
+    cs[q * ldg + j] = c;
+    sn[q * ldg + j] = sv;
+    double2 *gq = g + q * ldg;
+    zrot_d(gq[j], gq[j + 1], c, sv);
+    resid[q] = hypot(gq[j + 1].x, gq[j + 1].y);
+}
+
+// y[q] = R^-1 g[q](0:J) for every shift, R the rotated (H + sigma_s I)(0:J,
+// 0:J).  Column-oriented back substitution from the last column; each column
+// of R is rebuilt from H and the stored rotations when it is needed, so no
+// shift stores its triangle.
+__global__ __launch_bounds__(64) void k_ms_solve(int nsh, const int *__restrict__ slots,
+                                                 const int *__restrict__ jlen, const double2 *__restrict__ Hs,
+                                                 int ldH, size_t ldH_f, const double2 *__restrict__ sigma,
+                                                 const double *__restrict__ cs, const double2 *__restrict__ sn,
+                                                 const double2 *__restrict__ g, int ldg, double2 *__restrict__ y,
+                                                 int ldy) {
+    extern __shared__ double2 sh_ms[];  // col[ldH] | gl[ldg] | sn[ldg] | cs[ldg] (doubles)
+    double2 *col = sh_ms;
+    double2 *gl = col + ldH;
+    double2 *snl = gl + ldg;
+    double *csl = reinterpret_cast<double *>(snl + ldg);
+    const int s = blockIdx.x;
+    const int f = slots[blockIdx.y];
+    const int J = jlen[f];
+    const size_t q = (size_t)f * nsh + s;
+    for (int i = threadIdx.x; i < J; i += 64) {
+        gl[i] = g[q * ldg + i];
+        csl[i] = cs[q * ldg + i];
+        snl[i] = sn[q * ldg + i];
+    }
+    const double2 sg = sigma[s];
+    for (int jj = J - 1; jj >= 0; --jj) {
+        __syncthreads();
+        const double2 *Hc = Hs + ldH_f * f + (size_t)ldH * jj;
+        for (int i = threadIdx.x; i <= jj + 1; i += 64) col[i] = Hc[i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            col[jj].x += sg.x;
+            col[jj].y += sg.y;
+            double2 carry = col[0];
+            for (int i = 0; i <= jj; ++i) {
+                double2 nx = col[i + 1];
+                zrot_d(carry, nx, csl[i], snl[i]);
+                col[i] = carry;
+                carry = nx;
+            }
+            const double2 r = col[jj];
+            const double den = r.x * r.x + r.y * r.y;
+            const double2 v = gl[jj];
+            gl[jj] = make_double2((v.x * r.x + v.y * r.y) / den, (v.y * r.x - v.x * r.y) / den);
+        }
+        __syncthreads();
+        const double2 yj = gl[jj];
+        for (int i = threadIdx.x; i < jj; i += 64) {
+            const double2 p = cmul(col[i], yj);
+            gl[i].x -= p.x;
+            gl[i].y -= p.y;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < J; i += 64) y[q * ldy + i] = gl[i];
+}
+
+// x of every shift of one right-hand side f: V_f(:, 0:J) y[q], written to
+// sol + stride s + nv pol (the solution layout [frequency][nv x 3]).
+constexpr int kMsShifts = 8;
+__global__ __launch_bounds__(256) void k_ms_update_x(int nv, int nsh, int f, int pol, const int *__restrict__ jlen,
+                                                     const double2 *__restrict__ V, size_t ldv_slot,
+                                                     const double2 *__restrict__ y, int ldy,
+                                                     double2 *__restrict__ sol, size_t stride) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s0 = blockIdx.y * kMsShifts;
+    if (r >= nv) return;
+    const int J = jlen[f];
+    const int ns = min(kMsShifts, nsh - s0);
+    const double2 *v = V + ldv_slot * f + r;
+    const double2 *yq = y + ((size_t)f * nsh + s0) * ldy;
+    double2 acc[kMsShifts];
+#pragma unroll
+    for (int k = 0; k < kMsShifts; ++k) acc[k] = make_double2(0, 0);
+    for (int i = 0; i < J; ++i) {
+        const double2 vi = v[(size_t)nv * i];
+#pragma unroll
+        for (int k = 0; k < kMsShifts; ++k) {
+            if (k >= ns) break;
+            const double2 p = cmul(vi, yq[(size_t)k * ldy + i]);
+            acc[k].x += p.x;
+            acc[k].y += p.y;
+        }
+    }
+    for (int k = 0; k < ns; ++k) sol[stride * (s0 + k) + (size_t)nv * pol + r] = acc[k];
+}
+
 }  // namespace
 
 // Operator application for slots in `list` (device copy in S.slots_d).
@@ -578,12 +737,22 @@ static void apply_op(Batched &S, int nsl, bool fp64, const double2 *x, double2 *
     }
 }
 
+// lambda(omega) = 2 pi (eps_s + eps) / (eps_s - eps)  (bem_module.f90:898-906)
+double2 bem_lambda(const pxh_ctx *c, int i_freq) {
+    const double *e = c->eps.data() + 2 * (size_t)i_freq;
+    const double es = c->mesh.eps_solvent;
+    const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
+    const double den = dr * dr + di * di;
+    return make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
+}
+
 // Solve all (ifreq, pol) systems.  rhs_all: device, [n_freq][nv x 3].
 // VT: element type of the Krylov basis.
+// warm: the systems start from the solutions already in d_sol (else x = 0).
 template <typename VT>
 static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
                          int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
-                         double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
+                         bool warm, double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
                          std::vector<double> &tres_out, std::vector<double> &t_admit, std::vector<double> &t_done) {
     Batched S;
     DevBuf<VT> V;
@@ -667,12 +836,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         t_admit[next] = now_s();
         sys_of[s] = (int)next++;
         if (c->target == PXH_BEM) {
-            const double *e = c->eps.data() + 2 * (size_t)sy.ifreq;
-            // lambda(omega) = 2 pi (eps_s + eps)/(eps_s - eps)  (bem_module.f90:898-906)
-            const double es = c->mesh.eps_solvent;
-            const double nr = es + e[0], ni = e[1], dr = es - e[0], di = -e[1];
-            const double den = dr * dr + di * di;
-            S.h_lam[s] = make_double2(2.0 * M_PI * (nr * dr + ni * di) / den, 2.0 * M_PI * (ni * dr - nr * di) / den);
+            S.h_lam[s] = bem_lambda(c, sy.ifreq);
             upload(S, S.lam.p + s, &S.h_lam[s], 1);
         } else {
             S.h_par[s] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
@@ -686,9 +850,12 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             else
                 precond_diag_atoms(c, S.par.p + s, S.dinv.p + (size_t)nv * s);
         }
+        if (warm)
+            PXH_HIP(hipMemcpyAsync(S.X.p + (size_t)nv * s, d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
+                                   sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
         outer[s] = 0;
         total_it[s] = 0;
-        phase[s] = NEED_RES;  // the residual of x = 0 is b: computed below without a product
+        phase[s] = NEED_RES;  // with x = 0 the residual is b: computed below without a product
     };
 
     std::vector<int> res_first;  // slots whose residual is b (x = 0)
@@ -698,7 +865,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         for (int s = 0; s < B && next < systems.size(); ++s)
             if (phase[s] == FREE) {
                 admit(s);
-                res_first.push_back(s);
+                if (!warm) res_first.push_back(s);
             }
         // ----- residual checks (FP64)
         std::vector<int> need;
@@ -858,15 +1025,285 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
 }
 
 static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
-                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
+                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain, bool warm,
                        double2 *d_sol, std::vector<int> &iters_out, std::vector<double> &tres_out,
                        std::vector<double> &t_admit, std::vector<double> &t_done) {
     if (precision == PXH_MIXED && !plain)
-        return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
-                                     iters_out, tres_out, t_admit, t_done);
-    return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, d_sol,
-                                  iters_out, tres_out, t_admit, t_done);
+        return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, warm,
+                                     d_sol, iters_out, tres_out, t_admit, t_done);
+    return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, warm,
+                                  d_sol, iters_out, tres_out, t_admit, t_done);
 }
+
+// Frequency families with a scalar shift: A(omega_f) = A0 + sigma_f I and a
+// frequency-independent right-hand side.  BEM DPCM: A0 = D, sigma = lambda
+// (bem_module.f90:898-906); homogeneous omegaFQ: A0 = K-bar T, sigma = -z_q
+// (wfq_module.f90:234-246).  The omegaFQFmu operator shifts its charge and
+// dipole blocks by different amounts and is no such family.
+static bool shift_family(const pxh_ctx *c, int n_freq, const double *freq, std::vector<double2> &sigma) {
+    sigma.assign(n_freq, make_double2(0, 0));
+    if (c->target == PXH_BEM) {
+        if (c->mesh.variant != PXH_DPCM) return false;
+        for (int f = 0; f < n_freq; ++f) sigma[f] = bem_lambda(c, f);
+        return true;
+    }
+    const Atoms &A = c->atoms;
+    if (A.model != PXH_WFQ || A.hetero) return false;
+    for (int f = 0; f < n_freq; ++f) {
+        const SysParam p = make_sysparam(c, freq[f], f);
+        for (int t = 1; t < A.nt; ++t)
+            if (p.zq[t].x != p.zq[0].x || p.zq[t].y != p.zq[0].y) return false;
+        sigma[f] = make_double2(-p.zq[0].x, -p.zq[0].y);
+    }
+    return true;
+}
+
+namespace {
+struct PinnedDoubles {
+    double *p = nullptr;
+    explicit PinnedDoubles(size_t n) {
+        PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), sizeof(double) * std::max<size_t>(n, 1),
+                              hipHostMallocDefault));
+    }
+    ~PinnedDoubles() {
+        if (p) (void)hipHostFree(p);
+    }
+    PinnedDoubles(const PinnedDoubles &) = delete;
+    PinnedDoubles &operator=(const PinnedDoubles &) = delete;
+};
+}  // namespace
+
+// Multi-shift GMRES over a shift family (shift_family): for each field
+// direction one FP64 Arnoldi process on A0 (CGS2, x0 = 0) serves all
+// frequencies, and runs until every frequency's GMRES residual is below tol/2
+// or the basis holds krylov_dim vectors.  Every solution is then checked
+// against its own FP64 true residual b - A(omega) x; the systems above tol
+// (and those not converged in the basis) are listed in `unsolved`, their
+// current iterate left in d_sol as a starting point.
+static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems, int n_freq,
+                          const std::vector<double2> &sigma, const double *freq, const double2 *d_rhs_all,
+                          int rhs_stride, double tol, int kdim, double2 *d_sol, std::vector<int> &iters_out,
+                          std::vector<double> &tres_out, std::vector<double> &t_done, std::vector<int> &unsolved) {
+    hipStream_t st = c->stream;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
+    const bool bem = c->target == PXH_BEM;
+    std::vector<int> pols;
+    for (const System &sy : systems)
+        if (std::find(pols.begin(), pols.end(), sy.pol) == pols.end()) pols.push_back(sy.pol);
+    const int F = (int)pols.size();
+    const int nsh = n_freq;
+    const int m = std::max(1, std::min(kdim, nv));
+    Batched S;
+    S.c = c;
+    S.nv = nv;
+    S.m = m;
+    S.B = std::max(F, std::max(1, std::min<int>(c->opt.batch, (int)systems.size())));
+    const int B = S.B;
+    S.ldv_slot = (size_t)nv * (m + 1);
+    S.ldH = m + 2;
+    S.ldg = m + 2;
+    S.ldh = m + 2;
+    const int ldH = S.ldH, ldg = S.ldg, ldh = S.ldh;
+    const size_t ldH_f = (size_t)ldH * (m + 1);
+    const int nch = (nv + kDotRows - 1) / kDotRows;
+    const int nb = (nv + 255) / 256;
+    S.X.alloc((size_t)nv * B);
+    S.Bv.alloc((size_t)nv * B);
+    S.R.alloc((size_t)nv * B);
+    S.W.alloc((size_t)nv * B);
+    S.stage.alloc((size_t)nv * B);
+    S.dpart.alloc((size_t)F * ldh * nch);
+    S.wpart.alloc((size_t)F * nch);
+    S.npart.alloc((size_t)F * nb);
+    S.wn2.alloc(F);
+    S.jc2_d.alloc(F);
+    S.g.alloc((size_t)ldg * F);
+    S.hbuf.alloc((size_t)ldh * F);
+    S.hacc.alloc((size_t)ldh * F);
+    S.nrm.alloc(B);
+    S.jcount_d.alloc(B);
+    S.jp1_d.alloc(B);
+    S.jl_d.alloc(B);
+    S.slots_d.alloc(B);
+    S.ring.init(std::max(sizeof(SysParam), sizeof(int) * (size_t)B), 4 * B + 64);
+    if (bem) {
+        S.lam.alloc(B);
+        S.h_lam.assign(B, make_double2(0, 0));
+    } else {
+        S.par.alloc(B);
+        S.h_par.assign(B, SysParam{});
+    }
+    DevBuf<double2> V, Hs, gq, snq, yq, sig;
+    DevBuf<double> csq, msres;
+    V.alloc(S.ldv_slot * F);
+    Hs.alloc(ldH_f * F);
+    gq.alloc((size_t)nsh * F * ldg);
+    snq.alloc((size_t)nsh * F * ldg);
+    csq.alloc((size_t)nsh * F * ldg);
+    yq.alloc((size_t)nsh * F * ldg);
+    msres.alloc((size_t)nsh * F);
+    sig.upload(sigma.data(), nsh, st);
+    PinnedDoubles pin((size_t)nsh * F + B);
+    auto grid_rows = [&](int nsl) { return dim3((nv + 255) / 256, nsl); };
+
+    // system index of (frequency, field slot)
+    std::vector<int> sys_at((size_t)nsh * F, -1);
+    for (size_t k = 0; k < systems.size(); ++k) {
+        const int f = (int)(std::find(pols.begin(), pols.end(), systems[k].pol) - pols.begin());
+        sys_at[(size_t)f * nsh + systems[k].ifreq] = (int)k;
+    }
+    iters_out.assign(systems.size(), 0);
+    tres_out.assign(systems.size(), 0.0);
+    t_done.assign(systems.size(), 0.0);
+    unsolved.clear();
+
+    // base operator A0 on the field slots, start vectors b / ||b||
+    std::vector<int> all(F);
+    for (int f = 0; f < F; ++f) {
+        all[f] = f;
+        if (bem) {
+            upload(S, S.lam.p + f, &S.h_lam[f], 1);
+        } else {
+            S.h_par[f] = make_sysparam(c, freq[0], 0);
+            for (int t = 0; t < c->atoms.nt; ++t) S.h_par[f].zq[t] = make_double2(0, 0);
+            upload(S, S.par.p + f, &S.h_par[f], 1);
+        }
+        k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, f, d_rhs_all, pols[f], S.Bv.p, S.R.p, S.X.p);
+    }
+    upload_list(S, all);
+    k_norm<<<F, RB, 0, st>>>(nv, S.slots_d.p, S.R.p, (size_t)nv, S.nrm.p);
+    PXH_HIP(hipMemcpyAsync(pin.p, S.nrm.p, sizeof(double) * F, hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipStreamSynchronize(st));
+    std::vector<int> act;
+    for (int f = 0; f < F; ++f)
+        if (pin.p[f] > 0.0) act.push_back(f);  // a zero right-hand side has the solution 0
+    std::vector<int> conv_it((size_t)nsh * F, 0);
+    S.jcount.assign(B, 0);
+    S.h_jp1.assign(B, 0);
+    S.h_jl.assign(B, 0);
+    if (!act.empty()) {
+        upload_list(S, act);
+        k_start_cycle<double2><<<dim3((std::max(nv, ldg) + 255) / 256, (int)act.size()), 256, 0, st>>>(
+            nv, S.slots_d.p, S.R.p, S.nrm.p, V.p, S.ldv_slot, S.g.p, ldg);
+        k_ms_init<<<dim3(nsh, (int)act.size()), 64, 0, st>>>(nsh, S.slots_d.p, S.nrm.p, gq.p, ldg);
+        for (size_t k = 0; k < act.size(); ++k) c->stats.v[7] += 1;
+    }
+    const size_t lds_givens = sizeof(double2) * (ldH + ldg) + sizeof(double) * ldg;
+    const size_t lds_solve = sizeof(double2) * (ldH + 2 * ldg) + sizeof(double) * ldg;
+    while (!act.empty()) {
+        const int na = (int)act.size();
+        upload_list(S, act);
+        upload(S, S.jcount_d.p, S.jcount.data(), (size_t)B);
+        k_stage<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot, nullptr,
+                                                       S.stage.p);
+        apply_op(S, na, true, S.stage.p, S.W.p);
+        c->stats.v[2] += 1;
+        c->stats.v[0] += na;
+        for (int f : act) S.h_jp1[f] = S.jcount[f] + 1;
+        upload(S, S.jp1_d.p, S.h_jp1.data(), (size_t)B);
+        int jmax = 0;
+        for (int f : act) jmax = std::max(jmax, S.jcount[f]);
+        // classical Gram-Schmidt, two passes
+        const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDotGroup - 1) / kDotGroup);
+        const dim3 gs((jmax + 1 + 3) / 4, na);
+        k_dots_part<double2><<<gd, 256, 0, st>>>(nv, S.slots_d.p, S.jp1_d.p, V.p, S.ldv_slot, S.W.p, S.dpart.p, ldh,
+                                                 nch, S.wpart.p);
+        k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, S.jp1_d.p, S.dpart.p, ldh, nch, S.hbuf.p);
+        k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.wpart.p, nch, nullptr, 0, S.wn2.p);
+        k_gs_update<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jp1_d.p, V.p, S.ldv_slot, S.hbuf.p,
+                                                           ldh, S.W.p, S.npart.p);
+        k_reorth_decide<<<na, 64, 0, st>>>(S.slots_d.p, S.jp1_d.p, S.npart.p, nb, S.wn2.p, 2.0, S.jc2_d.p, S.nrm.p);
+        k_dots_part<double2><<<gd, 256, 0, st>>>(nv, S.slots_d.p, S.jc2_d.p, V.p, S.ldv_slot, S.W.p, S.dpart.p, ldh,
+                                                 nch, nullptr);
+        k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, S.jc2_d.p, S.dpart.p, ldh, nch, S.hacc.p);
+        k_gs_update<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jc2_d.p, V.p, S.ldv_slot, S.hacc.p,
+                                                           ldh, S.W.p, S.npart.p);
+        k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.npart.p, nb, S.jc2_d.p, 1, S.nrm.p);
+        k_add_h<<<na, 64, 0, st>>>(S.slots_d.p, S.jc2_d.p, S.hbuf.p, S.hacc.p, ldh);
+        k_scale_into<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jp1_d.p, S.W.p, (size_t)nv, S.nrm.p,
+                                                            V.p, S.ldv_slot);
+        k_ms_store_col<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hbuf.p, ldh, S.nrm.p, Hs.p, ldH, ldH_f);
+        k_ms_givens<<<dim3(nsh, na), 64, lds_givens, st>>>(nsh, S.slots_d.p, S.jcount_d.p, Hs.p, ldH, ldH_f, sig.p,
+                                                           csq.p, snq.p, gq.p, ldg, msres.p);
+        PXH_HIP(hipGetLastError());
+        PXH_HIP(hipMemcpyAsync(pin.p, msres.p, sizeof(double) * nsh * F, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipMemcpyAsync(pin.p + (size_t)nsh * F, S.nrm.p, sizeof(double) * F, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipStreamSynchronize(st));
+        std::vector<int> fin, keep;
+        for (int f : act) {
+            S.jcount[f] += 1;
+            const int j = S.jcount[f];
+            bool all_conv = true;
+            for (int s = 0; s < nsh; ++s) {
+                const size_t q = (size_t)f * nsh + s;
+                if (conv_it[q] == 0 && pin.p[q] < 0.5 * tol) conv_it[q] = j;
+                all_conv = all_conv && conv_it[q] != 0;
+            }
+            // a zero norm is an invariant subspace: every shift is solved
+            const bool breakdown = pin.p[(size_t)nsh * F + f] == 0.0;
+            if (all_conv || breakdown || j >= m)
+                fin.push_back(f);
+            else
+                keep.push_back(f);
+        }
+        for (int f : fin) {
+            S.h_jl[f] = S.jcount[f];
+            upload(S, S.jl_d.p, S.h_jl.data(), (size_t)B);
+            std::vector<int> one(1, f);
+            upload_list(S, one);
+            k_ms_solve<<<dim3(nsh, 1), 64, lds_solve, st>>>(nsh, S.slots_d.p, S.jl_d.p, Hs.p, ldH, ldH_f, sig.p,
+                                                            csq.p, snq.p, gq.p, ldg, yq.p, ldg);
+            k_ms_update_x<<<dim3((nv + 255) / 256, (nsh + kMsShifts - 1) / kMsShifts), 256, 0, st>>>(
+                nv, nsh, f, pols[f], S.jl_d.p, V.p, S.ldv_slot, yq.p, ldg, d_sol, (size_t)rhs_stride);
+            PXH_HIP(hipGetLastError());
+            const double t = now_s();
+            for (int s = 0; s < nsh; ++s) {
+                const size_t q = (size_t)f * nsh + s;
+                const int k = sys_at[q];
+                if (k < 0) continue;
+                iters_out[k] = conv_it[q] ? conv_it[q] : S.jcount[f];
+                t_done[k] = t;
+            }
+        }
+        act = keep;
+    }
+    // FP64 true residual of every system, B at a time
+    for (size_t k0 = 0; k0 < systems.size(); k0 += B) {
+        const int nc = (int)std::min<size_t>(B, systems.size() - k0);
+        std::vector<int> lst(nc);
+        for (int i = 0; i < nc; ++i) {
+            const System &sy = systems[k0 + i];
+            lst[i] = i;
+            if (bem) {
+                S.h_lam[i] = sigma[sy.ifreq];
+                upload(S, S.lam.p + i, &S.h_lam[i], 1);
+            } else {
+                S.h_par[i] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
+                upload(S, S.par.p + i, &S.h_par[i], 1);
+            }
+            k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, i, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
+                                                        S.Bv.p, S.R.p, S.X.p);
+            PXH_HIP(hipMemcpyAsync(S.X.p + (size_t)nv * i, d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
+                                   sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
+        }
+        upload_list(S, lst);
+        apply_op(S, nc, true, S.X.p, S.W.p);
+        c->stats.v[2] += 1;
+        c->stats.v[0] += nc;
+        k_residual<<<grid_rows(nc), 256, 0, st>>>(nv, S.slots_d.p, S.Bv.p, S.W.p, S.R.p);
+        k_norm<<<nc, RB, 0, st>>>(nv, S.slots_d.p, S.R.p, (size_t)nv, S.nrm.p);
+        PXH_HIP(hipGetLastError());
+        PXH_HIP(hipMemcpyAsync(pin.p, S.nrm.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+        PXH_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < nc; ++i) {
+            tres_out[k0 + i] = pin.p[i];
+            if (!(pin.p[i] < tol)) unsolved.push_back((int)(k0 + i));
+        }
+    }
+    return 0;
+}
+
 
 // polar[a + 3 b] = sum_i var_b(i) x_a(i) / E0 (+ dipoles); BEM: area-weighted for DPCM.
 __global__ void k_polar_partial(int n, int nv, int fmu, int bem_dpcm, const double4 *__restrict__ pos,
@@ -1001,10 +1438,34 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         // with FP32 products), so a mixed request runs the batched FP64 solve
         if (!bem && c->atoms.model == PXH_WFQ && precision == PXH_MIXED) precision = PXH_FP64;
         const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
+        const int prec = c->use_stored ? PXH_FP64 : precision;
+        std::vector<double2> sigma;
         try {
-            status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, c->use_stored ? PXH_FP64
-                                                                                                        : precision,
-                                 plain, sol.p, it_sys, tr_sys, t_adm, t_fin);
+            if (!plain && c->opt.multishift && shift_family(c, n_freq, freq, sigma)) {
+                // one Krylov basis for all frequencies; the few systems it
+                // leaves above tol continue in the batched solver from there
+                std::vector<int> rest;
+                status = run_multishift(c, nv, systems, n_freq, sigma, freq, rhs_all.p, (int)stride, tol, kdim,
+                                        sol.p, it_sys, tr_sys, t_fin, rest);
+                t_adm.assign(systems.size(), 0.0);
+                if (!rest.empty()) {
+                    std::vector<System> sub;
+                    for (int k : rest) sub.push_back(systems[k]);
+                    std::vector<int> it2;
+                    std::vector<double> tr2, ta2, tf2;
+                    const double t_ms = *std::max_element(t_fin.begin(), t_fin.end());
+                    status = run_batched(c, nv, sub, rhs_all.p, (int)stride, freq, tol, kdim, prec, false, true,
+                                         sol.p, it2, tr2, ta2, tf2);
+                    for (size_t i = 0; i < rest.size(); ++i) {
+                        it_sys[rest[i]] += it2[i];
+                        tr_sys[rest[i]] = tr2[i];
+                        t_fin[rest[i]] = t_ms + tf2[i];
+                    }
+                }
+            } else {
+                status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, prec, plain, false,
+                                     sol.p, it_sys, tr_sys, t_adm, t_fin);
+            }
         } catch (...) {
             c->use_stored = false;
             c->atoms.Tstore.release();
