@@ -509,6 +509,12 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
     if (M.variant == PXH_DPCM) {
         // y = D x + lambda x
         gemm(M.D.p, M.Df.p, x, 0, GemmOut{d_lambda, x, ld, 0, 1.0, y, ld, 0});
+    } else if (c->bem_op == 1) {
+        // y = -(D x + lambda x)
+        gemm(M.D.p, M.Df.p, x, 0, GemmOut{d_lambda, x, ld, 0, -1.0, y, ld, 0});
+    } else if (c->bem_op == 2) {
+        // y = S x
+        gemm(M.SA.p, M.SAf.p, x, 0, GemmOut{nullptr, nullptr, 0, 0, 1.0, y, ld, 0});
     } else {
         // u = S A^-1 x, then y = -(D u + lambda u)
         gemm(M.SA.p, M.SAf.p, x, 0, GemmOut{nullptr, nullptr, 0, 0, 1.0, c->work_t.p, n, 1});

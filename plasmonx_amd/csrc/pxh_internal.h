@@ -167,6 +167,10 @@ struct pxh_ctx {
     pxh::DevBuf<float> work_scale;            // per-column scales of that operand
     pxh::DevBuf<double> work_pack;            // packed operand of the BEM matrix products
     bool use_stored = false;                  // in-memory method: products use the stored T
+    // IEF operator form used by the products: 0 the system matrix
+    // -(D + lambda I) S; 1 the right-preconditioned -(D + lambda I) (unknown
+    // u = S q); 2 the matrix S alone
+    int bem_op = 0;
 };
 
 namespace pxh {

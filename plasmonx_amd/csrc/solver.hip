@@ -1040,11 +1040,23 @@ static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, c
 // (bem_module.f90:898-906); homogeneous omegaFQ: A0 = K-bar T, sigma = -z_q
 // (wfq_module.f90:234-246).  The omegaFQFmu operator shifts its charge and
 // dipole blocks by different amounts and is no such family.
-static bool shift_family(const pxh_ctx *c, int n_freq, const double *freq, std::vector<double2> &sigma) {
+//
+// BEM IEF, right-preconditioned by S^-1: the operator -(D + lambda I) S with
+// u = S q is -(D + lambda I), A0 = -D and sigma = -lambda (operator form
+// bem_op = 1).  The charges q = S^-1 u are not formed: alpha needs only
+// z_a^T u with z_a = S^-1 c_a (S symmetric), and the residual of the IEF
+// system at q is the residual of the shifted system at u.  Used when the
+// caller asks for polarizabilities only (polar_only).
+static bool shift_family(const pxh_ctx *c, int n_freq, const double *freq, bool polar_only,
+                         std::vector<double2> &sigma) {
     sigma.assign(n_freq, make_double2(0, 0));
     if (c->target == PXH_BEM) {
-        if (c->mesh.variant != PXH_DPCM) return false;
-        for (int f = 0; f < n_freq; ++f) sigma[f] = bem_lambda(c, f);
+        const bool ief = c->mesh.variant == PXH_IEFPCM;
+        if (ief && !polar_only) return false;
+        for (int f = 0; f < n_freq; ++f) {
+            const double2 l = bem_lambda(c, f);
+            sigma[f] = ief ? make_double2(-l.x, -l.y) : l;
+        }
         return true;
     }
     const Atoms &A = c->atoms;
@@ -1276,7 +1288,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
             const System &sy = systems[k0 + i];
             lst[i] = i;
             if (bem) {
-                S.h_lam[i] = sigma[sy.ifreq];
+                S.h_lam[i] = bem_lambda(c, sy.ifreq);
                 upload(S, S.lam.p + i, &S.h_lam[i], 1);
             } else {
                 S.h_par[i] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
@@ -1382,6 +1394,51 @@ void results_row(double freq, const double *pol, double *r) {
 int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_stride,
                           double2 *d_sol, int pol_mask);
 
+namespace {
+__global__ void k_pack_z(int n, const double2 *__restrict__ z, double4 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double4(z[i].x, z[(size_t)n + i].x, z[2 * (size_t)n + i].x, 0.0);
+}
+}  // namespace
+
+// z_a = S^-1 c_a for the three Cartesian components of the tessera centroids
+// (FP64 batched GMRES on the operator form S, to 1e-13 relative), packed as
+// one double4 per tessera.  S is real symmetric positive definite, so z is
+// real.
+static void ief_polar_weights(pxh_ctx *c, DevBuf<double4> &z4) {
+    const int n = c->mesh.n;
+    hipStream_t st = c->stream;
+    std::vector<double2> hc((size_t)3 * n);
+    double cmax = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double q = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double v = c->mesh.h_coord[3 * (size_t)i + k];
+            hc[(size_t)n * k + i] = make_double2(v, 0.0);
+            q += v * v;
+        }
+        cmax = std::max(cmax, std::sqrt(q));
+    }
+    DevBuf<double2> rhs, z;
+    rhs.upload(hc.data(), hc.size(), st);
+    z.alloc(hc.size());
+    PXH_HIP(hipMemsetAsync(z.p, 0, sizeof(double2) * hc.size(), st));
+    std::vector<System> sys;
+    for (int k = 0; k < 3; ++k) sys.push_back({0, k});
+    std::vector<int> it;
+    std::vector<double> tr, ta, tf;
+    const double f0 = 0.0;
+    const int saved = c->bem_op;
+    c->bem_op = 2;
+    const int stt = run_batched(c, n, sys, rhs.p, 3 * n, &f0, 1e-13 * std::max(cmax, 1e-300), std::min(n, 300),
+                                PXH_FP64, false, false, z.p, it, tr, ta, tf);
+    c->bem_op = saved;
+    PXH_CHECK(stt == PXH_OK, PXH_ENOCONV, "IEF: S z = c did not converge");
+    z4.alloc(n);
+    k_pack_z<<<(n + 255) / 256, 256, 0, st>>>(n, z.p, z4.p);
+    PXH_HIP(hipGetLastError());
+}
+
 int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
                    double tol, int kdim, int precision, double *variables, double *polar, double *results,
                    int32_t *iters, double *true_res) {
@@ -1417,6 +1474,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     }
     std::vector<int> it_sys;
     std::vector<double> tr_sys, t_adm, t_fin;
+    bool ief_u = false;      // IEF solved for u = S q (polarizabilities only)
+    DevBuf<double4> ief_z;   // then alpha uses z_a = S^-1 c_a in place of c_a
     std::vector<System> systems;
     for (int f = 0; f < n_freq; ++f)
         for (int k = 0; k < 3; ++k)
@@ -1440,8 +1499,17 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         const bool plain = precision == PXH_FP64 && c->opt.batch <= 1;
         const int prec = c->use_stored ? PXH_FP64 : precision;
         std::vector<double2> sigma;
+        // the IEF operator form is reset whatever happens below
+        struct BemOpGuard {
+            pxh_ctx *c;
+            ~BemOpGuard() { c->bem_op = 0; }
+        } guard{c};
         try {
-            if (!plain && c->opt.multishift && shift_family(c, n_freq, freq, sigma)) {
+            if (!plain && c->opt.multishift && shift_family(c, n_freq, freq, variables == nullptr, sigma)) {
+                if (bem && c->mesh.variant == PXH_IEFPCM) {
+                    c->bem_op = 1;  // unknown u = S q
+                    ief_u = true;
+                }
                 // one Krylov basis for all frequencies; the few systems it
                 // leaves above tol continue in the batched solver from there
                 std::vector<int> rest;
@@ -1466,6 +1534,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                 status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, prec, plain, false,
                                      sol.p, it_sys, tr_sys, t_adm, t_fin);
             }
+            if (ief_u) ief_polar_weights(c, ief_z);
         } catch (...) {
             c->use_stored = false;
             c->atoms.Tstore.release();
@@ -1505,7 +1574,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             hp[i] = make_double4(c->mesh.h_coord[3 * i], c->mesh.h_coord[3 * i + 1], c->mesh.h_coord[3 * i + 2], 0.0);
         bpos.upload(hp.data(), hp.size(), st);
         barea.upload(c->mesh.h_area.data(), nv, st);
-        posp = bpos.p;
+        posp = ief_u ? ief_z.p : bpos.p;
         area = barea.p;
     } else {
         posp = c->atoms.pos.p;

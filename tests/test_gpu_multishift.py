@@ -44,11 +44,20 @@ def _dpcm(ctx):
     return m.n, freq, fx["e0"], 7
 
 
-CASES = {"sodium_450w": _sodium, "graphene_disk": _graphene, "bem_dpcm": _dpcm}
+def _ief(ctx):
+    fx = load_fixture("bem_ief_accurate_inversion")
+    m = mesh_from_fixture(fx)
+    freq = np.linspace(3.0, 4.0, 40) / 27.211386245981
+    ctx.set_mesh(m)
+    ctx.set_permittivity(pm.silver_etchegoin(freq))
+    return m.n, freq, fx["e0"], 7
+
+
+CASES = {"sodium_450w": _sodium, "graphene_disk": _graphene, "bem_dpcm": _dpcm, "bem_ief": _ief}
 
 
 @pytest.mark.parametrize("case,method", [(k, m) for k in sorted(CASES) for m in (_lib.ITERATIVE, _lib.ITERATIVE_OTF)
-                                         if not (k == "bem_dpcm" and m == _lib.ITERATIVE_OTF)])
+                                         if not (k.startswith("bem") and m == _lib.ITERATIVE_OTF)])
 def test_multishift_matches_inversion(gpu_ctx, case, method):
     n, freq, e0, mask = CASES[case](gpu_ctx)
     tol = 1e-11 * np.sqrt(n)
@@ -90,3 +99,16 @@ def test_zero_field_direction_has_zero_solution(gpu_ctx):
                                  krylov_dim=500, precision=_lib.FP64, want_variables=True)
     assert np.all(got["variables"][:, 2, :] == 0)
     assert np.all(got["iters"][:, 2] == 0)
+
+
+def test_ief_with_variables_solves_for_the_charges(gpu_ctx):
+    """IEF through the multi-shift path returns polarizabilities only (it
+    solves for u = S q); asking for the charges takes the batched solver on
+    the IEF matrix itself, with the same polarizabilities."""
+    n, freq, e0, mask = _ief(gpu_ctx)
+    tol = 1e-11 * np.sqrt(n)
+    ref = gpu_ctx.solve_spectrum(n, freq[:6], e0, method=_lib.INVERSION, want_variables=True)
+    got = gpu_ctx.solve_spectrum(n, freq[:6], e0, method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=500,
+                                 precision=_lib.MIXED, want_variables=True)
+    assert relerr(got["polar"], ref["polar"]) < 1e-7
+    assert relerr(got["variables"], ref["variables"]) < 1e-7
