@@ -150,17 +150,28 @@ class Context:
 
     # ---- compute
     def solve_spectrum(self, n_var, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7, method=ITERATIVE_OTF, tol_abs=1e-8,
-                       krylov_dim=1000, precision=FP64, want_variables=False):
+                       krylov_dim=1000, precision=FP64, want_variables=False, x0=None):
+        """x0: starting iterates [n_freq][3][n_var] for the iterative methods
+        (sets the "initial_guess" option for this call); the solutions are
+        returned as "variables"."""
         freq = np.ascontiguousarray(freq_au, dtype=np.float64)
         nf = freq.shape[0]
         var = np.zeros((nf, 3, n_var), dtype=np.complex128) if want_variables else None
+        if x0 is not None:
+            var = np.array(x0, dtype=np.complex128, order="C").reshape(nf, 3, n_var)
+            want_variables = True
+            self.set_option("initial_guess", 1)
         polar = np.zeros((nf, 3, 3), dtype=np.complex128)  # [f][b][a] = alpha_ab
         res = np.zeros((20, nf), dtype=np.float64)  # column-major n_freq x 20
         iters = np.zeros((nf, 3), dtype=np.int32)
         tres = np.zeros((nf, 3), dtype=np.float64)
-        st = self.lib.pxh_solve_spectrum(self._h, nf, _ptr(freq), float(e0), rhs_form, pol_mask, method,
-                                         float(tol_abs), int(krylov_dim), precision, _ptr(var), _ptr(polar),
-                                         _ptr(res), _ptr(iters), _ptr(tres))
+        try:
+            st = self.lib.pxh_solve_spectrum(self._h, nf, _ptr(freq), float(e0), rhs_form, pol_mask, method,
+                                             float(tol_abs), int(krylov_dim), precision, _ptr(var), _ptr(polar),
+                                             _ptr(res), _ptr(iters), _ptr(tres))
+        finally:
+            if x0 is not None:
+                self.set_option("initial_guess", 0)
         self._check(st)
         out = {"polar": np.transpose(polar, (0, 2, 1)), "results": res.T.copy(), "iters": iters,
                "true_residual": tres}

@@ -127,6 +127,8 @@ struct Options {
     int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
     double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
     int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies
+    int initial_guess = 0;   // iterative methods start from the caller's variables
+    int x0_projection = 4;   // batched solver: start from the best x in the span of this many converged neighbours
 };
 
 struct Stats {

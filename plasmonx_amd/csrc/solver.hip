@@ -562,6 +562,204 @@ __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ j
     }
 }
 
+// ------------------------------------------------- projected starting iterate
+// For a homogeneous atomistic target A(omega) = M + S(omega) with S diagonal
+// (q rows -z_q(t_i), dipole rows Lambda_i(omega), the frequency-dependent
+// shifts of the operator).  Every converged system leaves x and M x (= A x -
+// S x) behind; a newly admitted system at omega starts from the x0 in the
+// span of up to kProjMax converged neighbours of the same field that
+// minimises ||b - A(omega) x0||, with A(omega) x_g = M x_g + S(omega) x_g
+// formed without a product.
+constexpr int kProjMax = 8;
+constexpr int kProjRows = 256;
+
+__device__ __forceinline__ double2 row_shift(int r, int n, int nt, const double4 *__restrict__ pos,
+                                             const double *__restrict__ nbfrac, const SysParam &p) {
+    if (r < n) {
+        const int t = (int)pos[r].w;
+        return make_double2(-p.zq[t].x, -p.zq[t].y);
+    }
+    const int i = (r - n) / 3;
+    double2 l = make_double2(0, 0);
+    for (int t = 0; t < nt; ++t) {
+        const double f = nbfrac[t + (size_t)nt * i];
+        l.x += f * p.inv_alpha[t].x;
+        l.y += f * p.inv_alpha[t].y;
+    }
+    return l;
+}
+
+// mx[loc] = W[s] - S(omega_s) X[s] for a converged slot (W = A X, FP64).
+__global__ void k_store_mx(int nv, int n, int nt, const double4 *__restrict__ pos, const double *__restrict__ nbfrac,
+                           const SysParam *__restrict__ par, int s, const double2 *__restrict__ W,
+                           const double2 *__restrict__ X, double2 *__restrict__ mx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nv) return;
+    const double2 sh = row_shift(r, n, nt, pos, nbfrac, par[s]);
+    const double2 x = X[(size_t)nv * s + r];
+    const double2 w = W[(size_t)nv * s + r];
+    mx[r] = make_double2(w.x - (sh.x * x.x - sh.y * x.y), w.y - (sh.x * x.y + sh.y * x.x));
+}
+
+// Per admitted slot (info: k, then k neighbour offsets into sol / mx):
+// partial sums over kProjRows rows of G^H G (upper triangle, k(k+1)/2) and
+// G^H b, G_i = mx_i + S(omega_s) x_i.  part[(a * nblk + blk) * L + e].
+__global__ __launch_bounds__(kProjRows) void k_proj_gram(int nv, int n, int nt, const double4 *__restrict__ pos,
+                                                         const double *__restrict__ nbfrac,
+                                                         const SysParam *__restrict__ par,
+                                                         const int *__restrict__ adm, const long long *__restrict__ info,
+                                                         const double2 *__restrict__ sol,
+                                                         const double2 *__restrict__ mx,
+                                                         const double2 *__restrict__ Bv, double2 *__restrict__ part) {
+    constexpr int L = kProjMax * (kProjMax + 1) / 2 + kProjMax;
+    __shared__ double2 red[kProjRows / 64][L];
+    const int a = blockIdx.y;
+    const int s = adm[a];
+    const long long *in = info + (size_t)a * (kProjMax + 1);
+    const int k = (int)in[0];
+    const int r = blockIdx.x * kProjRows + threadIdx.x;
+    double2 g[kProjMax];
+    double2 b = make_double2(0, 0);
+#pragma unroll
+    for (int i = 0; i < kProjMax; ++i) g[i] = make_double2(0, 0);
+    if (r < nv) {
+        const double2 sh = row_shift(r, n, nt, pos, nbfrac, par[s]);
+        for (int i = 0; i < k; ++i) {
+            const double2 x = sol[in[1 + i] + r];
+            const double2 m = mx[in[1 + i] + r];
+            g[i] = make_double2(m.x + sh.x * x.x - sh.y * x.y, m.y + sh.x * x.y + sh.y * x.x);
+        }
+        b = Bv[(size_t)nv * s + r];
+    }
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    int e = 0;
+    for (int i = 0; i < kProjMax; ++i) {
+        for (int j = i; j < kProjMax; ++j, ++e) {
+            double2 v = cconjmul(g[i], g[j]);
+            for (int off = 32; off > 0; off >>= 1) {
+                v.x += __shfl_xor(v.x, off, 64);
+                v.y += __shfl_xor(v.y, off, 64);
+            }
+            if (lane == 0) red[w][e] = v;
+        }
+    }
+    for (int i = 0; i < kProjMax; ++i, ++e) {
+        double2 v = cconjmul(g[i], b);
+        for (int off = 32; off > 0; off >>= 1) {
+            v.x += __shfl_xor(v.x, off, 64);
+            v.y += __shfl_xor(v.y, off, 64);
+        }
+        if (lane == 0) red[w][e] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < L; q += kProjRows) {
+        double2 v = red[0][q];
+        for (int ww = 1; ww < kProjRows / 64; ++ww) {
+            v.x += red[ww][q].x;
+            v.y += red[ww][q].y;
+        }
+        part[((size_t)a * gridDim.x + blockIdx.x) * L + q] = v;
+    }
+}
+
+// Coefficients of the projected start for each admitted slot: the partials
+// summed in a fixed order, the normal equations scaled to a unit diagonal and
+// solved by Cholesky; a column whose pivot falls below 1e-12 of its diagonal
+// (dependent on the others) gets coefficient 0.  One wavefront per slot.
+__global__ __launch_bounds__(64) void k_proj_solve(int nblk, const long long *__restrict__ info,
+                                                   const double2 *__restrict__ part, double2 *__restrict__ coef) {
+    constexpr int L = kProjMax * (kProjMax + 1) / 2 + kProjMax;
+    __shared__ double2 acc[L];
+    const int a = blockIdx.x;
+    const int k = (int)info[(size_t)a * (kProjMax + 1)];
+    for (int q = threadIdx.x; q < L; q += 64) {
+        double2 v = make_double2(0, 0);
+        for (int b = 0; b < nblk; ++b) {
+            const double2 p = part[((size_t)a * nblk + b) * L + q];
+            v.x += p.x;
+            v.y += p.y;
+        }
+        acc[q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double2 G[kProjMax][kProjMax], rhs[kProjMax], c[kProjMax];
+    double d[kProjMax];
+    int e = 0;
+    for (int i = 0; i < kProjMax; ++i)
+        for (int j = i; j < kProjMax; ++j, ++e) {
+            G[i][j] = acc[e];
+            G[j][i] = make_double2(acc[e].x, -acc[e].y);
+        }
+    for (int i = 0; i < kProjMax; ++i, ++e) rhs[i] = acc[e];
+    for (int i = 0; i < k; ++i) d[i] = G[i][i].x > 0.0 ? 1.0 / sqrt(G[i][i].x) : 0.0;
+    for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < k; ++j) G[i][j] = make_double2(G[i][j].x * d[i] * d[j], G[i][j].y * d[i] * d[j]);
+        rhs[i] = make_double2(rhs[i].x * d[i], rhs[i].y * d[i]);
+    }
+    // G = L L^H in place (lower triangle), dropped columns marked by L_ii = 0
+    bool keep[kProjMax];
+    for (int j = 0; j < k; ++j) {
+        double djj = G[j][j].x;
+        for (int q = 0; q < j; ++q)
+            if (keep[q]) djj -= G[j][q].x * G[j][q].x + G[j][q].y * G[j][q].y;
+        keep[j] = d[j] > 0.0 && djj > 1e-12;
+        const double ljj = keep[j] ? sqrt(djj) : 0.0;
+        G[j][j] = make_double2(ljj, 0.0);
+        for (int i = j + 1; i < k; ++i) {
+            double2 v = G[i][j];
+            for (int q = 0; q < j; ++q) {
+                if (!keep[q]) continue;
+                const double2 l1 = G[i][q], l2 = G[j][q];  // v -= L_iq conj(L_jq)
+                v.x -= l1.x * l2.x + l1.y * l2.y;
+                v.y -= l1.y * l2.x - l1.x * l2.y;
+            }
+            G[i][j] = keep[j] ? make_double2(v.x / ljj, v.y / ljj) : make_double2(0, 0);
+        }
+    }
+    // L y = rhs, L^H c = y
+    for (int i = 0; i < k; ++i) {
+        double2 v = rhs[i];
+        for (int q = 0; q < i; ++q) {
+            if (!keep[q]) continue;
+            const double2 p = cmul(G[i][q], c[q]);
+            v.x -= p.x;
+            v.y -= p.y;
+        }
+        c[i] = keep[i] ? make_double2(v.x / G[i][i].x, v.y / G[i][i].x) : make_double2(0, 0);
+    }
+    for (int i = k - 1; i >= 0; --i) {
+        double2 v = c[i];
+        for (int q = i + 1; q < k; ++q) {
+            if (!keep[i] || !keep[q]) continue;
+            const double2 p = cconjmul(G[q][i], c[q]);
+            v.x -= p.x;
+            v.y -= p.y;
+        }
+        c[i] = keep[i] ? make_double2(v.x / G[i][i].x, v.y / G[i][i].x) : make_double2(0, 0);
+    }
+    for (int i = 0; i < kProjMax; ++i)
+        coef[(size_t)a * kProjMax + i] = i < k ? make_double2(c[i].x * d[i], c[i].y * d[i]) : make_double2(0, 0);
+}
+
+// X[s] = sum_i coef_i x_i for each admitted slot.
+__global__ void k_proj_x0(int nv, const int *__restrict__ adm, const long long *__restrict__ info,
+                          const double2 *__restrict__ coef, const double2 *__restrict__ sol,
+                          double2 *__restrict__ X) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (r >= nv) return;
+    const long long *in = info + (size_t)a * (kProjMax + 1);
+    const int k = (int)in[0];
+    double2 v = make_double2(0, 0);
+    for (int i = 0; i < k; ++i) {
+        const double2 p = cmul(coef[(size_t)a * kProjMax + i], sol[in[1 + i] + r]);
+        v.x += p.x;
+        v.y += p.y;
+    }
+    X[(size_t)nv * adm[a] + r] = v;
+}
+
 // ------------------------------------------------------------ multi-shift
 // Shifted systems (A0 + sigma_s I) x_s = b, x_s(0) = 0, share one Krylov
 // space K_m(A0, b): a single Arnoldi process per right-hand side serves every
@@ -593,40 +791,53 @@ __global__ void k_ms_init(int nsh, const int *__restrict__ slots, const double *
 // Step j for every shift (grid nsh x listed right-hand sides): the stored
 // rotations 0..j-1 applied to column j of H + sigma_s I, then the rotation
 // that zeroes its subdiagonal, applied to g as well; resid[q] = |g_{j+1}|.
-// Lane 0 runs the rotation chain; the rotated column itself is not kept.
+// Lane 0 runs the rotation chain over chunks of kMsChunk entries that all
+// lanes stage in LDS; the rotated column itself is not kept.
+constexpr int kMsChunk = 512;
 __global__ __launch_bounds__(64) void k_ms_givens(int nsh, const int *__restrict__ slots,
                                                   const int *__restrict__ jcol, const double2 *__restrict__ Hs,
                                                   int ldH, size_t ldH_f, const double2 *__restrict__ sigma,
                                                   double *__restrict__ cs, double2 *__restrict__ sn,
                                                   double2 *__restrict__ g, int ldg, double *__restrict__ resid) {
-    extern __shared__ double2 sh_ms[];  // col[ldH] | sn[ldg] | cs[ldg] (doubles)
-    double2 *col = sh_ms;
-    double2 *snl = sh_ms + ldH;
-    double *csl = reinterpret_cast<double *>(sh_ms + ldH + ldg);
+    __shared__ double2 col[kMsChunk + 1], snl[kMsChunk];
+    __shared__ double csl[kMsChunk];
     const int s = blockIdx.x;
     const int f = slots[blockIdx.y];
     const int j = jcol[f];
     const size_t q = (size_t)f * nsh + s;
     const double2 *Hc = Hs + ldH_f * f + (size_t)ldH * j;
-    for (int i = threadIdx.x; i <= j + 1; i += 64) col[i] = Hc[i];
-    for (int i = threadIdx.x; i < j; i += 64) {
-        csl[i] = cs[q * ldg + i];
-        snl[i] = sn[q * ldg + i];
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     const double2 sg = sigma[s];
-    col[j].x += sg.x;
-    col[j].y += sg.y;
-    double2 carry = col[0];
-    for (int i = 0; i < j; ++i) {
-        double2 nx = col[i + 1];
-        zrot_d(carry, nx, csl[i], snl[i]);
-        carry = nx;
+    double2 carry = make_double2(0, 0);
+    // chunk [b0, b0 + kMsChunk) of the rotations; col holds entries b0 .. b0 + kMsChunk
+    for (int b0 = 0; b0 <= j; b0 += kMsChunk) {
+        const int nr = min(kMsChunk, j - b0);  // rotations in this chunk
+        __syncthreads();
+        for (int i = threadIdx.x; i <= nr; i += 64) {
+            double2 h = Hc[b0 + i];
+            if (b0 + i == j) {
+                h.x += sg.x;
+                h.y += sg.y;
+            }
+            col[i] = h;
+        }
+        for (int i = threadIdx.x; i < nr; i += 64) {
+            csl[i] = cs[q * ldg + b0 + i];
+            snl[i] = sn[q * ldg + b0 + i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (b0 == 0) carry = col[0];
+            for (int i = 0; i < nr; ++i) {
+                double2 nx = col[i + 1];
+                zrot_d(carry, nx, csl[i], snl[i]);
+                carry = nx;
+            }
+        }
     }
+    if (threadIdx.x != 0) return;
     double c;
     double2 sv;
-    zrotg_d(carry, col[j + 1], c, sv);
+    zrotg_d(carry, Hc[j + 1], c, sv);
     cs[q * ldg + j] = c;
     sn[q * ldg + j] = sv;
     double2 *gq = g + q * ldg;
@@ -634,61 +845,72 @@ __global__ __launch_bounds__(64) void k_ms_givens(int nsh, const int *__restrict
     resid[q] = hypot(gq[j + 1].x, gq[j + 1].y);
 }
 
-// y[q] = R^-1 g[q](0:J) for every shift, R the rotated (H + sigma_s I)(0:J,
-// 0:J).  Column-oriented back substitution from the last column; each column
-// of R is rebuilt from H and the stored rotations when it is needed, so no
-// shift stores its triangle.
+// g[q](0:J) <- R^-1 g[q](0:J) for every shift, R the rotated (H + sigma_s
+// I)(0:J, 0:J).  Column-oriented back substitution from the last column; each
+// column of R is rebuilt from H and the stored rotations when it is needed
+// (rotation chain in lane 0, staged by chunks, into the per-shift scratch
+// colq), so no shift stores its triangle.
 __global__ __launch_bounds__(64) void k_ms_solve(int nsh, const int *__restrict__ slots,
                                                  const int *__restrict__ jlen, const double2 *__restrict__ Hs,
                                                  int ldH, size_t ldH_f, const double2 *__restrict__ sigma,
                                                  const double *__restrict__ cs, const double2 *__restrict__ sn,
-                                                 const double2 *__restrict__ g, int ldg, double2 *__restrict__ y,
-                                                 int ldy) {
-    extern __shared__ double2 sh_ms[];  // col[ldH] | gl[ldg] | sn[ldg] | cs[ldg] (doubles)
-    double2 *col = sh_ms;
-    double2 *gl = col + ldH;
-    double2 *snl = gl + ldg;
-    double *csl = reinterpret_cast<double *>(snl + ldg);
+                                                 double2 *__restrict__ g, int ldg, double2 *__restrict__ colq) {
+    __shared__ double2 col[kMsChunk + 1], snl[kMsChunk];
+    __shared__ double csl[kMsChunk];
+    __shared__ double2 yj_s;
     const int s = blockIdx.x;
     const int f = slots[blockIdx.y];
     const int J = jlen[f];
     const size_t q = (size_t)f * nsh + s;
-    for (int i = threadIdx.x; i < J; i += 64) {
-        gl[i] = g[q * ldg + i];
-        csl[i] = cs[q * ldg + i];
-        snl[i] = sn[q * ldg + i];
-    }
+    double2 *gq = g + q * ldg;
+    double2 *cq = colq + q * ldg;
     const double2 sg = sigma[s];
     for (int jj = J - 1; jj >= 0; --jj) {
-        __syncthreads();
         const double2 *Hc = Hs + ldH_f * f + (size_t)ldH * jj;
-        for (int i = threadIdx.x; i <= jj + 1; i += 64) col[i] = Hc[i];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            col[jj].x += sg.x;
-            col[jj].y += sg.y;
-            double2 carry = col[0];
-            for (int i = 0; i <= jj; ++i) {
-                double2 nx = col[i + 1];
-                zrot_d(carry, nx, csl[i], snl[i]);
-                col[i] = carry;
-                carry = nx;
+        double2 carry = make_double2(0, 0);
+        // rotations 0..jj on column jj (entries 0..jj+1)
+        for (int b0 = 0; b0 <= jj; b0 += kMsChunk) {
+            const int nr = min(kMsChunk, jj + 1 - b0);
+            __syncthreads();
+            for (int i = threadIdx.x; i <= nr; i += 64) {
+                double2 h = Hc[b0 + i];
+                if (b0 + i == jj) {
+                    h.x += sg.x;
+                    h.y += sg.y;
+                }
+                col[i] = h;
             }
-            const double2 r = col[jj];
+            for (int i = threadIdx.x; i < nr; i += 64) {
+                csl[i] = cs[q * ldg + b0 + i];
+                snl[i] = sn[q * ldg + b0 + i];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (b0 == 0) carry = col[0];
+                for (int i = 0; i < nr; ++i) {
+                    double2 nx = col[i + 1];
+                    zrot_d(carry, nx, csl[i], snl[i]);
+                    cq[b0 + i] = carry;
+                    carry = nx;
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            const double2 r = cq[jj];
             const double den = r.x * r.x + r.y * r.y;
-            const double2 v = gl[jj];
-            gl[jj] = make_double2((v.x * r.x + v.y * r.y) / den, (v.y * r.x - v.x * r.y) / den);
+            const double2 v = gq[jj];
+            const double2 yv = make_double2((v.x * r.x + v.y * r.y) / den, (v.y * r.x - v.x * r.y) / den);
+            gq[jj] = yv;
+            yj_s = yv;
         }
         __syncthreads();
-        const double2 yj = gl[jj];
+        const double2 yj = yj_s;
         for (int i = threadIdx.x; i < jj; i += 64) {
-            const double2 p = cmul(col[i], yj);
-            gl[i].x -= p.x;
-            gl[i].y -= p.y;
+            const double2 p = cmul(cq[i], yj);
+            gq[i].x -= p.x;
+            gq[i].y -= p.y;
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < J; i += 64) y[q * ldy + i] = gl[i];
 }
 
 // x of every shift of one right-hand side f: V_f(:, 0:J) y[q], written to
@@ -814,8 +1036,43 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     }
     S.jcount.assign(B, 0);
     S.slots_d.alloc(std::max<size_t>(1, (size_t)B));
-    S.ring.init(std::max(sizeof(SysParam), sizeof(int) * (size_t)B), 4 * B + 64);
+    S.ring.init(std::max({sizeof(SysParam), sizeof(int) * (size_t)B, sizeof(long long) * (kProjMax + 1) * (size_t)B}),
+                4 * B + 64);
     PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * (size_t)B, hipHostMallocDefault));
+
+    // projected starting iterates (homogeneous atoms, cold start): the
+    // systems are admitted coarse to fine over the frequency grid, so that
+    // most start between converged neighbours
+    const bool proj = !plain && !warm && c->opt.x0_projection > 0 && c->target != PXH_BEM && !c->atoms.hetero;
+    const int kproj = std::min(kProjMax, std::max(1, c->opt.x0_projection));
+    int n_freq_all = 0;
+    for (const System &sy : systems) n_freq_all = std::max(n_freq_all, sy.ifreq + 1);
+    std::vector<int> perm(systems.size());
+    for (size_t k = 0; k < perm.size(); ++k) perm[k] = (int)k;
+    DevBuf<double2> mx, proj_part, proj_coef;
+    DevBuf<long long> proj_info;
+    DevBuf<int> proj_adm;
+    std::vector<char> have;  // [ifreq * 3 + pol]: x and M x stored
+    const int proj_nblk = (nv + kProjRows - 1) / kProjRows;
+    constexpr int kProjL = kProjMax * (kProjMax + 1) / 2 + kProjMax;
+    if (proj) {
+        auto level = [](int f) {
+            int l = 0;
+            while (l < 30 && (f >> l) % 2 == 0) ++l;
+            return f == 0 ? 31 : l;
+        };
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+            return level(systems[a].ifreq) > level(systems[b].ifreq);
+        });
+        mx.alloc((size_t)rhs_stride * n_freq_all);
+        proj_part.alloc((size_t)B * proj_nblk * kProjL);
+        proj_coef.alloc((size_t)B * kProjMax);
+        proj_info.alloc((size_t)B * (kProjMax + 1));
+        proj_adm.alloc(B);
+        have.assign((size_t)3 * n_freq_all, 0);
+    }
+    std::vector<int> adm_proj;                 // slots admitted this step with a projected start
+    std::vector<long long> h_info;
 
     enum Phase { FREE, INNER, NEED_RES };
     std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0);
@@ -832,9 +1089,10 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     int status = 0;
 
     auto admit = [&](int s) {
-        const System &sy = systems[next];
-        t_admit[next] = now_s();
-        sys_of[s] = (int)next++;
+        const int id = perm[next++];
+        const System &sy = systems[id];
+        t_admit[id] = now_s();
+        sys_of[s] = id;
         if (c->target == PXH_BEM) {
             S.h_lam[s] = bem_lambda(c, sy.ifreq);
             upload(S, S.lam.p + s, &S.h_lam[s], 1);
@@ -856,17 +1114,50 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         outer[s] = 0;
         total_it[s] = 0;
         phase[s] = NEED_RES;  // with x = 0 the residual is b: computed below without a product
+        if (proj) {
+            // the kproj nearest converged frequencies of this field
+            std::vector<std::pair<int, int>> cand;
+            for (int g = 0; g < n_freq_all; ++g)
+                if (have[(size_t)g * 3 + sy.pol]) cand.push_back({std::abs(g - sy.ifreq), g});
+            if (!cand.empty()) {
+                std::sort(cand.begin(), cand.end());
+                const int k = std::min<int>(kproj, (int)cand.size());
+                h_info.push_back(k);
+                for (int i = 0; i < kProjMax; ++i)
+                    h_info.push_back(i < k ? (long long)rhs_stride * cand[i].second + (long long)nv * sy.pol : 0);
+                adm_proj.push_back(s);
+            }
+        }
     };
 
     std::vector<int> res_first;  // slots whose residual is b (x = 0)
     while (done < systems.size()) {
         // fill free slots
         res_first.clear();
+        adm_proj.clear();
+        h_info.clear();
         for (int s = 0; s < B && next < systems.size(); ++s)
             if (phase[s] == FREE) {
                 admit(s);
-                if (!warm) res_first.push_back(s);
+                if (!warm && (adm_proj.empty() || adm_proj.back() != s)) res_first.push_back(s);
             }
+        if (!adm_proj.empty()) {
+            const int na = (int)adm_proj.size();
+            {
+                void *pi = S.ring.take();
+                PXH_CHECK(sizeof(long long) * h_info.size() <= S.ring.chunk, PXH_EINVAL, "projection list too long");
+                std::memcpy(pi, h_info.data(), sizeof(long long) * h_info.size());
+                PXH_HIP(hipMemcpyAsync(proj_info.p, pi, sizeof(long long) * h_info.size(), hipMemcpyHostToDevice, st));
+            }
+            upload(S, proj_adm.p, adm_proj.data(), adm_proj.size());
+            const Atoms &A = c->atoms;
+            k_proj_gram<<<dim3(proj_nblk, na), kProjRows, 0, st>>>(nv, A.n, A.nt, A.pos.p, A.d_nbfrac.p, S.par.p,
+                                                                   proj_adm.p, proj_info.p, d_sol, mx.p, S.Bv.p,
+                                                                   proj_part.p);
+            k_proj_solve<<<na, 64, 0, st>>>(proj_nblk, proj_info.p, proj_part.p, proj_coef.p);
+            k_proj_x0<<<grid_rows(na), 256, 0, st>>>(nv, proj_adm.p, proj_info.p, proj_coef.p, d_sol, S.X.p);
+            PXH_HIP(hipGetLastError());
+        }
         // ----- residual checks (FP64)
         std::vector<int> need;
         for (int s = 0; s < B; ++s)
@@ -901,6 +1192,14 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                     PXH_HIP(hipMemcpyAsync(d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
                                            S.X.p + (size_t)nv * s, sizeof(double2) * nv, hipMemcpyDeviceToDevice,
                                            st));
+                    if (proj && !first) {
+                        // W holds A(omega) x from this step's FP64 product
+                        const Atoms &A = c->atoms;
+                        k_store_mx<<<(nv + 255) / 256, 256, 0, st>>>(
+                            nv, A.n, A.nt, A.pos.p, A.d_nbfrac.p, S.par.p, s, S.W.p, S.X.p,
+                            mx.p + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol);
+                        have[(size_t)sy.ifreq * 3 + sy.pol] = 1;
+                    }
                     phase[s] = FREE;
                     ++done;
                     continue;
@@ -1146,14 +1445,14 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         S.par.alloc(B);
         S.h_par.assign(B, SysParam{});
     }
-    DevBuf<double2> V, Hs, gq, snq, yq, sig;
+    DevBuf<double2> V, Hs, gq, snq, colq, sig;
     DevBuf<double> csq, msres;
     V.alloc(S.ldv_slot * F);
     Hs.alloc(ldH_f * F);
     gq.alloc((size_t)nsh * F * ldg);
     snq.alloc((size_t)nsh * F * ldg);
     csq.alloc((size_t)nsh * F * ldg);
-    yq.alloc((size_t)nsh * F * ldg);
+    colq.alloc((size_t)nsh * F * ldg);
     msres.alloc((size_t)nsh * F);
     sig.upload(sigma.data(), nsh, st);
     PinnedDoubles pin((size_t)nsh * F + B);
@@ -1201,8 +1500,6 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         k_ms_init<<<dim3(nsh, (int)act.size()), 64, 0, st>>>(nsh, S.slots_d.p, S.nrm.p, gq.p, ldg);
         for (size_t k = 0; k < act.size(); ++k) c->stats.v[7] += 1;
     }
-    const size_t lds_givens = sizeof(double2) * (ldH + ldg) + sizeof(double) * ldg;
-    const size_t lds_solve = sizeof(double2) * (ldH + 2 * ldg) + sizeof(double) * ldg;
     while (!act.empty()) {
         const int na = (int)act.size();
         upload_list(S, act);
@@ -1236,7 +1533,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         k_scale_into<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jp1_d.p, S.W.p, (size_t)nv, S.nrm.p,
                                                             V.p, S.ldv_slot);
         k_ms_store_col<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hbuf.p, ldh, S.nrm.p, Hs.p, ldH, ldH_f);
-        k_ms_givens<<<dim3(nsh, na), 64, lds_givens, st>>>(nsh, S.slots_d.p, S.jcount_d.p, Hs.p, ldH, ldH_f, sig.p,
+        k_ms_givens<<<dim3(nsh, na), 64, 0, st>>>(nsh, S.slots_d.p, S.jcount_d.p, Hs.p, ldH, ldH_f, sig.p,
                                                            csq.p, snq.p, gq.p, ldg, msres.p);
         PXH_HIP(hipGetLastError());
         PXH_HIP(hipMemcpyAsync(pin.p, msres.p, sizeof(double) * nsh * F, hipMemcpyDeviceToHost, st));
@@ -1264,10 +1561,10 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
             upload(S, S.jl_d.p, S.h_jl.data(), (size_t)B);
             std::vector<int> one(1, f);
             upload_list(S, one);
-            k_ms_solve<<<dim3(nsh, 1), 64, lds_solve, st>>>(nsh, S.slots_d.p, S.jl_d.p, Hs.p, ldH, ldH_f, sig.p,
-                                                            csq.p, snq.p, gq.p, ldg, yq.p, ldg);
+            k_ms_solve<<<dim3(nsh, 1), 64, 0, st>>>(nsh, S.slots_d.p, S.jl_d.p, Hs.p, ldH, ldH_f, sig.p, csq.p,
+                                                    snq.p, gq.p, ldg, colq.p);
             k_ms_update_x<<<dim3((nv + 255) / 256, (nsh + kMsShifts - 1) / kMsShifts), 256, 0, st>>>(
-                nv, nsh, f, pols[f], S.jl_d.p, V.p, S.ldv_slot, yq.p, ldg, d_sol, (size_t)rhs_stride);
+                nv, nsh, f, pols[f], S.jl_d.p, V.p, S.ldv_slot, gq.p, ldg, d_sol, (size_t)rhs_stride);
             PXH_HIP(hipGetLastError());
             const double t = now_s();
             for (int s = 0; s < nsh; ++s) {
@@ -1455,7 +1752,13 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     DevBuf<double2> rhs_all, sol;
     rhs_all.alloc(stride * n_freq);
     sol.alloc(stride * n_freq);
-    PXH_HIP(hipMemsetAsync(sol.p, 0, sizeof(double2) * stride * n_freq, st));
+    // starting iterates: the caller's variables with the "initial_guess"
+    // option on the iterative methods (else x0 = 0, as the reference)
+    const bool x0_given = c->opt.initial_guess && variables && method != PXH_INVERSION;
+    if (x0_given)
+        PXH_HIP(hipMemcpyAsync(sol.p, variables, sizeof(double2) * stride * n_freq, hipMemcpyHostToDevice, st));
+    else
+        PXH_HIP(hipMemsetAsync(sol.p, 0, sizeof(double2) * stride * n_freq, st));
     // right-hand sides per frequency (frequency dependent only for alloys)
     if (bem) {
         dynamic_rhs_bem(c, e0, rhs_all.p);
@@ -1505,7 +1808,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             ~BemOpGuard() { c->bem_op = 0; }
         } guard{c};
         try {
-            if (!plain && c->opt.multishift && shift_family(c, n_freq, freq, variables == nullptr, sigma)) {
+            if (!plain && !x0_given && c->opt.multishift &&
+                shift_family(c, n_freq, freq, variables == nullptr, sigma)) {
                 if (bem && c->mesh.variant == PXH_IEFPCM) {
                     c->bem_op = 1;  // unknown u = S q
                     ief_u = true;
@@ -1531,7 +1835,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                     }
                 }
             } else {
-                status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, prec, plain, false,
+                status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, prec, plain, x0_given,
                                      sol.p, it_sys, tr_sys, t_adm, t_fin);
             }
             if (ief_u) ief_polar_weights(c, ief_z);
