@@ -570,8 +570,10 @@ __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ j
 // span of up to kProjMax converged neighbours of the same field that
 // minimises ||b - A(omega) x0||, with A(omega) x_g = M x_g + S(omega) x_g
 // formed without a product.
-constexpr int kProjMax = 8;
-constexpr int kProjRows = 256;
+constexpr int kProjMax = 16;
+constexpr int kProjL = kProjMax * (kProjMax + 1) / 2 + kProjMax;  // upper triangle of G^H G, then G^H b
+constexpr int kProjChunk = 64;                                     // rows staged in LDS at a time
+constexpr int kProjRows = 2048;                                    // rows per workgroup
 
 __device__ __forceinline__ double2 row_shift(int r, int n, int nt, const double4 *__restrict__ pos,
                                              const double *__restrict__ nbfrac, const SysParam &p) {
@@ -602,80 +604,98 @@ __global__ void k_store_mx(int nv, int n, int nt, const double4 *__restrict__ po
 }
 
 // Per admitted slot (info: k, then k neighbour offsets into sol / mx):
-// partial sums over kProjRows rows of G^H G (upper triangle, k(k+1)/2) and
-// G^H b, G_i = mx_i + S(omega_s) x_i.  part[(a * nblk + blk) * L + e].
-__global__ __launch_bounds__(kProjRows) void k_proj_gram(int nv, int n, int nt, const double4 *__restrict__ pos,
-                                                         const double *__restrict__ nbfrac,
-                                                         const SysParam *__restrict__ par,
-                                                         const int *__restrict__ adm, const long long *__restrict__ info,
-                                                         const double2 *__restrict__ sol,
-                                                         const double2 *__restrict__ mx,
-                                                         const double2 *__restrict__ Bv, double2 *__restrict__ part) {
-    constexpr int L = kProjMax * (kProjMax + 1) / 2 + kProjMax;
-    __shared__ double2 red[kProjRows / 64][L];
+// partial sums over kProjRows rows of G^H G (upper triangle) and G^H b,
+// G_i = mx_i + S(omega_s) x_i.  Rows are staged kProjChunk at a time in LDS
+// as columns G_0..G_{k-1}, b; work-item e < k(k+1)/2 + k owns one entry.
+// part[(a * nblk + blk) * kProjL + e].
+__global__ __launch_bounds__(256) void k_proj_gram(int nv, int n, int nt, const double4 *__restrict__ pos,
+                                                   const double *__restrict__ nbfrac,
+                                                   const SysParam *__restrict__ par, const int *__restrict__ adm,
+                                                   const long long *__restrict__ info,
+                                                   const double2 *__restrict__ sol, const double2 *__restrict__ mx,
+                                                   const double2 *__restrict__ Bv, double2 *__restrict__ part) {
+    __shared__ double2 tile[kProjMax + 1][kProjChunk];
+    __shared__ double2 shift[kProjChunk];
+    __shared__ long long off[kProjMax];
     const int a = blockIdx.y;
     const int s = adm[a];
     const long long *in = info + (size_t)a * (kProjMax + 1);
     const int k = (int)in[0];
-    const int r = blockIdx.x * kProjRows + threadIdx.x;
-    double2 g[kProjMax];
-    double2 b = make_double2(0, 0);
-#pragma unroll
-    for (int i = 0; i < kProjMax; ++i) g[i] = make_double2(0, 0);
-    if (r < nv) {
-        const double2 sh = row_shift(r, n, nt, pos, nbfrac, par[s]);
-        for (int i = 0; i < k; ++i) {
-            const double2 x = sol[in[1 + i] + r];
-            const double2 m = mx[in[1 + i] + r];
-            g[i] = make_double2(m.x + sh.x * x.x - sh.y * x.y, m.y + sh.x * x.y + sh.y * x.x);
+    if (threadIdx.x < kProjMax) off[threadIdx.x] = in[1 + threadIdx.x];
+    // entry of this work-item: (i, j) with i <= j < k, or G_i^H b
+    const int t = threadIdx.x;
+    const int ntri = k * (k + 1) / 2;
+    int ei = -1, ej = -1;
+    if (t < ntri) {
+        int i = 0, rem = t;
+        while (rem >= k - i) {
+            rem -= k - i;
+            ++i;
         }
-        b = Bv[(size_t)nv * s + r];
+        ei = i;
+        ej = i + rem;
+    } else if (t < ntri + k) {
+        ei = t - ntri;
+        ej = k;  // the right-hand side column
     }
-    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-    int e = 0;
-    for (int i = 0; i < kProjMax; ++i) {
-        for (int j = i; j < kProjMax; ++j, ++e) {
-            double2 v = cconjmul(g[i], g[j]);
-            for (int off = 32; off > 0; off >>= 1) {
-                v.x += __shfl_xor(v.x, off, 64);
-                v.y += __shfl_xor(v.y, off, 64);
+    double2 acc = make_double2(0, 0);
+    const int r0 = blockIdx.x * kProjRows;
+    const SysParam &p = par[s];
+    for (int c0 = r0; c0 < min(nv, r0 + kProjRows); c0 += kProjChunk) {
+        __syncthreads();
+        if (t < kProjChunk) {
+            const int r = c0 + t;
+            shift[t] = r < nv ? row_shift(r, n, nt, pos, nbfrac, p) : make_double2(0, 0);
+        }
+        __syncthreads();
+        for (int e = t; e < (k + 1) * kProjChunk; e += 256) {
+            const int col = e / kProjChunk, rr = e % kProjChunk;
+            const int r = c0 + rr;
+            double2 v = make_double2(0, 0);
+            if (r < nv) {
+                if (col < k) {
+                    const double2 x = sol[off[col] + r];
+                    const double2 m = mx[off[col] + r];
+                    const double2 sh = shift[rr];
+                    v = make_double2(m.x + sh.x * x.x - sh.y * x.y, m.y + sh.x * x.y + sh.y * x.x);
+                } else {
+                    v = Bv[(size_t)nv * s + r];
+                }
             }
-            if (lane == 0) red[w][e] = v;
+            tile[col][rr] = v;
+        }
+        __syncthreads();
+        if (ei >= 0) {
+            for (int rr = 0; rr < kProjChunk; ++rr) {
+                const double2 u = cconjmul(tile[ei][rr], tile[ej][rr]);
+                acc.x += u.x;
+                acc.y += u.y;
+            }
         }
     }
-    for (int i = 0; i < kProjMax; ++i, ++e) {
-        double2 v = cconjmul(g[i], b);
-        for (int off = 32; off > 0; off >>= 1) {
-            v.x += __shfl_xor(v.x, off, 64);
-            v.y += __shfl_xor(v.y, off, 64);
-        }
-        if (lane == 0) red[w][e] = v;
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < L; q += kProjRows) {
-        double2 v = red[0][q];
-        for (int ww = 1; ww < kProjRows / 64; ++ww) {
-            v.x += red[ww][q].x;
-            v.y += red[ww][q].y;
-        }
-        part[((size_t)a * gridDim.x + blockIdx.x) * L + q] = v;
-    }
+    // entries are numbered for the slot's k; k_proj_solve reads them so
+    if (t < kProjL) part[((size_t)a * gridDim.x + blockIdx.x) * kProjL + t] = acc;
 }
 
 // Coefficients of the projected start for each admitted slot: the partials
 // summed in a fixed order, the normal equations scaled to a unit diagonal and
 // solved by Cholesky; a column whose pivot falls below 1e-12 of its diagonal
-// (dependent on the others) gets coefficient 0.  One wavefront per slot.
+// (dependent on the others) gets coefficient 0.  One wavefront per slot; the
+// small matrix lives in LDS.
 __global__ __launch_bounds__(64) void k_proj_solve(int nblk, const long long *__restrict__ info,
                                                    const double2 *__restrict__ part, double2 *__restrict__ coef) {
-    constexpr int L = kProjMax * (kProjMax + 1) / 2 + kProjMax;
-    __shared__ double2 acc[L];
+    __shared__ double2 acc[kProjL];
+    __shared__ double2 G[kProjMax][kProjMax];
+    __shared__ double2 c[kProjMax];
+    __shared__ double d[kProjMax];
+    __shared__ int keep[kProjMax];
     const int a = blockIdx.x;
     const int k = (int)info[(size_t)a * (kProjMax + 1)];
-    for (int q = threadIdx.x; q < L; q += 64) {
+    const int ntri = k * (k + 1) / 2;
+    for (int q = threadIdx.x; q < ntri + k; q += 64) {
         double2 v = make_double2(0, 0);
         for (int b = 0; b < nblk; ++b) {
-            const double2 p = part[((size_t)a * nblk + b) * L + q];
+            const double2 p = part[((size_t)a * nblk + b) * kProjL + q];
             v.x += p.x;
             v.y += p.y;
         }
@@ -683,22 +703,19 @@ __global__ __launch_bounds__(64) void k_proj_solve(int nblk, const long long *__
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double2 G[kProjMax][kProjMax], rhs[kProjMax], c[kProjMax];
-    double d[kProjMax];
     int e = 0;
-    for (int i = 0; i < kProjMax; ++i)
-        for (int j = i; j < kProjMax; ++j, ++e) {
+    for (int i = 0; i < k; ++i)
+        for (int j = i; j < k; ++j, ++e) {
             G[i][j] = acc[e];
             G[j][i] = make_double2(acc[e].x, -acc[e].y);
         }
-    for (int i = 0; i < kProjMax; ++i, ++e) rhs[i] = acc[e];
+    for (int i = 0; i < k; ++i, ++e) c[i] = acc[e];  // G^H b, then y, then the coefficients
     for (int i = 0; i < k; ++i) d[i] = G[i][i].x > 0.0 ? 1.0 / sqrt(G[i][i].x) : 0.0;
     for (int i = 0; i < k; ++i) {
         for (int j = 0; j < k; ++j) G[i][j] = make_double2(G[i][j].x * d[i] * d[j], G[i][j].y * d[i] * d[j]);
-        rhs[i] = make_double2(rhs[i].x * d[i], rhs[i].y * d[i]);
+        c[i] = make_double2(c[i].x * d[i], c[i].y * d[i]);
     }
-    // G = L L^H in place (lower triangle), dropped columns marked by L_ii = 0
-    bool keep[kProjMax];
+    // G = L L^H in place (lower triangle); a dropped column has L_jj = 0
     for (int j = 0; j < k; ++j) {
         double djj = G[j][j].x;
         for (int q = 0; q < j; ++q)
@@ -717,9 +734,9 @@ __global__ __launch_bounds__(64) void k_proj_solve(int nblk, const long long *__
             G[i][j] = keep[j] ? make_double2(v.x / ljj, v.y / ljj) : make_double2(0, 0);
         }
     }
-    // L y = rhs, L^H c = y
+    // L y = G^H b, L^H c = y
     for (int i = 0; i < k; ++i) {
-        double2 v = rhs[i];
+        double2 v = c[i];
         for (int q = 0; q < i; ++q) {
             if (!keep[q]) continue;
             const double2 p = cmul(G[i][q], c[q]);
@@ -1054,7 +1071,6 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     DevBuf<int> proj_adm;
     std::vector<char> have;  // [ifreq * 3 + pol]: x and M x stored
     const int proj_nblk = (nv + kProjRows - 1) / kProjRows;
-    constexpr int kProjL = kProjMax * (kProjMax + 1) / 2 + kProjMax;
     if (proj) {
         auto level = [](int f) {
             int l = 0;
@@ -1151,7 +1167,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             }
             upload(S, proj_adm.p, adm_proj.data(), adm_proj.size());
             const Atoms &A = c->atoms;
-            k_proj_gram<<<dim3(proj_nblk, na), kProjRows, 0, st>>>(nv, A.n, A.nt, A.pos.p, A.d_nbfrac.p, S.par.p,
+            k_proj_gram<<<dim3(proj_nblk, na), 256, 0, st>>>(nv, A.n, A.nt, A.pos.p, A.d_nbfrac.p, S.par.p,
                                                                    proj_adm.p, proj_info.p, d_sol, mx.p, S.Bv.p,
                                                                    proj_part.p);
             k_proj_solve<<<na, 64, 0, st>>>(proj_nblk, proj_info.p, proj_part.p, proj_coef.p);
