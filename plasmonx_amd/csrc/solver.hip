@@ -565,7 +565,8 @@ __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ j
 // ------------------------------------------------- projected starting iterate
 // For a homogeneous atomistic target A(omega) = M + S(omega) with S diagonal
 // (q rows -z_q(t_i), dipole rows Lambda_i(omega), the frequency-dependent
-// shifts of the operator).  Every converged system leaves x and M x (= A x -
+// shifts of the operator); for alloys M keeps a mild frequency dependence
+// (the w_j/w_i ratios) and is taken at the neighbour's frequency.  Every converged system leaves x and M x (= A x -
 // S x) behind; a newly admitted system at omega starts from the x0 in the
 // span of up to kProjMax converged neighbours of the same field that
 // minimises ||b - A(omega) x0||, with A(omega) x_g = M x_g + S(omega) x_g
@@ -1057,10 +1058,13 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                 4 * B + 64);
     PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * (size_t)B, hipHostMallocDefault));
 
-    // projected starting iterates (homogeneous atoms, cold start): the
+    // projected starting iterates (atomistic targets, cold start): the
     // systems are admitted coarse to fine over the frequency grid, so that
-    // most start between converged neighbours
-    const bool proj = !plain && !warm && c->opt.x0_projection > 0 && c->target != PXH_BEM && !c->atoms.hetero;
+    // most start between converged neighbours.  For alloys the q rows also
+    // carry the frequency-dependent ratios w_j/w_i; M x is then taken at the
+    // neighbour's frequency, so the projection minimises a nearby residual
+    // (the start is still checked by its FP64 true residual)
+    const bool proj = !plain && !warm && c->opt.x0_projection > 0 && c->target != PXH_BEM;
     const int kproj = std::min(kProjMax, std::max(1, c->opt.x0_projection));
     int n_freq_all = 0;
     for (const System &sy : systems) n_freq_all = std::max(n_freq_all, sy.ifreq + 1);
