@@ -292,21 +292,28 @@ struct GemmOut {
 // (b % 8) * per + b / 8, so each XCD (b % 8) works on a contiguous range of
 // row tiles and reads their matrix rows through its own L2 for every column
 // tile.
+//
+// With ksplit > 1 the columns of Mat are also split over ksplit workgroups
+// per tile (products of few systems, which would leave the chip idle); each
+// writes its raw sums to part[(ks coltiles + ct) rowpad + row][32] and
+// k_gemm_reduce adds them in a fixed order and applies the epilogue.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bem_gemm(int n, int cpad, int steps_per_wave, int rowtiles, int coltiles,
-                                                  const T *__restrict__ Mat, const double *__restrict__ Xp,
-                                                  const int *__restrict__ slots, int nsys, GemmOut o) {
+                                                  int ksplit, const T *__restrict__ Mat,
+                                                  const double *__restrict__ Xp, const int *__restrict__ slots,
+                                                  int nsys, GemmOut o, double *__restrict__ part) {
     __shared__ double red[4][16 * 64];
-    const int total = rowtiles * coltiles;
+    const int total = rowtiles * coltiles * ksplit;
     const int per = (total + 7) / 8;
     const int L = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (L >= total) return;
-    const int rt = L / coltiles, ct = L % coltiles;
+    const int ks = L % ksplit;
+    const int rt = L / ksplit / coltiles, ct = (L / ksplit) % coltiles;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = lane & 15, kk = lane >> 4;
     const int r0 = rt * kGemmRows;
     const int ra = min(r0 + 2 * i, n - 1), rb = min(r0 + 2 * i + 1, n - 1);
-    const int s0 = w * steps_per_wave;
+    const int s0 = (ks * 4 + w) * steps_per_wave;
     const double *xb = Xp + (size_t)ct * cpad * 32 + 2 * i;
     d4v acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
     // four K steps (16 matrix columns) per iteration, the next four loaded
@@ -359,6 +366,10 @@ __global__ __launch_bounds__(256) void k_bem_gemm(int n, int cpad, int steps_per
         const int l = e & 63, v = (e >> 6) & 3, fc = (e >> 8) & 1, fr = e >> 9;
         const int r = r0 + 2 * ((l >> 4) + 4 * v) + fr;
         const int col = fc * 16 + (l & 15);
+        if (ksplit > 1) {
+            part[(((size_t)ks * coltiles + ct) * rowtiles * kGemmRows + r) * 32 + col] = sum;
+            continue;
+        }
         const int k = ct * kGemmSys + col / 2, part = col & 1;
         if (r >= n || k >= nsys) continue;
         const int slot = slots[k];
@@ -370,6 +381,28 @@ __global__ __launch_bounds__(256) void k_bem_gemm(int n, int cpad, int steps_per
         }
         reinterpret_cast<double *>(o.y)[2 * ((size_t)o.ldy * (o.y_by_k ? k : slot) + r) + part] = o.sign * val;
     }
+}
+
+// Sum of the ksplit partials of each (row, real column) in order, then the
+// epilogue of k_bem_gemm.  One work-item per (column tile, row, real column).
+__global__ void k_gemm_reduce(int n, int rowpad, int coltiles, int ksplit, const double *__restrict__ part,
+                              const int *__restrict__ slots, int nsys, GemmOut o) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)coltiles * n * 32) return;
+    const int col = (int)(e % 32);
+    const int r = (int)((e / 32) % n);
+    const int ct = (int)(e / 32 / n);
+    const int k = ct * kGemmSys + col / 2, pt = col & 1;
+    if (k >= nsys) return;
+    double val = 0.0;
+    for (int ks = 0; ks < ksplit; ++ks) val += part[(((size_t)ks * coltiles + ct) * rowpad + r) * 32 + col];
+    const int slot = slots[k];
+    if (o.lam) {
+        const double2 l2 = o.lam[slot];
+        const double2 uu = o.u[(size_t)o.ldu * (o.u_by_k ? k : slot) + r];
+        val += pt ? l2.x * uu.y + l2.y * uu.x : l2.x * uu.x - l2.y * uu.y;
+    }
+    reinterpret_cast<double *>(o.y)[2 * ((size_t)o.ldy * (o.y_by_k ? k : slot) + r) + pt] = o.sign * val;
 }
 
 __global__ void k_bem_rhs_dpcm(int n, const double *__restrict__ normal, double e0, double2 *__restrict__ rhs) {
@@ -485,26 +518,35 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
     const int n = M.n;
     hipStream_t st = c->stream;
     c->work_t.alloc((size_t)nsys * n);
-    // K steps of 4 columns, split over the 4 waves of a workgroup in
-    // multiples of 4; the packed X is zero-padded to cover every wave's range
-    const int ksteps = (n + 3) / 4;
-    const int spw = (ksteps + 15) / 16 * 4;
-    const int cpad = 16 * spw;
     const int coltiles = (nsys + kGemmSys - 1) / kGemmSys;
     const int rowtiles = (n + kGemmRows - 1) / kGemmRows;
+    // K steps of 4 columns over ksplit workgroups x 4 waves, in multiples of
+    // 4 per wave; ksplit brings the launch to ~1024 workgroups (at least 64
+    // columns per wave), and the packed X is zero-padded to cover every range
+    const int ksteps = (n + 3) / 4;
+    int ksplit = std::max(1, std::min((1024 + rowtiles * coltiles - 1) / (rowtiles * coltiles), n / 256));
+    ksplit = std::min(ksplit, 16);
+    const int spw = (ksteps + 16 * ksplit - 1) / (16 * ksplit) * 4;
+    const int cpad = 16 * spw * ksplit;
     c->work_pack.alloc((size_t)coltiles * cpad * 32);
+    if (ksplit > 1) c->work_gemm.alloc((size_t)ksplit * coltiles * rowtiles * kGemmRows * 32);
     const size_t pack_items = (size_t)coltiles * cpad * kGemmSys;
     const unsigned pack_blocks = (unsigned)((pack_items + 255) / 256);
-    const unsigned gemm_blocks = 8u * (unsigned)((rowtiles * coltiles + 7) / 8);
+    const unsigned gemm_blocks = 8u * (unsigned)((rowtiles * coltiles * ksplit + 7) / 8);
+    const unsigned red_blocks = (unsigned)(((size_t)coltiles * n * 32 + 255) / 256);
     auto gemm = [&](const double *Md, const float *Mf, const double2 *xin, int x_by_k, const GemmOut &o) {
         k_gemm_pack<<<pack_blocks, 256, 0, st>>>(n, cpad, nsys, d_sys_list, xin, x_by_k ? n : ld, x_by_k, coltiles,
                                                  c->work_pack.p);
+        double *part = ksplit > 1 ? c->work_gemm.p : nullptr;
         if (fp32)
-            k_bem_gemm<float><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, Mf, c->work_pack.p,
-                                                           d_sys_list, nsys, o);
+            k_bem_gemm<float><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, ksplit, Mf,
+                                                           c->work_pack.p, d_sys_list, nsys, o, part);
         else
-            k_bem_gemm<double><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, Md, c->work_pack.p,
-                                                            d_sys_list, nsys, o);
+            k_bem_gemm<double><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, ksplit, Md,
+                                                            c->work_pack.p, d_sys_list, nsys, o, part);
+        if (ksplit > 1)
+            k_gemm_reduce<<<red_blocks, 256, 0, st>>>(n, rowtiles * kGemmRows, coltiles, ksplit, part, d_sys_list,
+                                                      nsys, o);
     };
     if (M.variant == PXH_DPCM) {
         // y = D x + lambda x

@@ -207,6 +207,33 @@ def test_bem_matvec_matches_oracle(gpu_ctx, case, drop, nvec):
         assert relerr(got[k], ref[k]) < 1e-6, k
 
 
+@pytest.mark.parametrize("variant", ["dpcm", "iefpcm"])
+@pytest.mark.parametrize("nvec", [3, 40])
+def test_bem_matvec_split_k(gpu_ctx, variant, nvec):
+    """1280 tesserae: the product columns are split over several workgroups
+    per tile (partials summed by a second kernel); against the oracle's
+    matrices as above."""
+    from plasmonx_amd import geometry
+    from plasmonx_amd.model import Mesh
+    m = Mesh(verts_bohr=geometry.icosphere(60.0, 3, seed=1) / 0.529177210903, variant=variant,
+             green="approximate", sphere_r_bohr=60.0 / 0.529177210903)
+    freq = np.array([3.2, 3.6]) / 27.211386245981
+    eps = pm.silver_etchegoin(freq)
+    gpu_ctx.set_mesh(m)
+    gpu_ctx.set_permittivity(eps)
+    D, Di, SA = pxo.bem_d(m)
+    lam = 2 * np.pi * (m.eps_solvent + eps[1]) / (m.eps_solvent - eps[1])
+    A = D + lam * np.eye(m.n)
+    if variant == "iefpcm":
+        A = -A @ SA
+    rng = np.random.default_rng(nvec)
+    x = rng.normal(size=(nvec, m.n)) + 1j * rng.normal(size=(nvec, m.n))
+    ref = x @ A.T
+    got = gpu_ctx.matvec(freq[1], 1, x)
+    for k in range(nvec):
+        assert relerr(got[k], ref[k]) < 1e-12, k
+
+
 @pytest.mark.parametrize("mfma", [0, 1])
 @pytest.mark.parametrize("nvec", [1, 5, 50])
 def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
