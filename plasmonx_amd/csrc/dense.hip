@@ -474,7 +474,8 @@ size_t freq_chunk(pxh_ctx *c, int n, int n_freq) {
 
 void constant_matrix_atoms(pxh_ctx *c, double *d_m);
 
-int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_stride,
+int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_fstride,
+                          int rhs_stride,
                           double2 *d_sol, int pol_mask) {
     hipStream_t st = c->stream;
     const bool bem = c->target == PXH_BEM;
@@ -502,8 +503,8 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
         PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, st));
         lu_factor(c, n, nf, A.p, piv.p, info.p);
         double2 *X = d_sol + (size_t)rhs_stride * f0;
-        k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, 3, pol_mask, d_rhs_all + (size_t)rhs_stride * f0,
-                                                                 (size_t)rhs_stride, X, (size_t)rhs_stride);
+        k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, 3, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0,
+                                                                 (size_t)rhs_fstride, X, (size_t)rhs_stride);
         k_lu_solve<<<dim3(3, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X, (size_t)rhs_stride, n);
         if (cc) {
             k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, X1.p, (size_t)n);

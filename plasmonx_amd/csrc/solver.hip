@@ -931,34 +931,81 @@ __global__ __launch_bounds__(64) void k_ms_solve(int nsh, const int *__restrict_
     }
 }
 
-// x of every shift of one right-hand side f: V_f(:, 0:J) y[q], written to
-// sol + stride s + nv pol (the solution layout [frequency][nv x 3]).
+// Column jcol[s] of a per-slot basis from the slot vectors: B[s][jcol[s]] = W[s].
+__global__ void k_save_col(int nv, const int *__restrict__ slots, const int *__restrict__ jcol,
+                           const double2 *__restrict__ W, double2 *__restrict__ Bs, size_t ldb_slot) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    if (r >= nv) return;
+    Bs[ldb_slot * s + (size_t)nv * jcol[s] + r] = W[(size_t)nv * s + r];
+}
+
+// Solutions and FP64 true residuals of every shift of one right-hand side f:
+// x_s = V_f(:, 0:J) y_s, written to sol + stride s + nv pol, and
+// b - A(omega_s) x_s = b - (W_f y_s + sigma_s x_s) with W_f(:, j) = A0 v_j,
+// the FP64 products of the Arnoldi process (A0 x_s = sum_j y_sj A0 v_j).
+// part[(blk * nsh + s)] = sum over the workgroup's rows of |r|^2.
 constexpr int kMsShifts = 8;
-__global__ __launch_bounds__(256) void k_ms_update_x(int nv, int nsh, int f, int pol, const int *__restrict__ jlen,
-                                                     const double2 *__restrict__ V, size_t ldv_slot,
-                                                     const double2 *__restrict__ y, int ldy,
-                                                     double2 *__restrict__ sol, size_t stride) {
+__global__ __launch_bounds__(256) void k_ms_finish(int nv, int nsh, int f, int pol, const int *__restrict__ jlen,
+                                                   const double2 *__restrict__ V, const double2 *__restrict__ Wb,
+                                                   size_t ldv_slot, const double2 *__restrict__ y, int ldy,
+                                                   const double2 *__restrict__ sigma, const double2 *__restrict__ b,
+                                                   double2 *__restrict__ sol, size_t stride,
+                                                   double *__restrict__ part) {
+    __shared__ double red[4][kMsShifts];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int s0 = blockIdx.y * kMsShifts;
-    if (r >= nv) return;
     const int J = jlen[f];
     const int ns = min(kMsShifts, nsh - s0);
-    const double2 *v = V + ldv_slot * f + r;
     const double2 *yq = y + ((size_t)f * nsh + s0) * ldy;
-    double2 acc[kMsShifts];
+    double2 ax[kMsShifts], aw[kMsShifts];
 #pragma unroll
-    for (int k = 0; k < kMsShifts; ++k) acc[k] = make_double2(0, 0);
-    for (int i = 0; i < J; ++i) {
-        const double2 vi = v[(size_t)nv * i];
+    for (int k = 0; k < kMsShifts; ++k) ax[k] = aw[k] = make_double2(0, 0);
+    if (r < nv) {
+        const double2 *v = V + ldv_slot * f + r;
+        const double2 *w = Wb + ldv_slot * f + r;
+        for (int i = 0; i < J; ++i) {
+            const double2 vi = v[(size_t)nv * i], wi = w[(size_t)nv * i];
 #pragma unroll
-        for (int k = 0; k < kMsShifts; ++k) {
-            if (k >= ns) break;
-            const double2 p = cmul(vi, yq[(size_t)k * ldy + i]);
-            acc[k].x += p.x;
-            acc[k].y += p.y;
+            for (int k = 0; k < kMsShifts; ++k) {
+                if (k >= ns) break;
+                const double2 yk = yq[(size_t)k * ldy + i];
+                const double2 p = cmul(vi, yk), q = cmul(wi, yk);
+                ax[k].x += p.x;
+                ax[k].y += p.y;
+                aw[k].x += q.x;
+                aw[k].y += q.y;
+            }
         }
     }
-    for (int k = 0; k < ns; ++k) sol[stride * (s0 + k) + (size_t)nv * pol + r] = acc[k];
+    const double2 bv = r < nv ? b[r] : make_double2(0, 0);
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < kMsShifts; ++k) {
+        double q2 = 0.0;
+        if (k < ns && r < nv) {
+            sol[stride * (s0 + k) + (size_t)nv * pol + r] = ax[k];
+            const double2 sg = sigma[s0 + k];
+            const double2 sx = cmul(sg, ax[k]);
+            const double rx = bv.x - (aw[k].x + sx.x), ry = bv.y - (aw[k].y + sx.y);
+            q2 = rx * rx + ry * ry;
+        }
+        for (int off = 32; off > 0; off >>= 1) q2 += __shfl_xor(q2, off, 64);
+        if (lane == 0) red[wv][k] = q2;
+    }
+    __syncthreads();
+    if (threadIdx.x < ns)
+        part[(size_t)blockIdx.x * nsh + s0 + threadIdx.x] =
+            (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// tres[q] = sqrt(sum over workgroups of part), fixed order; one thread per shift.
+__global__ void k_ms_tres(int nsh, int nblk, int f, const double *__restrict__ part, double *__restrict__ tres) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsh) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += part[(size_t)b * nsh + s];
+    tres[(size_t)f * nsh + s] = sqrt(acc);
 }
 
 }  // namespace
@@ -991,7 +1038,7 @@ double2 bem_lambda(const pxh_ctx *c, int i_freq) {
 // warm: the systems start from the solutions already in d_sol (else x = 0).
 template <typename VT>
 static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
-                         int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
+                         int rhs_fstride, int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
                          bool warm, double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
                          std::vector<double> &tres_out, std::vector<double> &t_admit, std::vector<double> &t_done) {
     Batched S;
@@ -1120,7 +1167,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             S.h_par[s] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
             upload(S, S.par.p + s, &S.h_par[s], 1);
         }
-        k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, s, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
+        k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, s, d_rhs_all + (size_t)rhs_fstride * sy.ifreq, sy.pol,
                                                     S.Bv.p, S.R.p, S.X.p);
         if (S.precond) {
             if (c->target == PXH_BEM)
@@ -1344,13 +1391,13 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
 }
 
 static int run_batched(pxh_ctx *c, int nv, const std::vector<System> &systems, const double2 *d_rhs_all,
-                       int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain, bool warm,
+                       int rhs_fstride, int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain, bool warm,
                        double2 *d_sol, std::vector<int> &iters_out, std::vector<double> &tres_out,
                        std::vector<double> &t_admit, std::vector<double> &t_done) {
     if (precision == PXH_MIXED && !plain)
-        return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, warm,
+        return run_batched_t<float2>(c, nv, systems, d_rhs_all, rhs_fstride, rhs_stride, freq, tol, kdim, precision, plain, warm,
                                      d_sol, iters_out, tres_out, t_admit, t_done);
-    return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_stride, freq, tol, kdim, precision, plain, warm,
+    return run_batched_t<double2>(c, nv, systems, d_rhs_all, rhs_fstride, rhs_stride, freq, tol, kdim, precision, plain, warm,
                                   d_sol, iters_out, tres_out, t_admit, t_done);
 }
 
@@ -1413,7 +1460,7 @@ struct PinnedDoubles {
 // current iterate left in d_sol as a starting point.
 static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems, int n_freq,
                           const std::vector<double2> &sigma, const double *freq, const double2 *d_rhs_all,
-                          int rhs_stride, double tol, int kdim, double2 *d_sol, std::vector<int> &iters_out,
+                          int rhs_fstride, int rhs_stride, double tol, int kdim, double2 *d_sol, std::vector<int> &iters_out,
                           std::vector<double> &tres_out, std::vector<double> &t_done, std::vector<int> &unsolved) {
     hipStream_t st = c->stream;
     const auto t_start = std::chrono::steady_clock::now();
@@ -1457,7 +1504,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
     S.jp1_d.alloc(B);
     S.jl_d.alloc(B);
     S.slots_d.alloc(B);
-    S.ring.init(std::max(sizeof(SysParam), sizeof(int) * (size_t)B), 4 * B + 64);
+    S.ring.init(std::max(sizeof(SysParam), sizeof(int) * (size_t)B) * (size_t)B, 4 * B + 64);
     if (bem) {
         S.lam.alloc(B);
         S.h_lam.assign(B, make_double2(0, 0));
@@ -1465,9 +1512,14 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         S.par.alloc(B);
         S.h_par.assign(B, SysParam{});
     }
-    DevBuf<double2> V, Hs, gq, snq, colq, sig;
-    DevBuf<double> csq, msres;
+    DevBuf<double2> V, Wb, Hs, gq, snq, colq, sig;
+    DevBuf<double> csq, msres, mspart, mstres;
     V.alloc(S.ldv_slot * F);
+    Wb.alloc(S.ldv_slot * F);  // A0 v_j, kept for the true residuals
+    const int fin_blk = (nv + 255) / 256;
+    mspart.alloc((size_t)fin_blk * nsh);
+    mstres.alloc((size_t)nsh * F);
+    PXH_HIP(hipMemsetAsync(mstres.p, 0, sizeof(double) * nsh * F, st));
     Hs.alloc(ldH_f * F);
     gq.alloc((size_t)nsh * F * ldg);
     snq.alloc((size_t)nsh * F * ldg);
@@ -1527,6 +1579,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         k_stage<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot, nullptr,
                                                        S.stage.p);
         apply_op(S, na, true, S.stage.p, S.W.p);
+        k_save_col<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, S.W.p, Wb.p, S.ldv_slot);
         c->stats.v[2] += 1;
         c->stats.v[0] += na;
         for (int f : act) S.h_jp1[f] = S.jcount[f] + 1;
@@ -1583,8 +1636,10 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
             upload_list(S, one);
             k_ms_solve<<<dim3(nsh, 1), 64, 0, st>>>(nsh, S.slots_d.p, S.jl_d.p, Hs.p, ldH, ldH_f, sig.p, csq.p,
                                                     snq.p, gq.p, ldg, colq.p);
-            k_ms_update_x<<<dim3((nv + 255) / 256, (nsh + kMsShifts - 1) / kMsShifts), 256, 0, st>>>(
-                nv, nsh, f, pols[f], S.jl_d.p, V.p, S.ldv_slot, gq.p, ldg, d_sol, (size_t)rhs_stride);
+            k_ms_finish<<<dim3(fin_blk, (nsh + kMsShifts - 1) / kMsShifts), 256, 0, st>>>(
+                nv, nsh, f, pols[f], S.jl_d.p, V.p, Wb.p, S.ldv_slot, gq.p, ldg, sig.p, S.Bv.p + (size_t)nv * f,
+                d_sol, (size_t)rhs_stride, mspart.p);
+            k_ms_tres<<<(nsh + 255) / 256, 256, 0, st>>>(nsh, fin_blk, f, mspart.p, mstres.p);
             PXH_HIP(hipGetLastError());
             const double t = now_s();
             for (int s = 0; s < nsh; ++s) {
@@ -1597,38 +1652,18 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         }
         act = keep;
     }
-    // FP64 true residual of every system, B at a time
-    for (size_t k0 = 0; k0 < systems.size(); k0 += B) {
-        const int nc = (int)std::min<size_t>(B, systems.size() - k0);
-        std::vector<int> lst(nc);
-        for (int i = 0; i < nc; ++i) {
-            const System &sy = systems[k0 + i];
-            lst[i] = i;
-            if (bem) {
-                S.h_lam[i] = bem_lambda(c, sy.ifreq);
-                upload(S, S.lam.p + i, &S.h_lam[i], 1);
-            } else {
-                S.h_par[i] = make_sysparam(c, freq[sy.ifreq], sy.ifreq);
-                upload(S, S.par.p + i, &S.h_par[i], 1);
-            }
-            k_load_rhs<<<(nv + 255) / 256, 256, 0, st>>>(nv, i, d_rhs_all + (size_t)rhs_stride * sy.ifreq, sy.pol,
-                                                        S.Bv.p, S.R.p, S.X.p);
-            PXH_HIP(hipMemcpyAsync(S.X.p + (size_t)nv * i, d_sol + (size_t)rhs_stride * sy.ifreq + (size_t)nv * sy.pol,
-                                   sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
-        }
-        upload_list(S, lst);
-        apply_op(S, nc, true, S.X.p, S.W.p);
-        c->stats.v[2] += 1;
-        c->stats.v[0] += nc;
-        k_residual<<<grid_rows(nc), 256, 0, st>>>(nv, S.slots_d.p, S.Bv.p, S.W.p, S.R.p);
-        k_norm<<<nc, RB, 0, st>>>(nv, S.slots_d.p, S.R.p, (size_t)nv, S.nrm.p);
-        PXH_HIP(hipGetLastError());
-        PXH_HIP(hipMemcpyAsync(pin.p, S.nrm.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));
+    // FP64 true residuals (k_ms_finish), one readback
+    {
+        std::vector<double> tr((size_t)nsh * F);
+        PXH_HIP(hipMemcpyAsync(tr.data(), mstres.p, sizeof(double) * tr.size(), hipMemcpyDeviceToHost, st));
         PXH_HIP(hipStreamSynchronize(st));
-        for (int i = 0; i < nc; ++i) {
-            tres_out[k0 + i] = pin.p[i];
-            if (!(pin.p[i] < tol)) unsolved.push_back((int)(k0 + i));
-        }
+        for (int f = 0; f < F; ++f)
+            for (int s = 0; s < nsh; ++s) {
+                const int k = sys_at[(size_t)f * nsh + s];
+                if (k < 0) continue;
+                tres_out[k] = tr[(size_t)f * nsh + s];
+                if (!(tres_out[k] < tol)) unsolved.push_back(k);
+            }
     }
     return 0;
 }
@@ -1636,9 +1671,11 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
 
 // polar[a + 3 b] = sum_i var_b(i) x_a(i) / E0 (+ dipoles); BEM: area-weighted for DPCM.
 __global__ void k_polar_partial(int n, int nv, int fmu, int bem_dpcm, const double4 *__restrict__ pos,
-                                const double *__restrict__ area, const double2 *__restrict__ var,
-                                double2 *__restrict__ out /* [blocks][9] */) {
+                                const double *__restrict__ area, const double2 *__restrict__ var_all,
+                                size_t var_stride, double2 *__restrict__ out_all /* [freq][blocks][9] */) {
     __shared__ double2 sh[9][RB];
+    const double2 *var = var_all + var_stride * blockIdx.y;
+    double2 *out = out_all + (size_t)gridDim.x * 9 * blockIdx.y;
     const int tid = threadIdx.x;
     double2 acc[9];
     for (int k = 0; k < 9; ++k) acc[k] = make_double2(0, 0);
@@ -1708,7 +1745,8 @@ void results_row(double freq, const double *pol, double *r) {
     for (int k = 0; k < 4; ++k) r[16 + k] = r[8 + k] + r[12 + k];
 }
 
-int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_stride,
+int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const double2 *d_rhs_all, int rhs_fstride,
+                          int rhs_stride,
                           double2 *d_sol, int pol_mask);
 
 namespace {
@@ -1747,7 +1785,7 @@ static void ief_polar_weights(pxh_ctx *c, DevBuf<double4> &z4) {
     const double f0 = 0.0;
     const int saved = c->bem_op;
     c->bem_op = 2;
-    const int stt = run_batched(c, n, sys, rhs.p, 3 * n, &f0, 1e-13 * std::max(cmax, 1e-300), std::min(n, 300),
+    const int stt = run_batched(c, n, sys, rhs.p, 0, 3 * n, &f0, 1e-13 * std::max(cmax, 1e-300), std::min(n, 300),
                                 PXH_FP64, false, false, z.p, it, tr, ta, tf);
     c->bem_op = saved;
     PXH_CHECK(stt == PXH_OK, PXH_ENOCONV, "IEF: S z = c did not converge");
@@ -1770,7 +1808,11 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         PXH_CHECK(c->alpha_nf >= n_freq, PXH_EINVAL, "alpha(omega) table shorter than the frequency list");
     const size_t stride = (size_t)nv * 3;
     DevBuf<double2> rhs_all, sol;
-    rhs_all.alloc(stride * n_freq);
+    // right-hand sides: one per frequency for alloys (H-bar(omega) enters
+    // them), else one shared by all frequencies (frequency stride 0)
+    const bool rhs_per_freq = !bem && c->atoms.hetero;
+    const size_t rhs_fstride = rhs_per_freq ? stride : 0;
+    rhs_all.alloc(rhs_per_freq ? stride * n_freq : stride);
     sol.alloc(stride * n_freq);
     // starting iterates: the caller's variables with the "initial_guess"
     // option on the iterative methods (else x0 = 0, as the reference)
@@ -1779,22 +1821,11 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
         PXH_HIP(hipMemcpyAsync(sol.p, variables, sizeof(double2) * stride * n_freq, hipMemcpyHostToDevice, st));
     else
         PXH_HIP(hipMemsetAsync(sol.p, 0, sizeof(double2) * stride * n_freq, st));
-    // right-hand sides per frequency (frequency dependent only for alloys)
-    if (bem) {
+    if (bem)
         dynamic_rhs_bem(c, e0, rhs_all.p);
-        for (int f = 1; f < n_freq; ++f)
-            PXH_HIP(hipMemcpyAsync(rhs_all.p + stride * f, rhs_all.p, sizeof(double2) * stride,
-                                   hipMemcpyDeviceToDevice, st));
-    } else {
-        for (int f = 0; f < n_freq; ++f) {
-            if (f > 0 && !c->atoms.hetero) {
-                PXH_HIP(hipMemcpyAsync(rhs_all.p + stride * f, rhs_all.p, sizeof(double2) * stride,
-                                       hipMemcpyDeviceToDevice, st));
-                continue;
-            }
+    else
+        for (int f = 0; f < (rhs_per_freq ? n_freq : 1); ++f)
             dynamic_rhs_atoms(c, freq[f], e0, rhs_form, rhs_all.p + stride * f);
-        }
-    }
     std::vector<int> it_sys;
     std::vector<double> tr_sys, t_adm, t_fin;
     bool ief_u = false;      // IEF solved for u = S q (polarizabilities only)
@@ -1805,7 +1836,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             if (pol_mask & (1 << k)) systems.push_back({f, k});
     int status = 0;
     if (method == PXH_INVERSION) {
-        status = solve_dense_inversion(c, n_freq, freq, rhs_all.p, (int)stride, sol.p, pol_mask);
+        status = solve_dense_inversion(c, n_freq, freq, rhs_all.p, (int)rhs_fstride, (int)stride, sol.p, pol_mask);
         it_sys.assign(systems.size(), 0);
         tr_sys.assign(systems.size(), 0.0);
     } else {
@@ -1837,7 +1868,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                 // one Krylov basis for all frequencies; the few systems it
                 // leaves above tol continue in the batched solver from there
                 std::vector<int> rest;
-                status = run_multishift(c, nv, systems, n_freq, sigma, freq, rhs_all.p, (int)stride, tol, kdim,
+                status = run_multishift(c, nv, systems, n_freq, sigma, freq, rhs_all.p, (int)rhs_fstride, (int)stride,
+                                        tol, kdim,
                                         sol.p, it_sys, tr_sys, t_fin, rest);
                 t_adm.assign(systems.size(), 0.0);
                 if (!rest.empty()) {
@@ -1846,7 +1878,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                     std::vector<int> it2;
                     std::vector<double> tr2, ta2, tf2;
                     const double t_ms = *std::max_element(t_fin.begin(), t_fin.end());
-                    status = run_batched(c, nv, sub, rhs_all.p, (int)stride, freq, tol, kdim, prec, false, true,
+                    status = run_batched(c, nv, sub, rhs_all.p, (int)rhs_fstride, (int)stride, freq, tol, kdim, prec,
+                                         false, true,
                                          sol.p, it2, tr2, ta2, tf2);
                     for (size_t i = 0; i < rest.size(); ++i) {
                         it_sys[rest[i]] += it2[i];
@@ -1855,7 +1888,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                     }
                 }
             } else {
-                status = run_batched(c, nv, systems, rhs_all.p, (int)stride, freq, tol, kdim, prec, plain, x0_given,
+                status = run_batched(c, nv, systems, rhs_all.p, (int)rhs_fstride, (int)stride, freq, tol, kdim, prec,
+                                     plain, x0_given,
                                      sol.p, it_sys, tr_sys, t_adm, t_fin);
             }
             if (ief_u) ief_polar_weights(c, ief_z);
@@ -1906,9 +1940,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     const int n_sites = bem ? nv : c->atoms.n;
     const int fmu = (!bem && c->atoms.model == PXH_WFQFMU) ? 1 : 0;
     const int dpcm = (bem && c->mesh.variant == PXH_DPCM) ? 1 : 0;
-    for (int f = 0; f < n_freq; ++f)
-        k_polar_partial<<<nb, RB, 0, st>>>(n_sites, nv, fmu, dpcm, posp, area, sol.p + stride * f,
-                                           part.p + (size_t)nb * 9 * f);
+    k_polar_partial<<<dim3(nb, n_freq), RB, 0, st>>>(n_sites, nv, fmu, dpcm, posp, area, sol.p, stride, part.p);
     PXH_HIP(hipGetLastError());
     std::vector<double2> hp((size_t)nb * 9 * n_freq);
     PXH_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, st));
