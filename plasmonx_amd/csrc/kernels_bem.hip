@@ -427,6 +427,7 @@ __global__ void k_bem_rhs_ief(int n, const double *__restrict__ Dief, const doub
 
 void build_mesh_device(pxh_ctx *c) {
     Mesh &M = c->mesh;
+    M.ief_z_valid = false;
     const int n = M.n;
     hipStream_t st = c->stream;
     // tesserae (bem_module.f90:420-491), host side: O(n)

@@ -111,6 +111,8 @@ struct Mesh {
     DevBuf<double> Dief;                // n*n, IEF: D (unrefined diagonal) - 2 pi I
     DevBuf<double> SA;                  // n*n, IEF: S A^-1
     DevBuf<float> Df, SAf;              // FP32 copies for the mixed path
+    DevBuf<double4> ief_z;              // IEF: S^-1 c (x, y, z, 0) per tessera, once computed
+    bool ief_z_valid = false;
     bool built = false;
 };
 

@@ -1759,8 +1759,10 @@ __global__ void k_pack_z(int n, const double2 *__restrict__ z, double4 *__restri
 // z_a = S^-1 c_a for the three Cartesian components of the tessera centroids
 // (FP64 batched GMRES on the operator form S, to 1e-13 relative), packed as
 // one double4 per tessera.  S is real symmetric positive definite, so z is
-// real.
-static void ief_polar_weights(pxh_ctx *c, DevBuf<double4> &z4) {
+// real.  It depends on the mesh alone and is kept with it.
+static void ief_polar_weights(pxh_ctx *c) {
+    if (c->mesh.ief_z_valid) return;
+    DevBuf<double4> &z4 = c->mesh.ief_z;
     const int n = c->mesh.n;
     hipStream_t st = c->stream;
     std::vector<double2> hc((size_t)3 * n);
@@ -1792,6 +1794,8 @@ static void ief_polar_weights(pxh_ctx *c, DevBuf<double4> &z4) {
     z4.alloc(n);
     k_pack_z<<<(n + 255) / 256, 256, 0, st>>>(n, z.p, z4.p);
     PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(st));
+    c->mesh.ief_z_valid = true;
 }
 
 int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
@@ -1828,8 +1832,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             dynamic_rhs_atoms(c, freq[f], e0, rhs_form, rhs_all.p + stride * f);
     std::vector<int> it_sys;
     std::vector<double> tr_sys, t_adm, t_fin;
-    bool ief_u = false;      // IEF solved for u = S q (polarizabilities only)
-    DevBuf<double4> ief_z;   // then alpha uses z_a = S^-1 c_a in place of c_a
+    bool ief_u = false;      // IEF solved for u = S q (polarizabilities only): alpha uses z_a = S^-1 c_a
     std::vector<System> systems;
     for (int f = 0; f < n_freq; ++f)
         for (int k = 0; k < 3; ++k)
@@ -1892,7 +1895,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
                                      plain, x0_given,
                                      sol.p, it_sys, tr_sys, t_adm, t_fin);
             }
-            if (ief_u) ief_polar_weights(c, ief_z);
+            if (ief_u) ief_polar_weights(c);
         } catch (...) {
             c->use_stored = false;
             c->atoms.Tstore.release();
@@ -1932,7 +1935,7 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
             hp[i] = make_double4(c->mesh.h_coord[3 * i], c->mesh.h_coord[3 * i + 1], c->mesh.h_coord[3 * i + 2], 0.0);
         bpos.upload(hp.data(), hp.size(), st);
         barea.upload(c->mesh.h_area.data(), nv, st);
-        posp = ief_u ? ief_z.p : bpos.p;
+        posp = ief_u ? c->mesh.ief_z.p : bpos.p;
         area = barea.p;
     } else {
         posp = c->atoms.pos.p;
