@@ -154,7 +154,14 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * products on MFMA), "matvec_mixed" (pxh_matvec uses the low-precision
  * product), "splits", "max_outer", "reorth_kappa" (batched classical
  * Gram-Schmidt: second pass only when the first leaves less than kappa of
- * the norm; default 1/sqrt(2), >= 1 = always). */
+ * the norm; default 1/sqrt(2), >= 1 = always), "multishift" (1, the
+ * default: spectra of BEM DPCM, BEM IEF without `variables`, and
+ * homogeneous omegaFQ share one Krylov basis for all frequencies; 0 = the
+ * batched solver), "x0_projection" (0..16, default 8: the batched solver
+ * starts each system from the best combination of that many converged
+ * neighbouring frequencies; 0 = x0 = 0), "initial_guess" (1: the iterative
+ * methods start from the caller's `variables`; default 0, x0 = 0 as the
+ * reference). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
 /* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector
