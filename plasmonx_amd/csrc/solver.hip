@@ -761,21 +761,38 @@ __global__ __launch_bounds__(64) void k_proj_solve(int nblk, const long long *__
 }
 
 // X[s] = sum_i coef_i x_i for each admitted slot.
-__global__ void k_proj_x0(int nv, const int *__restrict__ adm, const long long *__restrict__ info,
-                          const double2 *__restrict__ coef, const double2 *__restrict__ sol,
-                          double2 *__restrict__ X) {
+__global__ void k_proj_x0(int nv, int n, int nt, const double4 *__restrict__ pos, const double *__restrict__ nbfrac,
+                          const SysParam *__restrict__ par, const int *__restrict__ adm,
+                          const long long *__restrict__ info, const double2 *__restrict__ coef,
+                          const double2 *__restrict__ sol, const double2 *__restrict__ mx,
+                          const double2 *__restrict__ Bv, int with_res, double2 *__restrict__ X,
+                          double2 *__restrict__ R) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int a = blockIdx.y;
     if (r >= nv) return;
+    const int s = adm[a];
     const long long *in = info + (size_t)a * (kProjMax + 1);
     const int k = (int)in[0];
-    double2 v = make_double2(0, 0);
+    double2 v = make_double2(0, 0), m = make_double2(0, 0);
     for (int i = 0; i < k; ++i) {
-        const double2 p = cmul(coef[(size_t)a * kProjMax + i], sol[in[1 + i] + r]);
+        const double2 ci = coef[(size_t)a * kProjMax + i];
+        const double2 p = cmul(ci, sol[in[1 + i] + r]);
         v.x += p.x;
         v.y += p.y;
+        if (with_res) {
+            const double2 q = cmul(ci, mx[in[1 + i] + r]);
+            m.x += q.x;
+            m.y += q.y;
+        }
     }
-    X[(size_t)nv * adm[a] + r] = v;
+    X[(size_t)nv * s + r] = v;
+    if (with_res) {
+        // A(omega) x0 = sum_i c_i (M x_i + S(omega) x_i) = M x0 + S(omega) x0
+        const double2 sh = row_shift(r, n, nt, pos, nbfrac, par[s]);
+        const double2 sx = cmul(sh, v);
+        const double2 b = Bv[(size_t)nv * s + r];
+        R[(size_t)nv * s + r] = make_double2(b.x - (m.x + sx.x), b.y - (m.y + sx.y));
+    }
 }
 
 // ------------------------------------------------------------ multi-shift
@@ -1197,7 +1214,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         }
     };
 
-    std::vector<int> res_first;  // slots whose residual is b (x = 0)
+    std::vector<int> res_first;  // slots whose residual R is already known (x = 0, or a projected start)
     while (done < systems.size()) {
         // fill free slots
         res_first.clear();
@@ -1222,7 +1239,15 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                                                                    proj_adm.p, proj_info.p, d_sol, mx.p, S.Bv.p,
                                                                    proj_part.p);
             k_proj_solve<<<na, 64, 0, st>>>(proj_nblk, proj_info.p, proj_part.p, proj_coef.p);
-            k_proj_x0<<<grid_rows(na), 256, 0, st>>>(nv, proj_adm.p, proj_info.p, proj_coef.p, d_sol, S.X.p);
+            // homogeneous targets: M is frequency independent, so the
+            // residual of the start follows from the neighbours' FP64
+            // products without one of its own; alloys take a product
+            const int with_res = A.hetero ? 0 : 1;
+            k_proj_x0<<<grid_rows(na), 256, 0, st>>>(nv, A.n, A.nt, A.pos.p, A.d_nbfrac.p, S.par.p, proj_adm.p,
+                                                     proj_info.p, proj_coef.p, d_sol, mx.p, S.Bv.p, with_res,
+                                                     S.X.p, S.R.p);
+            if (with_res)
+                for (int sl : adm_proj) res_first.push_back(sl);
             PXH_HIP(hipGetLastError());
         }
         // ----- residual checks (FP64)
