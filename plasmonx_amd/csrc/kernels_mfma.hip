@@ -28,6 +28,8 @@
 // 3 MFMAs.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 
@@ -362,15 +364,27 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     for (int g = 0; g < groups; ++g) {
         const int ns = std::min(SYS_PER_GROUP, nsys - g * SYS_PER_GROUP);
         const int *lst = d_sys_list + g * SYS_PER_GROUP;
-        k_col_scale<<<32 * NCB, 256, 0, c->stream>>>(nv, ns, lst, x, ld, c->work_scale.p, 32 * NCB);
-        k_prep_x<NCB><<<jsteps, 256, 0, c->stream>>>(n, ns, lst, x, ld, c->work_scale.p, c->work_xh.p,
-                                                     c->work_xl.p);
-        // the pair kernel alone, timed on the library's stream
-        const int t_kernel = timer_begin(c);
-        k_tsweep_mfma<NCB><<<iblocks * splits, 256, 0, c->stream>>>(
-            n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p,
-            c->work_xl.p, c->work_part.p, nv, splits, per, nsys, g * SYS_PER_GROUP);
-        timer_end(c, t_kernel, 12);
+        // column blocks of this pass: a partly filled batch runs only the
+        // blocks it occupies (16 systems each)
+        auto pass = [&](auto ncb_tag) {
+            constexpr int K = decltype(ncb_tag)::value;
+            k_col_scale<<<32 * K, 256, 0, c->stream>>>(nv, ns, lst, x, ld, c->work_scale.p, 32 * K);
+            k_prep_x<K><<<jsteps, 256, 0, c->stream>>>(n, ns, lst, x, ld, c->work_scale.p, c->work_xh.p,
+                                                       c->work_xl.p);
+            // the pair kernel alone, timed on the library's stream
+            const int t_kernel = timer_begin(c);
+            k_tsweep_mfma<K><<<iblocks * splits, 256, 0, c->stream>>>(
+                n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p,
+                c->work_xl.p, c->work_part.p, nv, splits, per, nsys, g * SYS_PER_GROUP);
+            timer_end(c, t_kernel, 12);
+        };
+        const int ncb = (ns + 15) / 16;
+        if (ncb == 1)
+            pass(std::integral_constant<int, 1>{});
+        else if (ncb == 2)
+            pass(std::integral_constant<int, 2>{});
+        else
+            pass(std::integral_constant<int, 3>{});
         c->stats.v[13] += 1;
     }
     PXH_HIP(hipGetLastError());

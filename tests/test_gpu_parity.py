@@ -235,7 +235,7 @@ def test_bem_matvec_split_k(gpu_ctx, variant, nvec):
 
 
 @pytest.mark.parametrize("mfma", [0, 1])
-@pytest.mark.parametrize("nvec", [1, 5, 50])
+@pytest.mark.parametrize("nvec", [1, 5, 20, 50])
 def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
     """The low-precision product (FP32 VALU tiles, or fp16 hi/lo MFMA) against
     the FP64 product on a 675-atom Ag sphere: relative error <= 2e-5 per vector
