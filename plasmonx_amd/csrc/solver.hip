@@ -1159,7 +1159,8 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     std::vector<long long> h_info;
 
     enum Phase { FREE, INNER, NEED_RES };
-    std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0);
+    std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0), res_wait(B, 0);
+    constexpr int kResBatch = 4, kResWait = 2;
     std::vector<double> beta(B, 0.0), target(B, 0.0);
     std::vector<int> total_it(B, 0);
     size_t next = 0, done = 0;
@@ -1250,14 +1251,33 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                 for (int sl : adm_proj) res_first.push_back(sl);
             PXH_HIP(hipGetLastError());
         }
-        // ----- residual checks (FP64)
-        std::vector<int> need;
-        for (int s = 0; s < B; ++s)
-            if (phase[s] == NEED_RES) need.push_back(s);
+        // ----- residual checks (FP64).  The slots whose x needs a product
+        // wait (idle, at most kResWait steps) until kResBatch of them can
+        // share one FP64 product, or no slot is left in an inner cycle; an
+        // idle slot costs the low-precision products nothing
+        std::vector<int> need, prod;
+        int n_inner = 0;
+        bool overdue = false;
+        for (int s = 0; s < B; ++s) {
+            if (phase[s] == INNER) ++n_inner;
+            if (phase[s] != NEED_RES) continue;
+            if (std::find(res_first.begin(), res_first.end(), s) != res_first.end()) {
+                need.push_back(s);
+            } else {
+                prod.push_back(s);
+                overdue = overdue || res_wait[s] >= kResWait;
+            }
+        }
+        if (!prod.empty() && !plain && (int)prod.size() < kResBatch && n_inner > 0 && !overdue) {
+            for (int s : prod) res_wait[s] += 1;
+            prod.clear();
+        }
+        for (int s : prod) {
+            res_wait[s] = 0;
+            need.push_back(s);
+        }
+        std::sort(need.begin(), need.end());
         if (!need.empty()) {
-            std::vector<int> prod;  // slots that need A x (x != 0)
-            for (int s : need)
-                if (std::find(res_first.begin(), res_first.end(), s) == res_first.end()) prod.push_back(s);
             if (!prod.empty()) {
                 upload_list(S, prod);
                 apply_op(S, (int)prod.size(), true, S.X.p, S.W.p);
