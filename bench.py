@@ -140,7 +140,8 @@ def main():
     achieved = st["sweep_flops_fp32"] / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-            "kernel": "k_tsweep_mfma<3> (fp16 hi/lo split, 3 MFMA products per element pair)",
+            "kernel": "k_tsweep_mfma<NCB> (fp16 hi/lo split, 3 MFMA products per element pair; NCB = 16-system "
+                      "column blocks occupied, 3 for a full 48-system pass)",
             "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 3)}
     latency = {"p50_ms": round(st["latency_p50_ms"], 1), "p99_ms": round(st["latency_p99_ms"], 1),
                "definition": "per frequency point: admission of its first field to convergence of its last, "
