@@ -349,6 +349,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
         } else if (k == "x0_projection") {
             PXH_CHECK(value >= 0 && value <= 16, PXH_EINVAL, "x0_projection must be 0..16");
             c->opt.x0_projection = (int)value;
+        } else if (k == "dcgs2") {
+            c->opt.dcgs2 = value != 0.0;
         } else if (k == "initial_guess") {
             c->opt.initial_guess = value != 0.0;
         } else if (k == "multishift") {

@@ -130,6 +130,7 @@ struct Options {
     double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
     int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies
     int initial_guess = 0;   // iterative methods start from the caller's variables
+    int dcgs2 = 1;           // batched solver: Gram-Schmidt with delayed re-orthogonalisation
     int x0_projection = 8;   // batched solver: start from the best x in the span of this many converged neighbours
 };
 

@@ -562,6 +562,239 @@ __global__ void k_add_h(const int *__restrict__ slots, const int *__restrict__ j
     }
 }
 
+// ------------------------------------------- delayed re-orthogonalisation
+// Classical Gram-Schmidt with the second pass of each vector delayed to the
+// next step (DCGS2: Swirydowicz, Langou, Ananthan, Yang, Thomas, Numer. Linear
+// Algebra Appl. 2020; Bielich et al., Parallel Comput. 2022), so that a step
+// reads the basis twice instead of four times.  At step j the basis holds
+// the final q_0..q_{j-1} and the candidate u_j (one pass, normalised).  One
+// pass of dots gives a = Q^H u_j, beta = u_j^H u_j, b = Q^H w, gamma = u_j^H w
+// for w = A u_j; then
+//   nu = sqrt(beta - |a|^2),  q_j = (u_j - Q a) / nu,
+//   column j-1 of H becomes final: h_{i,j-1} += h_{j,j-1} a_i, h_{j,j-1} *= nu,
+//   f = (b, (gamma - a^H b) / nu),  e = H a (final columns 0..j-1),
+//   column j (pending): c = (f - e) / nu,
+//   w <- (w - Q f_{0..j-1} - q_j f_j) / nu,  h_{j+1,j} = ||w||, u_{j+1} = w / ||w||,
+// with A q_j = (w - A Q a) / nu and A Q a = Q_{j+1} e.  One pass of updates
+// forms q_j and the new w together.
+
+// Dots of basis columns 0..jc-1 of each slot against W and against column
+// jc-1 itself (the candidate): partial sums per chunk of kDotRows rows, as
+// k_dots_part, into pw and pu.
+constexpr int kDot2Group = 4;  // basis vectors per pass of the two-vector dots (register budget)
+// Dots of basis columns 0..jc-1 of each slot against W and against column
+// jc-1 itself (the candidate): partial sums per chunk of kDotRows rows, as
+// k_dots_part, into pw and pu; kDot2Group columns per workgroup layer.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dots2_part(int nv, const int *__restrict__ slots,
+                                                    const int *__restrict__ jcount, const VT *__restrict__ V,
+                                                    size_t ldv_slot, const double2 *__restrict__ W,
+                                                    double2 *__restrict__ pw, double2 *__restrict__ pu, int ldh,
+                                                    int nch) {
+    constexpr int U = kDotRows / 64;
+    const int ch = blockIdx.x * 4 + threadIdx.x / 64;
+    const int s = slots[blockIdx.y];
+    const int lane = threadIdx.x % 64;
+    if (ch >= nch) return;
+    const int jc = jcount[s];
+    const int i0 = blockIdx.z * kDot2Group;
+    if (i0 >= jc) return;
+    const int r0 = ch * kDotRows + lane;
+    double2 w[U], u[U];
+    const double2 *ws = W + (size_t)nv * s;
+    const VT *us = V + ldv_slot * s + (size_t)nv * (jc - 1);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const bool in = r0 + 64 * k < nv;
+        w[k] = in ? ws[r0 + 64 * k] : make_double2(0, 0);
+        u[k] = in ? to_d2(us[r0 + 64 * k]) : make_double2(0, 0);
+    }
+    double2 aw[kDot2Group], au[kDot2Group];
+#pragma unroll
+    for (int g = 0; g < kDot2Group; ++g) {
+        aw[g] = au[g] = make_double2(0, 0);
+        if (i0 + g < jc) {
+            const VT *v = V + ldv_slot * s + (size_t)nv * (i0 + g);
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                if (r0 + 64 * k < nv) {
+                    const double2 vv = to_d2(v[r0 + 64 * k]);
+                    const double2 p = cconjmul(vv, w[k]), q = cconjmul(vv, u[k]);
+                    aw[g].x += p.x;
+                    aw[g].y += p.y;
+                    au[g].x += q.x;
+                    au[g].y += q.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kDot2Group; ++g) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            aw[g].x += __shfl_xor(aw[g].x, off, 64);
+            aw[g].y += __shfl_xor(aw[g].y, off, 64);
+            au[g].x += __shfl_xor(au[g].x, off, 64);
+            au[g].y += __shfl_xor(au[g].y, off, 64);
+        }
+    }
+    if (lane < kDot2Group && i0 + lane < jc) {
+        double2 vw = aw[0], vu = au[0];
+#pragma unroll
+        for (int g = 1; g < kDot2Group; ++g)
+            if (lane == g) {
+                vw = aw[g];
+                vu = au[g];
+            }
+        pw[((size_t)s * ldh + i0 + lane) * nch + ch] = vw;
+        pu[((size_t)s * ldh + i0 + lane) * nch + ch] = vu;
+    }
+}
+
+// Per slot (one wavefront), step j = jcount[s]: from b (hb, gamma at j) and a
+// (ha, beta at j) -> nu, the finished column j-1 (into Hu, and gcol / ghn for
+// the Givens step), the pending column j (pend) and the update coefficients
+// (acoef: a, fcoef: f).  nu_out = 0 flags a candidate inside span(Q)
+// (breakdown: the Krylov space is invariant).
+__global__ __launch_bounds__(64) void k_dcgs_coef(const int *__restrict__ slots, const int *__restrict__ jcount,
+                                                  const double2 *__restrict__ hb, const double2 *__restrict__ ha,
+                                                  int ldh, double2 *__restrict__ pend, double2 *__restrict__ Hu,
+                                                  int ldH, size_t ldH_slot, double2 *__restrict__ gcol,
+                                                  double *__restrict__ ghn, double2 *__restrict__ acoef,
+                                                  double2 *__restrict__ fcoef, double *__restrict__ nu_out) {
+    const int s = slots[blockIdx.x];
+    const int j = jcount[s];
+    const int t = threadIdx.x;
+    const double2 *bs = hb + (size_t)s * ldh;
+    const double2 *as = ha + (size_t)s * ldh;
+    double2 *ps = pend + (size_t)s * ldh;
+    double2 *Hs = Hu + ldH_slot * s;
+    // |a|^2 and a^H b over i < j
+    double aa = 0.0, abr = 0.0, abi = 0.0;
+    for (int i = t; i < j; i += 64) {
+        const double2 ai = as[i], bi = bs[i];
+        aa += ai.x * ai.x + ai.y * ai.y;
+        abr += ai.x * bi.x + ai.y * bi.y;
+        abi += ai.x * bi.y - ai.y * bi.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        aa += __shfl_xor(aa, off, 64);
+        abr += __shfl_xor(abr, off, 64);
+        abi += __shfl_xor(abi, off, 64);
+    }
+    const double beta = as[j].x;
+    const double nu2 = beta - aa;
+    const bool bd = !(nu2 > 1e-24 * beta);
+    const double nu = bd ? 1.0 : sqrt(nu2);
+    if (t == 0) nu_out[s] = bd ? 0.0 : nu;
+    // column j-1 becomes final
+    if (j >= 1) {
+        const double eta = ps[j].x;
+        for (int i = t; i < j; i += 64) {
+            const double2 v = make_double2(ps[i].x + eta * as[i].x, ps[i].y + eta * as[i].y);
+            Hs[(size_t)ldH * (j - 1) + i] = v;
+            gcol[(size_t)s * ldh + i] = v;
+        }
+        if (t == 0) {
+            const double sub = bd ? 0.0 : eta * nu;
+            Hs[(size_t)ldH * (j - 1) + j] = make_double2(sub, 0.0);
+            ghn[s] = sub;
+        }
+    }
+    __syncthreads();
+    // e = H a over the final columns 0..j-1 (column k has rows 0..k+1);
+    // f; pending column j: c = (f - e) / nu
+    const double2 gam = bs[j];
+    const double2 fj = make_double2((gam.x - abr) / nu, (gam.y - abi) / nu);
+    for (int i = t; i <= j; i += 64) {
+        double2 e = make_double2(0, 0);
+        for (int k = (i > 0 ? i - 1 : 0); k < j; ++k) {
+            const double2 h = Hs[(size_t)ldH * k + i];
+            const double2 p = cmul(h, as[k]);
+            e.x += p.x;
+            e.y += p.y;
+        }
+        const double2 f = i < j ? bs[i] : fj;
+        ps[i] = make_double2((f.x - e.x) / nu, (f.y - e.y) / nu);
+        fcoef[(size_t)s * ldh + i] = f;
+        if (i < j) acoef[(size_t)s * ldh + i] = as[i];
+    }
+}
+
+// q_j = (u_j - Q a) / nu into the basis, W <- (W - Q f_{0..j-1} - q_j f_j) / nu,
+// with per-workgroup partial sums of ||W||^2.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dcgs_update(int nv, const int *__restrict__ slots,
+                                                     const int *__restrict__ jcount, VT *__restrict__ V,
+                                                     size_t ldv_slot, const double2 *__restrict__ acoef,
+                                                     const double2 *__restrict__ fcoef, int ldh,
+                                                     const double *__restrict__ nu_in, double2 *__restrict__ W,
+                                                     double *__restrict__ npart) {
+    __shared__ double sq[4];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    const int j = jcount[s];
+    const double nu = nu_in[s] > 0.0 ? nu_in[s] : 1.0;
+    double q2 = 0.0;
+    if (r < nv) {
+        VT *vs = V + ldv_slot * s + r;
+        const double2 *as = acoef + (size_t)s * ldh;
+        const double2 *fs = fcoef + (size_t)s * ldh;
+        double2 qa = make_double2(0, 0), qf = make_double2(0, 0);
+        for (int i = 0; i < j; ++i) {
+            const double2 v = to_d2(vs[(size_t)nv * i]);
+            const double2 p = cmul(v, as[i]), q = cmul(v, fs[i]);
+            qa.x += p.x;
+            qa.y += p.y;
+            qf.x += q.x;
+            qf.y += q.y;
+        }
+        const double2 u = to_d2(vs[(size_t)nv * j]);
+        const double2 qj = make_double2((u.x - qa.x) / nu, (u.y - qa.y) / nu);
+        vs[(size_t)nv * j] = from_d2<VT>(qj);
+        const double2 fj = fs[j];
+        const double2 qfj = cmul(qj, fj);
+        double2 &wv = W[(size_t)nv * s + r];
+        wv = make_double2((wv.x - qf.x - qfj.x) / nu, (wv.y - qf.y - qfj.y) / nu);
+        q2 = wv.x * wv.x + wv.y * wv.y;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q2 += __shfl_xor(q2, off, 64);
+    if (threadIdx.x % 64 == 0) sq[threadIdx.x / 64] = q2;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[(size_t)s * gridDim.x + blockIdx.x] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+}
+
+// pend[s][jcount[s] + 1] = ||w|| (the pending column's subdiagonal).
+__global__ void k_dcgs_sub(const int *__restrict__ slots, const int *__restrict__ jcount,
+                           const double *__restrict__ nrm, double2 *__restrict__ pend, int ldh) {
+    const int s = slots[blockIdx.x];
+    if (threadIdx.x == 0) pend[(size_t)s * ldh + jcount[s] + 1] = make_double2(nrm[s], 0.0);
+}
+
+// Closing a cycle after J = jcount[s] final vectors: the pending column J-1
+// made final against the last candidate (W = ||w|| u_J still): with
+// d = Q^H W (hd), column J-1 += d and its subdiagonal = sqrt(||W||^2 - |d|^2).
+__global__ __launch_bounds__(64) void k_dcgs_close(const int *__restrict__ slots, const int *__restrict__ jcount,
+                                                   const double2 *__restrict__ hd, const double2 *__restrict__ pend,
+                                                   int ldh, double2 *__restrict__ gcol, double *__restrict__ ghn) {
+    const int s = slots[blockIdx.x];
+    const int J = jcount[s];
+    const double2 *ps = pend + (size_t)s * ldh;
+    const double2 *ds = hd + (size_t)s * ldh;
+    double dd = 0.0;
+    for (int i = threadIdx.x; i < J; i += 64) {
+        const double2 d = ds[i];
+        dd += d.x * d.x + d.y * d.y;
+        gcol[(size_t)s * ldh + i] = make_double2(ps[i].x + d.x, ps[i].y + d.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) dd += __shfl_xor(dd, off, 64);
+    if (threadIdx.x == 0) {
+        const double eta = ps[J].x;
+        ghn[s] = sqrt(fmax(eta * eta - dd, 0.0));
+    }
+}
+
 // ------------------------------------------------- projected starting iterate
 // For a homogeneous atomistic target A(omega) = M + S(omega) with S diagonal
 // (q rows -z_q(t_i), dipole rows Lambda_i(omega), the frequency-dependent
@@ -1120,7 +1353,8 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     S.slots_d.alloc(std::max<size_t>(1, (size_t)B));
     S.ring.init(std::max({sizeof(SysParam), sizeof(int) * (size_t)B, sizeof(long long) * (kProjMax + 1) * (size_t)B}),
                 4 * B + 64);
-    PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * (size_t)B, hipHostMallocDefault));
+    PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * 2 * (size_t)B,
+                          hipHostMallocDefault));
 
     // projected starting iterates (atomistic targets, cold start): the
     // systems are admitted coarse to fine over the frequency grid, so that
@@ -1155,6 +1389,25 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         proj_adm.alloc(B);
         have.assign((size_t)3 * n_freq_all, 0);
     }
+    // delayed re-orthogonalisation (DCGS2) on the batched path
+    const bool dcgs = !plain && c->opt.dcgs2;
+    DevBuf<double2> dc_part_u, dc_a, dc_pend, dc_Hu, dc_acoef, dc_fcoef, dc_gcol;
+    DevBuf<double> dc_nu, dc_ghn;
+    DevBuf<int> dc_jm1, dc_slots2;
+    if (dcgs) {
+        dc_part_u.alloc((size_t)B * S.ldh * nch);
+        dc_a.alloc((size_t)B * S.ldh);
+        dc_pend.alloc((size_t)B * S.ldh);
+        dc_Hu.alloc(S.ldH_slot * B);
+        dc_acoef.alloc((size_t)B * S.ldh);
+        dc_fcoef.alloc((size_t)B * S.ldh);
+        dc_gcol.alloc((size_t)B * S.ldh);
+        dc_nu.alloc(B);
+        dc_ghn.alloc(B);
+        dc_jm1.alloc(B);
+        dc_slots2.alloc(B);
+    }
+    std::vector<int> h_jm1(B, 0);
     std::vector<int> adm_proj;                 // slots admitted this step with a projected start
     std::vector<long long> h_info;
 
@@ -1364,6 +1617,98 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         for (int s : act) S.h_jp1[s] = S.jcount[s] + 1;
         upload(S, S.jp1_d.p, S.h_jp1.data(), (size_t)B);
         int *jp1_dp = S.jp1_d.p;
+        if (dcgs) {
+            const int nb = (nv + 255) / 256;
+            const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDot2Group - 1) / kDot2Group);
+            const dim3 gs((jmax + 1 + 3) / 4, na);
+            // one pass of dots over q_0..q_{j-1}, u_j against w and u_j
+            k_dots2_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
+                                                 dc_part_u.p, S.ldh, nch);
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hbuf.p);
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, dc_part_u.p, S.ldh, nch, dc_a.p);
+            k_dcgs_coef<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hbuf.p, dc_a.p, S.ldh, dc_pend.p, dc_Hu.p,
+                                           S.ldH, S.ldH_slot, dc_gcol.p, dc_ghn.p, dc_acoef.p, dc_fcoef.p,
+                                           dc_nu.p);
+            // one pass of updates: q_j into the basis and the new w
+            k_dcgs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot,
+                                                            dc_acoef.p, dc_fcoef.p, S.ldh, dc_nu.p, S.W.p,
+                                                            S.npart.p);
+            k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.npart.p, nb, nullptr, 1, S.nrm.p);
+            k_dcgs_sub<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.nrm.p, dc_pend.p, S.ldh);
+            k_scale_into<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p,
+                                                           V.p, S.ldv_slot);
+            // Givens step on the column that became final (j - 1)
+            std::vector<int> gl;
+            for (int s : act)
+                if (S.jcount[s] >= 1) {
+                    gl.push_back(s);
+                    h_jm1[s] = S.jcount[s] - 1;
+                }
+            const size_t lds_g = sizeof(double2) * (S.ldH + S.ldg) + sizeof(double) * S.ldg;
+            if (!gl.empty()) {
+                upload(S, dc_slots2.p, gl.data(), gl.size());
+                upload(S, dc_jm1.p, h_jm1.data(), (size_t)B);
+                k_givens<<<(int)gl.size(), 64, lds_g, st>>>(dc_slots2.p, dc_jm1.p, dc_gcol.p, S.ldh, dc_ghn.p, S.H.p,
+                                                           S.ldH, S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg,
+                                                           S.resid.p);
+            }
+            PXH_HIP(hipGetLastError());
+            PXH_HIP(hipMemcpyAsync(S.pin_res, S.resid.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipMemcpyAsync(S.pin_res + B, dc_nu.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipStreamSynchronize(st));
+            std::vector<int> fin, close;
+            for (int s : act) {
+                const int j = S.jcount[s];
+                const double res = S.pin_res[s], nu = S.pin_res[B + s];
+                total_it[s] += 1;
+                if (nu == 0.0) {
+                    // the candidate lies in span(Q): q_0..q_{j-1} solve the cycle exactly
+                    S.h_jl[s] = j;
+                    fin.push_back(s);
+                    continue;
+                }
+                S.jcount[s] = j + 1;
+                if ((j >= 1 && res < target[s]) || S.jcount[s] >= m) {
+                    S.h_jl[s] = S.jcount[s];
+                    fin.push_back(s);
+                    close.push_back(s);
+                }
+            }
+            if (!close.empty()) {
+                // the pending column of each closing slot made final against
+                // its last candidate, then its Givens step
+                const int nc = (int)close.size();
+                int jcl = 0;
+                for (int s : close) {
+                    jcl = std::max(jcl, S.jcount[s]);
+                    h_jm1[s] = S.jcount[s] - 1;
+                }
+                upload_list(S, close);
+                upload(S, S.jcount_d.p, S.jcount.data(), (size_t)B);
+                upload(S, dc_jm1.p, h_jm1.data(), (size_t)B);
+                const dim3 gd2((nch + 3) / 4, nc, (jcl + kDotGroup - 1) / kDotGroup);
+                const dim3 gs2((jcl + 3) / 4, nc);
+                k_dots_part<VT><<<gd2, 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot, S.W.p,
+                                                     S.dpart.p, S.ldh, nch, nullptr);
+                k_dots_sum<<<gs2, 256, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.dpart.p, S.ldh, nch, S.hacc.p);
+                k_dcgs_close<<<nc, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hacc.p, dc_pend.p, S.ldh, dc_gcol.p,
+                                                dc_ghn.p);
+                k_givens<<<nc, 64, lds_g, st>>>(S.slots_d.p, dc_jm1.p, dc_gcol.p, S.ldh, dc_ghn.p, S.H.p, S.ldH,
+                                               S.ldH_slot, S.g.p, S.ldg, S.cs.p, S.sn.p, S.ldg, S.resid.p);
+            }
+            if (!fin.empty()) {
+                upload_list(S, fin);
+                upload(S, S.jl_d.p, S.h_jl.data(), (size_t)B);
+                k_trsv<<<(int)fin.size(), 64, 0, st>>>(S.slots_d.p, S.jl_d.p, S.H.p, S.ldH, S.ldH_slot, S.g.p,
+                                                       S.ldg, S.y.p, S.ldh);
+                k_update_x<VT><<<grid_rows((int)fin.size()), 256, 0, st>>>(
+                    nv, S.slots_d.p, S.jl_d.p, V.p, S.ldv_slot, S.y.p, S.ldh, S.precond ? S.dinv.p : nullptr,
+                    S.X.p, 1);
+                PXH_HIP(hipGetLastError());
+                for (int s : fin) phase[s] = NEED_RES;
+            }
+            continue;
+        }
         const bool mgs = plain && (jmax + 1 < 25);
         if constexpr (std::is_same<VT, double2>::value) {
             if (mgs) {
