@@ -769,6 +769,8 @@ void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
     else if (c->atoms.model == PXH_WFQ && nsys >= 8)  // charges only: room for 8 systems per pass
         apply_atoms_impl<double, 8>(c, nsys, d_sys_list, d_par, x, y, ld);
+    else if (c->atoms.model == PXH_WFQFMU && nsys >= 4)  // 4 systems share each pair block (2 waves / SIMD)
+        apply_atoms_impl<double, 4>(c, nsys, d_sys_list, d_par, x, y, ld);
     else
         apply_atoms_impl<double, 2>(c, nsys, d_sys_list, d_par, x, y, ld);
 }
