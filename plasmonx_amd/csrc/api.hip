@@ -349,6 +349,12 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
         } else if (k == "x0_projection") {
             PXH_CHECK(value >= 0 && value <= 16, PXH_EINVAL, "x0_projection must be 0..16");
             c->opt.x0_projection = (int)value;
+        } else if (k == "res_batch") {
+            PXH_CHECK(value >= 1, PXH_EINVAL, "res_batch must be >= 1");
+            c->opt.res_batch = (int)value;
+        } else if (k == "res_wait") {
+            PXH_CHECK(value >= 0, PXH_EINVAL, "res_wait must be >= 0");
+            c->opt.res_wait = (int)value;
         } else if (k == "dcgs2") {
             c->opt.dcgs2 = value != 0.0;
         } else if (k == "initial_guess") {

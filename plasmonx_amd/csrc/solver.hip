@@ -1413,7 +1413,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
 
     enum Phase { FREE, INNER, NEED_RES };
     std::vector<int> phase(B, FREE), sys_of(B, -1), outer(B, 0), res_wait(B, 0);
-    constexpr int kResBatch = 4, kResWait = 2;
+    const int kResBatch = std::max(1, c->opt.res_batch), kResWait = std::max(0, c->opt.res_wait);
     std::vector<double> beta(B, 0.0), target(B, 0.0);
     std::vector<int> total_it(B, 0);
     size_t next = 0, done = 0;
@@ -1505,7 +1505,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
             PXH_HIP(hipGetLastError());
         }
         // ----- residual checks (FP64).  The slots whose x needs a product
-        // wait (idle, at most kResWait steps) until kResBatch of them can
+        // wait (idle, at most res_wait steps) until res_batch of them can
         // share one FP64 product, or no slot is left in an inner cycle; an
         // idle slot costs the low-precision products nothing
         std::vector<int> need, prod;
