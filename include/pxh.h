@@ -161,7 +161,11 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * starts each system from the best combination of that many converged
  * neighbouring frequencies; 0 = x0 = 0), "initial_guess" (1: the iterative
  * methods start from the caller's `variables`; default 0, x0 = 0 as the
- * reference). */
+ * reference), "dcgs2" (1: the batched solver's Gram-Schmidt delays the
+ * second pass of each vector to the next step), "res_batch" / "res_wait"
+ * (FP64 residual products wait for 8 systems, at most 5 steps), "autotune"
+ * (1: the first full low-precision pass of a new atom count times candidate
+ * source splits and keeps the fastest for the process). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
 /* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector

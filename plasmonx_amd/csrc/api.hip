@@ -355,6 +355,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
         } else if (k == "res_wait") {
             PXH_CHECK(value >= 0, PXH_EINVAL, "res_wait must be >= 0");
             c->opt.res_wait = (int)value;
+        } else if (k == "autotune") {
+            c->opt.autotune = value != 0.0;
         } else if (k == "dcgs2") {
             c->opt.dcgs2 = value != 0.0;
         } else if (k == "initial_guess") {

@@ -28,7 +28,11 @@
 // 3 MFMAs.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include <algorithm>
 #include <cmath>
@@ -332,6 +336,62 @@ float mfma_t_scale(const Atoms &A) {
     return std::ldexp(1.f, e);
 }
 
+namespace {
+
+// Source splits chosen by timing, per device and atom count, shared by every
+// context of the process (so that one target always sums its partials the
+// same way).
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, int> g_tuned_splits;
+
+template <int K>
+void launch_sweep(pxh_ctx *c, int iblocks, int splits, int per, float rfar2, float tsc, int ns, int nsys, int goff) {
+    const Atoms &A = c->atoms;
+    k_tsweep_mfma<K><<<iblocks * splits, 256, 0, c->stream>>>(
+        A.n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p, c->work_xl.p,
+        c->work_part.p, A.n_var(), splits, per, nsys, goff);
+}
+
+}  // namespace
+
+void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
+                    int ld, int splits, bool fp32_class);
+
+namespace {
+
+// Time one 48-system product for each candidate split count (the pass over
+// the prepared first group plus the reduction of the split partials) and
+// keep the fastest.
+template <int K>
+int tune_splits(pxh_ctx *c, const std::vector<int> &cand, int iblocks, int jsteps, float rfar2, float tsc, int ns,
+                int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y, int ld) {
+    hipEvent_t e0, e1;
+    PXH_HIP(hipEventCreate(&e0));
+    PXH_HIP(hipEventCreate(&e1));
+    int best = cand[0];
+    float best_ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep)  // the first round warms the caches
+        for (int sp : cand) {
+            const int per = (jsteps + sp - 1) / sp;
+            PXH_HIP(hipEventRecord(e0, c->stream));
+            launch_sweep<K>(c, iblocks, sp, per, rfar2, tsc, ns, nsys, 0);
+            finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sp, true);
+            PXH_HIP(hipEventRecord(e1, c->stream));
+            PXH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PXH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep == 1 && ms < best_ms) {
+                best_ms = ms;
+                best = sp;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace
+
 // MFMA path of the FP32-class product for omegaFQFmu (Gaussian kernel).
 void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld) {
@@ -351,15 +411,43 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         splits = std::max(1, std::min(splits, jsteps / 16));
     }
     splits = std::min(splits, jsteps);
-    const int per = (jsteps + splits - 1) / splits;
-    c->work_part.alloc((size_t)splits * nsys * nv);
-    c->work_t.alloc((size_t)nsys * nv);
     const size_t xelems = (size_t)jsteps * 4 * NCB * 64 * 8;
     c->work_xh.alloc(xelems);
     c->work_xl.alloc(xelems);
     c->work_scale.alloc(32 * NCB);
+    c->work_t.alloc((size_t)nsys * nv);
     const float tsc = mfma_t_scale(A);
     const float rfar2 = (float)far_field_r2(A, false);
+    if (c->opt.splits <= 0 && c->opt.autotune && nsys >= SYS_PER_GROUP && jsteps >= 8 * 16) {
+        // autotuner: the heuristic, half and twice it, and 8..32, on the
+        // first full pass that this atom count meets in the process
+        const std::pair<int, int> key{c->device, n};
+        int tuned = 0;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned_splits.find(key);
+            if (it != g_tuned_splits.end()) tuned = it->second;
+        }
+        if (!tuned) {
+            std::vector<int> cand{splits};
+            for (int sp : {splits / 2 / 8 * 8, splits * 2, 8, 16, 24, 32})
+                if (sp >= 8 && sp <= jsteps / 16 && std::find(cand.begin(), cand.end(), sp) == cand.end())
+                    cand.push_back(sp);
+            c->work_part.alloc((size_t)(*std::max_element(cand.begin(), cand.end())) * nsys * nv);
+            k_col_scale<<<32 * NCB, 256, 0, c->stream>>>(nv, SYS_PER_GROUP, d_sys_list, x, ld, c->work_scale.p,
+                                                         32 * NCB);
+            k_prep_x<NCB><<<jsteps, 256, 0, c->stream>>>(n, SYS_PER_GROUP, d_sys_list, x, ld, c->work_scale.p,
+                                                         c->work_xh.p, c->work_xl.p);
+            tuned = tune_splits<NCB>(c, cand, iblocks, jsteps, rfar2, tsc, SYS_PER_GROUP, nsys, d_sys_list, d_par, x,
+                                     y, ld);
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned_splits.emplace(key, tuned);
+            tuned = g_tuned_splits[key];
+        }
+        splits = tuned;
+    }
+    const int per = (jsteps + splits - 1) / splits;
+    c->work_part.alloc((size_t)splits * nsys * nv);
     const int t_product = timer_begin(c);
     for (int g = 0; g < groups; ++g) {
         const int ns = std::min(SYS_PER_GROUP, nsys - g * SYS_PER_GROUP);
@@ -373,9 +461,7 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
                                                        c->work_xl.p);
             // the pair kernel alone, timed on the library's stream
             const int t_kernel = timer_begin(c);
-            k_tsweep_mfma<K><<<iblocks * splits, 256, 0, c->stream>>>(
-                n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p,
-                c->work_xl.p, c->work_part.p, nv, splits, per, nsys, g * SYS_PER_GROUP);
+            launch_sweep<K>(c, iblocks, splits, per, rfar2, tsc, ns, nsys, g * SYS_PER_GROUP);
             timer_end(c, t_kernel, 12);
         };
         const int ncb = (ns + 15) / 16;

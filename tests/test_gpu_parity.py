@@ -259,6 +259,33 @@ def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
         assert err < 2e-5, (k, err)
 
 
+def test_autotuned_mfma_product(gpu_ctx):
+    """3.4k-atom Ag sphere, 48 vectors: the first full MFMA pass of this atom
+    count times the candidate source splits and keeps the fastest; the
+    product with the tuned splits and with the fixed heuristic both stay
+    within the low-precision bar (2e-5 per vector) of the FP64 product."""
+    from plasmonx_amd.workloads import c2_silver_sphere
+    s, freq = c2_silver_sphere(radius=24.0, n_freq=2)
+    assert s.n_atoms >= 2048
+    gpu_ctx.set_option("kbar_cutoff", 1)
+    gpu_ctx.set_atoms(s)
+    gpu_ctx.set_alpha(s.alpha_table(freq))
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(48, s.n_var)) + 1j * rng.normal(size=(48, s.n_var))
+    ref = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("matvec_mixed", 1)
+    outs = []
+    for tune in (1, 0, 1):
+        gpu_ctx.set_option("autotune", tune)
+        outs.append(gpu_ctx.matvec(freq[1], 1, x))
+    gpu_ctx.set_option("autotune", 1)
+    gpu_ctx.set_option("matvec_mixed", 0)
+    for got in outs:
+        for k in range(48):
+            assert relerr(got[k], ref[k]) < 2e-5, k
+    assert np.array_equal(outs[0], outs[2])  # the tuned choice is kept
+
+
 @pytest.mark.parametrize("case", ["bem_ief_accurate_inversion", "bem_ief_approximate_inversion"])
 def test_bem_ief_iterative_agrees_with_inversion(gpu_ctx, case):
     """The iterative IEF applies the full operator -(D + lambda I) S A^-1, so
