@@ -54,33 +54,53 @@ constexpr int kAtomsI = 32;        // target atoms per workgroup
 constexpr int kAtomsJ = 16;        // source atoms per step (K = 64)
 constexpr int kStepsPerFlush = 8;  // source steps between FP32 -> FP64 flushes
 
-// Per-column scale 2^e with max|x_col| * 2^e in [2^(kXTarget-1), 2^kXTarget).
-__global__ __launch_bounds__(256) void k_col_scale(int nv, int nsys, const int *__restrict__ sys_list,
-                                                   const double2 *__restrict__ x, int ld, float *__restrict__ scale,
-                                                   int ncols) {
-    const int c = blockIdx.x;  // column index 0..ncols-1
-    const int k = c / 2;
-    __shared__ double sh[256];
-    double m = 0.0;
-    if (k < nsys) {
-        const double2 *xs = x + (size_t)sys_list[k] * ld;
-        for (int r = threadIdx.x; r < nv; r += 256) m = fmax(m, fabs((c & 1) ? xs[r].y : xs[r].x));
+// Per-column scale 2^e with max|x_col| * 2^e in [2^(kXTarget-1), 2^kXTarget),
+// in two stages: max |re| and max |im| of each system over chunks of
+// kScaleRows rows (one workgroup per chunk and system), then the scales.
+constexpr int kScaleRows = 4096;
+
+__global__ __launch_bounds__(256) void k_col_max(int nv, const int *__restrict__ sys_list,
+                                                 const double2 *__restrict__ x, int ld, double *__restrict__ part) {
+    __shared__ double sre[4], sim[4];
+    const int ch = blockIdx.x, k = blockIdx.y;
+    const double2 *xs = x + (size_t)sys_list[k] * ld;
+    double mre = 0.0, mim = 0.0;
+    const int r1 = min(nv, (ch + 1) * kScaleRows);
+    for (int r = ch * kScaleRows + threadIdx.x; r < r1; r += 256) {
+        const double2 v = xs[r];
+        mre = fmax(mre, fabs(v.x));
+        mim = fmax(mim, fabs(v.y));
     }
-    sh[threadIdx.x] = m;
+    for (int off = 32; off > 0; off >>= 1) {
+        mre = fmax(mre, __shfl_xor(mre, off, 64));
+        mim = fmax(mim, __shfl_xor(mim, off, 64));
+    }
+    if (threadIdx.x % 64 == 0) {
+        sre[threadIdx.x / 64] = mre;
+        sim[threadIdx.x / 64] = mim;
+    }
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        float s = 0.f;
-        if (sh[0] > 0.0) {
-            int e;
-            frexp(sh[0], &e);  // max = f 2^e, f in [0.5, 1)
-            s = ldexpf(1.f, kXTarget - e);
-        }
-        scale[c] = s;
+        part[((size_t)k * gridDim.x + ch) * 2] = fmax(fmax(sre[0], sre[1]), fmax(sre[2], sre[3]));
+        part[((size_t)k * gridDim.x + ch) * 2 + 1] = fmax(fmax(sim[0], sim[1]), fmax(sim[2], sim[3]));
     }
+}
+
+__global__ void k_col_scale(int nsys, int nchunks, const double *__restrict__ part, float *__restrict__ scale,
+                            int ncols) {
+    const int c = threadIdx.x;  // column: re / im of system c / 2
+    if (c >= ncols) return;
+    const int k = c / 2;
+    double m = 0.0;
+    if (k < nsys)
+        for (int ch = 0; ch < nchunks; ++ch) m = fmax(m, part[((size_t)k * nchunks + ch) * 2 + (c & 1)]);
+    float s = 0.f;
+    if (m > 0.0) {
+        int e;
+        frexp(m, &e);  // max = f 2^e, f in [0.5, 1)
+        s = ldexpf(1.f, kXTarget - e);
+    }
+    scale[c] = s;
 }
 
 // X in fp16 hi / lo, laid out as the MFMA B fragments of every source step:
@@ -338,6 +358,13 @@ float mfma_t_scale(const Atoms &A) {
 
 namespace {
 
+void col_scales(pxh_ctx *c, int nv, int ns, const int *lst, const double2 *x, int ld, int ncols) {
+    const int nchunks = (nv + kScaleRows - 1) / kScaleRows;
+    c->work_colmax.alloc((size_t)2 * nchunks * std::max(ns, 1));
+    k_col_max<<<dim3(nchunks, ns), 256, 0, c->stream>>>(nv, lst, x, ld, c->work_colmax.p);
+    k_col_scale<<<1, 256, 0, c->stream>>>(ns, nchunks, c->work_colmax.p, c->work_scale.p, ncols);
+}
+
 // Source splits chosen by timing, per device and atom count, shared by every
 // context of the process (so that one target always sums its partials the
 // same way).
@@ -434,8 +461,7 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
                 if (sp >= 8 && sp <= jsteps / 16 && std::find(cand.begin(), cand.end(), sp) == cand.end())
                     cand.push_back(sp);
             c->work_part.alloc((size_t)(*std::max_element(cand.begin(), cand.end())) * nsys * nv);
-            k_col_scale<<<32 * NCB, 256, 0, c->stream>>>(nv, SYS_PER_GROUP, d_sys_list, x, ld, c->work_scale.p,
-                                                         32 * NCB);
+            col_scales(c, nv, SYS_PER_GROUP, d_sys_list, x, ld, 32 * NCB);
             k_prep_x<NCB><<<jsteps, 256, 0, c->stream>>>(n, SYS_PER_GROUP, d_sys_list, x, ld, c->work_scale.p,
                                                          c->work_xh.p, c->work_xl.p);
             tuned = tune_splits<NCB>(c, cand, iblocks, jsteps, rfar2, tsc, SYS_PER_GROUP, nsys, d_sys_list, d_par, x,
@@ -456,7 +482,7 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
         // blocks it occupies (16 systems each)
         auto pass = [&](auto ncb_tag) {
             constexpr int K = decltype(ncb_tag)::value;
-            k_col_scale<<<32 * K, 256, 0, c->stream>>>(nv, ns, lst, x, ld, c->work_scale.p, 32 * K);
+            col_scales(c, nv, ns, lst, x, ld, 32 * K);
             k_prep_x<K><<<jsteps, 256, 0, c->stream>>>(n, ns, lst, x, ld, c->work_scale.p, c->work_xh.p,
                                                        c->work_xl.p);
             // the pair kernel alone, timed on the library's stream

@@ -173,6 +173,7 @@ struct pxh_ctx {
     pxh::DevBuf<double2> work_part, work_t;  // scratch of the operator products
     pxh::DevBuf<_Float16> work_xh, work_xl;   // fp16 hi / lo operand of the MFMA products
     pxh::DevBuf<float> work_scale;            // per-column scales of that operand
+    pxh::DevBuf<double> work_colmax;          // chunk maxima behind those scales
     pxh::DevBuf<double> work_pack;            // packed operand of the BEM matrix products
     pxh::DevBuf<double> work_gemm;            // split-K partial sums of those products
     bool use_stored = false;                  // in-memory method: products use the stored T

@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_ms_finish(int nv, int nsh, int f, int p
         if (lane == 0) red[wv][k] = q2;
     }
     __syncthreads();
-    if (threadIdx.x < ns)
+    if ((int)threadIdx.x < ns)
         part[(size_t)blockIdx.x * nsh + s0 + threadIdx.x] =
             (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
@@ -1852,6 +1852,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
                           const std::vector<double2> &sigma, const double *freq, const double2 *d_rhs_all,
                           int rhs_fstride, int rhs_stride, double tol, int kdim, double2 *d_sol, std::vector<int> &iters_out,
                           std::vector<double> &tres_out, std::vector<double> &t_done, std::vector<int> &unsolved) {
+    PXH_CHECK(rhs_fstride == 0, PXH_EINVAL, "multi-shift GMRES needs one right-hand side for all frequencies");
     hipStream_t st = c->stream;
     const auto t_start = std::chrono::steady_clock::now();
     auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
