@@ -459,14 +459,16 @@ void dense_t(pxh_ctx *c, DevBuf<double> &T) {
     dense_t_atoms(c, T.p, false);
 }
 
-// Frequencies factorised together: at most kMaxFreqChunk, within half of
-// the free HBM and within the context's budget.
+// Frequencies factorised together: at most kMaxFreqChunk and
+// kDenseChunkBytes of matrices, within half of the free HBM and within the
+// context's budget.
 size_t freq_chunk(pxh_ctx *c, int n, int n_freq) {
     size_t free_b = 0, total_b = 0;
     PXH_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t per = (size_t)n * n * sizeof(double2) * 2 + 1;
     size_t k = std::max<size_t>(1, (free_b / 2) / per);
     if (c->budget) k = std::max<size_t>(1, std::min(k, c->budget / per));
+    k = std::min(k, std::max<size_t>(1, kDenseChunkBytes / ((size_t)n * n * sizeof(double2))));
     return std::min<size_t>(std::min<size_t>(k, kMaxFreqChunk), (size_t)n_freq);
 }
 

@@ -33,7 +33,8 @@ struct Error {
 // Maximum number of atom types handled by the pair kernels.
 constexpr int kMaxTypes = 8;
 // Frequencies whose dense matrices are factorised together (inversion).
-constexpr size_t kMaxFreqChunk = 16;
+constexpr size_t kMaxFreqChunk = 64;                  // dense path: frequencies factorised together ...
+constexpr size_t kDenseChunkBytes = size_t(8) << 30;  // ... within this many bytes of matrices
 
 // Per type-pair constants of the interaction kernels (index ti + nt*tj).
 struct PairConst {

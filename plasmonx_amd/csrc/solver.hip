@@ -2386,7 +2386,8 @@ size_t estimate_solve_bytes(const pxh_ctx *c, int n_freq, int pol_mask, int meth
     const size_t nsys = (size_t)n_freq * (size_t)__builtin_popcount(pol_mask & 7);
     size_t b = 2 * nv * 3 * (size_t)n_freq * sizeof(double2);  // right-hand sides and solutions
     if (method == PXH_INVERSION) {
-        const size_t k = std::min<size_t>(n_freq, kMaxFreqChunk);
+        const size_t k = std::min<size_t>(std::min<size_t>(n_freq, kMaxFreqChunk),
+                                          std::max<size_t>(1, kDenseChunkBytes / (nv * nv * sizeof(double2))));
         b += k * nv * nv * 2 * sizeof(double2) + 2 * nv * nv * sizeof(double);
     } else {
         // omegaFQ (charges only): the FP64 pair kernel issues at the FP32 rate
