@@ -31,29 +31,34 @@ __device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
 
 // Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
 // frequency.  Pivots (global row indices) go to piv[f][k..k+nb).  Row swaps
-// are applied inside the panel only.
-__global__ __launch_bounds__(256) void k_panel(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f,
-                                               int *__restrict__ piv, int *__restrict__ info) {
+// are applied inside the panel only.  One pass over the rows per column:
+// scale column c, update the remaining panel columns with row c (staged in
+// LDS), and find the pivot candidates of column c + 1 on the way.
+constexpr int kPanelThreads = 512;
+__global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, double2 *__restrict__ A,
+                                                         size_t lda_f, int *__restrict__ piv,
+                                                         int *__restrict__ info) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
-    __shared__ double sval[256];
-    __shared__ int sidx[256];
+    __shared__ double sval[kPanelThreads];
+    __shared__ int sidx[kPanelThreads];
+    __shared__ double2 urow[NB];
     const int tid = threadIdx.x;
-    for (int c = k; c < k + nb; ++c) {
-        // pivot search in column c, rows c..n-1 (first maximum, as IZAMAX)
-        double best = -1.0;
-        int bi = c;
-        for (int r = c + tid; r < n; r += 256) {
-            const double v = cabs2(Af[r + (size_t)n * c]);
-            if (v > best) {
-                best = v;
-                bi = r;
-            }
+    // pivot candidates of column k (first maximum, as IZAMAX)
+    double best = -1.0;
+    int bi = k;
+    for (int r = k + tid; r < n; r += kPanelThreads) {
+        const double v = cabs2(Af[r + (size_t)n * k]);
+        if (v > best) {
+            best = v;
+            bi = r;
         }
+    }
+    for (int c = k; c < k + nb; ++c) {
         sval[tid] = best;
         sidx[tid] = bi;
         __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
+        for (int off = kPanelThreads / 2; off > 0; off >>= 1) {
             if (tid < off) {
                 if (sval[tid + off] > sval[tid] || (sval[tid + off] == sval[tid] && sidx[tid + off] < sidx[tid])) {
                     sval[tid] = sval[tid + off];
@@ -69,26 +74,47 @@ __global__ __launch_bounds__(256) void k_panel(int n, int k, int nb, double2 *__
             piv[(size_t)n * f + c] = p;
             if (pv == 0.0 && info[f] == 0) info[f] = c + 1;
         }
-        if (pv == 0.0) continue;
-        // swap rows c and p inside the panel
+        best = -1.0;
+        bi = c + 1;
+        if (pv == 0.0) {
+            // singular column: nothing to eliminate; candidates of the next
+            if (c + 1 < k + nb)
+                for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
+                    const double v = cabs2(Af[r + (size_t)n * (c + 1)]);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
+            continue;
+        }
+        // swap rows c and p inside the panel, row c of the remaining columns to LDS
         if (p != c)
-            for (int cc = k + tid; cc < k + nb; cc += 256) {
+            for (int cc = k + tid; cc < k + nb; cc += kPanelThreads) {
                 const double2 t = Af[c + (size_t)n * cc];
                 Af[c + (size_t)n * cc] = Af[p + (size_t)n * cc];
                 Af[p + (size_t)n * cc] = t;
             }
         __syncthreads();
-        const double2 d = Af[c + (size_t)n * c];
-        for (int r = c + 1 + tid; r < n; r += 256) Af[r + (size_t)n * c] = cdiv(Af[r + (size_t)n * c], d);
+        for (int cc = c + 1 + tid; cc < k + nb; cc += kPanelThreads) urow[cc - k] = Af[c + (size_t)n * cc];
         __syncthreads();
-        // rank-1 update of the remaining panel columns
-        for (int cc = c + 1; cc < k + nb; ++cc) {
-            const double2 u = Af[c + (size_t)n * cc];
-            for (int r = c + 1 + tid; r < n; r += 256) {
-                const double2 l = Af[r + (size_t)n * c];
-                const double2 pr = cmul(l, u);
-                Af[r + (size_t)n * cc].x -= pr.x;
-                Af[r + (size_t)n * cc].y -= pr.y;
+        const double2 d = Af[c + (size_t)n * c];
+        for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
+            const double2 l = cdiv(Af[r + (size_t)n * c], d);
+            Af[r + (size_t)n * c] = l;
+            for (int cc = c + 1; cc < k + nb; ++cc) {
+                const double2 pr = cmul(l, urow[cc - k]);
+                double2 a = Af[r + (size_t)n * cc];
+                a.x -= pr.x;
+                a.y -= pr.y;
+                Af[r + (size_t)n * cc] = a;
+                if (cc == c + 1) {
+                    const double v = cabs2(a);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
             }
         }
         __syncthreads();
@@ -181,14 +207,20 @@ __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, doubl
         }
 }
 
-// Solve (LU) x = P b in place for nrhs columns; one workgroup per (rhs, frequency).
+// Solve (LU) x = P b in place for the right-hand sides selected by mask;
+// one workgroup per (rhs, frequency).  Blocked by kSolveBlock columns: wave 0
+// solves the diagonal block with the values in its lanes (shuffles), then all
+// work-items update the rows outside the block.
+constexpr int kSolveBlock = 32;
 __global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restrict__ A, size_t lda_f,
                                                   const int *__restrict__ piv, double2 *__restrict__ X, size_t ldx_f,
-                                                  int ldx) {
+                                                  int ldx, int mask) {
     const int f = blockIdx.z, rcol = blockIdx.x;
+    if (!((mask >> rcol) & 1)) return;
     const double2 *Af = A + lda_f * f;
     double2 *x = X + ldx_f * f + (size_t)ldx * rcol;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid % 64;
+    __shared__ double2 xb[kSolveBlock];
     if (tid == 0)
         for (int c = 0; c < n; ++c) {
             const int p = piv[(size_t)n * f + c];
@@ -199,23 +231,66 @@ __global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restri
             }
         }
     __syncthreads();
-    for (int c = 0; c < n; ++c) {  // unit lower
-        const double2 v = x[c];
-        for (int r = c + 1 + tid; r < n; r += 256) {
-            const double2 p = cmul(Af[r + (size_t)n * c], v);
-            x[r].x -= p.x;
-            x[r].y -= p.y;
+    // unit lower, blocks top to bottom
+    for (int c0 = 0; c0 < n; c0 += kSolveBlock) {
+        const int cb = min(kSolveBlock, n - c0);
+        if (tid < 64) {
+            double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
+            for (int j = 0; j < cb; ++j) {
+                const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
+                if (lane > j && lane < cb) {
+                    const double2 p = cmul(Af[(c0 + lane) + (size_t)n * (c0 + j)], xj);
+                    v.x -= p.x;
+                    v.y -= p.y;
+                }
+            }
+            if (lane < cb) {
+                x[c0 + lane] = v;
+                xb[lane] = v;
+            }
+        }
+        __syncthreads();
+        for (int r = c0 + cb + tid; r < n; r += 256) {
+            double2 acc = make_double2(0, 0);
+            for (int j = 0; j < cb; ++j) {
+                const double2 p = cmul(Af[r + (size_t)n * (c0 + j)], xb[j]);
+                acc.x += p.x;
+                acc.y += p.y;
+            }
+            x[r].x -= acc.x;
+            x[r].y -= acc.y;
         }
         __syncthreads();
     }
-    for (int c = n - 1; c >= 0; --c) {  // upper
-        if (tid == 0) x[c] = cdiv(x[c], Af[c + (size_t)n * c]);
+    // upper, blocks bottom to top
+    for (int c1 = n; c1 > 0; c1 -= kSolveBlock) {
+        const int c0 = max(0, c1 - kSolveBlock), cb = c1 - c0;
+        if (tid < 64) {
+            double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
+            for (int j = cb - 1; j >= 0; --j) {
+                if (lane == j) v = cdiv(v, Af[(c0 + j) + (size_t)n * (c0 + j)]);
+                const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
+                if (lane < j) {
+                    const double2 p = cmul(Af[(c0 + lane) + (size_t)n * (c0 + j)], xj);
+                    v.x -= p.x;
+                    v.y -= p.y;
+                }
+            }
+            if (lane < cb) {
+                x[c0 + lane] = v;
+                xb[lane] = v;
+            }
+        }
         __syncthreads();
-        const double2 v = x[c];
-        for (int r = tid; r < c; r += 256) {
-            const double2 p = cmul(Af[r + (size_t)n * c], v);
-            x[r].x -= p.x;
-            x[r].y -= p.y;
+        for (int r = tid; r < c0; r += 256) {
+            double2 acc = make_double2(0, 0);
+            for (int j = 0; j < cb; ++j) {
+                const double2 p = cmul(Af[r + (size_t)n * (c0 + j)], xb[j]);
+                acc.x += p.x;
+                acc.y += p.y;
+            }
+            x[r].x -= acc.x;
+            x[r].y -= acc.y;
         }
         __syncthreads();
     }
@@ -401,7 +476,7 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, int *info) {
     const size_t ldf = (size_t)n * n;
     for (int k = 0; k < n; k += NB) {
         const int nb = std::min(NB, n - k);
-        k_panel<<<dim3(1, 1, nf), 256, 0, st>>>(n, k, nb, A, ldf, piv, info);
+        k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, info);
         k_swap_rest<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, k, nb, A, ldf, piv);
         const int rest = n - k - nb;
         if (rest > 0) {
@@ -414,7 +489,7 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, int *info) {
 }
 
 // Assemble A(omega_f) for frequencies [f0, f0 + nf) into A.
-void assemble(pxh_ctx *c, int f0, int nf, const double *freq, double2 *A, DevBuf<double> &M, DevBuf<double> &T,
+void assemble(pxh_ctx *c, int f0, int nf, const double *freq, double2 *A, const double *M, DevBuf<double> &T,
               DevBuf<SysParam> &par, DevBuf<double2> &lam) {
     hipStream_t st = c->stream;
     if (c->target == PXH_BEM) {
@@ -443,7 +518,7 @@ void assemble(pxh_ctx *c, int f0, int nf, const double *freq, double2 *A, DevBuf
     for (int f = 0; f < nf; ++f) hp[f] = make_sysparam(c, freq[f0 + f], f0 + f);
     par.upload(hp.data(), nf, st);
     k_assemble_atoms<<<dim3((nv + 255) / 256, nv, nf), 256, 0, st>>>(
-        a.n, nv, a.nt, a.model == PXH_WFQFMU, M.p, a.pos.p, a.d_nbfrac.p, par.p, A, (size_t)nv * nv);
+        a.n, nv, a.nt, a.model == PXH_WFQFMU, M, a.pos.p, a.d_nbfrac.p, par.p, A, (size_t)nv * nv);
     if (a.hetero)
         k_assemble_hetero_q<<<dim3((nv + 255) / 256, a.n, nf), 256, 0, st>>>(
             a.n, nv, a.nt, a.kb_rowptr.p, a.kb_col.p, a.kb_kij.p, a.kb_kji.p, a.pos.p, T.p, par.p, A,
@@ -482,10 +557,22 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     hipStream_t st = c->stream;
     const bool bem = c->target == PXH_BEM;
     const int n = bem ? c->mesh.n : c->atoms.n_var();
-    DevBuf<double> M, T;
+    DevBuf<double> Mtmp, T;
+    const double *Mp = nullptr;
     if (!bem) {
-        M.alloc((size_t)n * n);
-        constant_matrix_atoms(c, M.p);
+        // the constant matrix depends on the target alone: kept with it up
+        // to kKeepMconstBytes, else built for this call
+        Atoms &At = c->atoms;
+        if (!At.Mconst_valid) {
+            const bool keep = (size_t)n * n * sizeof(double) <= kKeepMconstBytes;
+            DevBuf<double> &dst = keep ? At.Mconst : Mtmp;
+            dst.alloc((size_t)n * n);
+            constant_matrix_atoms(c, dst.p);
+            At.Mconst_valid = keep;
+            Mp = dst.p;
+        } else {
+            Mp = At.Mconst.p;
+        }
         if (c->atoms.hetero) dense_t(c, T);
     }
     const int chunk = (int)freq_chunk(c, n, n_freq);
@@ -501,16 +588,17 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     int status = 0;
     for (int f0 = 0; f0 < n_freq; f0 += chunk) {
         const int nf = std::min(chunk, n_freq - f0);
-        assemble(c, f0, nf, freq, A.p, M, T, par, lam);
+        assemble(c, f0, nf, freq, A.p, Mp, T, par, lam);
         PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, st));
         lu_factor(c, n, nf, A.p, piv.p, info.p);
         double2 *X = d_sol + (size_t)rhs_stride * f0;
         k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, 3, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0,
                                                                  (size_t)rhs_fstride, X, (size_t)rhs_stride);
-        k_lu_solve<<<dim3(3, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X, (size_t)rhs_stride, n);
+        k_lu_solve<<<dim3(3, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X, (size_t)rhs_stride, n,
+                                                   pol_mask);
         if (cc) {
             k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, X1.p, (size_t)n);
-            k_lu_solve<<<dim3(1, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X1.p, (size_t)n, n);
+            k_lu_solve<<<dim3(1, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X1.p, (size_t)n, n, 1);
             k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, X, (size_t)rhs_stride, X1.p, (size_t)n);
         }
         PXH_HIP(hipGetLastError());

@@ -827,6 +827,7 @@ SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq) {
 
 void build_atoms_device(pxh_ctx *c) {
     Atoms &A = c->atoms;
+    A.Mconst_valid = false;
     std::vector<double4> hp(A.n);
     for (int i = 0; i < A.n; ++i)
         hp[i] = make_double4(A.h_coord[3 * i], A.h_coord[3 * i + 1], A.h_coord[3 * i + 2], (double)A.h_type[i]);
@@ -895,6 +896,7 @@ void build_kbar_csr(pxh_ctx *c, double cut, DevBuf<int> &rowptr, DevBuf<int> &co
 // own rounding), about a third of the entries.
 void build_kbar(pxh_ctx *c) {
     Atoms &A = c->atoms;
+    A.Mconst_valid = false;
     const bool cut = c->opt.kbar_cutoff != 0;
     build_kbar_csr(c, cut ? 1e-18 : 0.0, A.kb_rowptr, A.kb_col, A.kb_kij, A.kb_kji, A.kb_nnz);
     if (cut)

@@ -35,6 +35,7 @@ constexpr int kMaxTypes = 8;
 // Frequencies whose dense matrices are factorised together (inversion).
 constexpr size_t kMaxFreqChunk = 64;                  // dense path: frequencies factorised together ...
 constexpr size_t kDenseChunkBytes = size_t(8) << 30;  // ... within this many bytes of matrices
+constexpr size_t kKeepMconstBytes = size_t(2) << 30;  // dense constant matrices kept with the target up to this
 
 // Per type-pair constants of the interaction kernels (index ti + nt*tj).
 struct PairConst {
@@ -96,6 +97,8 @@ struct Atoms {
     size_t kbm_nnz = 0;
     bool kbm_built = false;
     DevBuf<double> Tstore;              // stored T for the in-memory method (n_var^2)
+    DevBuf<double> Mconst;              // dense constant matrix of the inversion path, kept when small
+    bool Mconst_valid = false;
     // diagonal of K-bar T_qq: S1_i = sum_j K_ij (T_qq(j,i) - eta_i), and for
     // alloys S2_i[t] = sum_{j of type t} K_ji (T_qq(j,i) - eta_i)
     DevBuf<double> diag_s1, diag_s2;
