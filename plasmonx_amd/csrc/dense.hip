@@ -29,20 +29,33 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
 }
 __device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
 
+// The panel's row interchanges as one permutation of the rows they touch,
+// for the other columns: window row k + i takes the old row src_w[i]; rows
+// below the window that the interchanges reach, dst_o[j] (j < m_o), take
+// the old window row src_o[j].
+struct SwapSet {
+    int src_w[NB];
+    int dst_o[NB];
+    int src_o[NB];
+    int m_o;
+};
+
 // Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
-// frequency.  Pivots (global row indices) go to piv[f][k..k+nb).  Row swaps
-// are applied inside the panel only.  One pass over the rows per column:
-// scale column c, update the remaining panel columns with row c (staged in
-// LDS), and find the pivot candidates of column c + 1 on the way.
+// frequency.  Pivots (global row indices) go to piv[f][k..k+nb) and their net
+// permutation to swaps[f].  Row swaps are applied inside the panel only.  One
+// pass over the rows per column: scale column c, update the remaining panel
+// columns with row c (staged in LDS), and find the pivot candidates of column
+// c + 1 on the way.
 constexpr int kPanelThreads = 512;
 __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, double2 *__restrict__ A,
                                                          size_t lda_f, int *__restrict__ piv,
-                                                         int *__restrict__ info) {
+                                                         SwapSet *__restrict__ swaps, int *__restrict__ info) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     __shared__ double sval[kPanelThreads];
     __shared__ int sidx[kPanelThreads];
     __shared__ double2 urow[NB];
+    __shared__ int spiv[NB];
     const int tid = threadIdx.x;
     // pivot candidates of column k (first maximum, as IZAMAX)
     double best = -1.0;
@@ -72,6 +85,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
         __syncthreads();
         if (tid == 0) {
             piv[(size_t)n * f + c] = p;
+            spiv[c - k] = pv == 0.0 ? c : p;
             if (pv == 0.0 && info[f] == 0) info[f] = c + 1;
         }
         best = -1.0;
@@ -119,45 +133,84 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
         }
         __syncthreads();
     }
-}
-
-// Apply the panel's row swaps to the columns outside [k, k+nb).
-__global__ void k_swap_rest(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f, const int *__restrict__ piv) {
-    const int f = blockIdx.z;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n || (col >= k && col < k + nb)) return;
-    double2 *Af = A + lda_f * f;
-    for (int c = k; c < k + nb; ++c) {
-        const int p = piv[(size_t)n * f + c];
-        if (p != c) {
-            const double2 t = Af[c + (size_t)n * col];
-            Af[c + (size_t)n * col] = Af[p + (size_t)n * col];
-            Af[p + (size_t)n * col] = t;
+    // net permutation of the touched rows, wave 0: lane i < nb holds window
+    // row k + i, lanes from 32 the rows below the window in order of first
+    // touch; each lane carries the old row now at its position
+    if (tid < 64) {
+        const int lane = tid;
+        int row = lane < nb ? k + lane : -1;
+        int cont = row;
+        int extra = 0;
+        for (int i = 0; i < nb; ++i) {
+            const int p = spiv[i];
+            if (p == k + i) continue;
+            int slot;
+            if (p < k + nb) {
+                slot = p - k;
+            } else {
+                const unsigned long long hit = __ballot(lane >= 32 && row == p);
+                if (hit) {
+                    slot = __ffsll((long long)hit) - 1;
+                } else {
+                    slot = 32 + extra++;
+                    if (lane == slot) row = cont = p;
+                }
+            }
+            const int ci = __shfl(cont, i, 64), cp = __shfl(cont, slot, 64);
+            if (lane == i) cont = cp;
+            if (lane == slot) cont = ci;
         }
+        SwapSet &S = swaps[f];
+        if (lane < nb) S.src_w[lane] = cont;
+        if (lane >= 32 && lane < 32 + extra) {
+            S.dst_o[lane - 32] = row;
+            S.src_o[lane - 32] = cont;
+        }
+        if (lane == 0) S.m_o = extra;
     }
 }
 
-// U12 = L11^-1 A12 (unit lower L11 of the panel), one work-item per column.
-__global__ void k_trsm_u12(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f) {
+// The panel's row interchanges applied to every column outside [k, k+nb),
+// and U12 = L11^-1 A12 (unit lower L11 of the panel) for the columns right of
+// it; one work-item per column, the window of the column in registers (loops
+// over the constant panel width, fully unrolled).  All old values a column
+// needs are read before any of its rows is written.
+__global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f,
+                                                   const SwapSet *__restrict__ swaps) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     __shared__ double2 L[NB][NB];
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-        const int r = e % nb, c = e / nb;
-        L[r][c] = Af[(k + r) + (size_t)n * (k + c)];
-    }
-    __syncthreads();
-    const int col = k + nb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
-    double2 x[NB];
-    for (int r = 0; r < nb; ++r) x[r] = Af[(k + r) + (size_t)n * col];
-    for (int r = 0; r < nb; ++r)
-        for (int q = r + 1; q < nb; ++q) {
-            const double2 p = cmul(L[q][r], x[r]);
-            x[q].x -= p.x;
-            x[q].y -= p.y;
+    __shared__ SwapSet S;
+    const bool trsm = k + nb < n;  // then nb == NB
+    if (trsm)
+        for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+            const int r = e % NB, c = e / NB;
+            L[r][c] = Af[(k + r) + (size_t)n * (k + c)];
         }
-    for (int r = 0; r < nb; ++r) Af[(k + r) + (size_t)n * col] = x[r];
+    for (int e = threadIdx.x; e < (int)(sizeof(SwapSet) / sizeof(int)); e += blockDim.x)
+        reinterpret_cast<int *>(&S)[e] = reinterpret_cast<const int *>(swaps + f)[e];
+    __syncthreads();
+    const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= n - nb) return;
+    const int col = cidx < k ? cidx : cidx + nb;
+    double2 *a = Af + (size_t)n * col;
+    double2 x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = i < nb ? a[S.src_w[i]] : make_double2(0, 0);
+    for (int j = 0; j < S.m_o; ++j) a[S.dst_o[j]] = a[S.src_o[j]];
+    if (col > k) {
+#pragma unroll
+        for (int r = 0; r < NB; ++r)
+#pragma unroll
+            for (int q = r + 1; q < NB; ++q) {
+                const double2 p = cmul(L[q][r], x[r]);
+                x[q].x -= p.x;
+                x[q].y -= p.y;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i < nb) a[k + i] = x[i];
 }
 
 // A22 -= L21 U12, 64 x 64 output tile per workgroup of 256 (4 x 4 per item).
@@ -207,39 +260,57 @@ __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, doubl
         }
 }
 
-// Solve (LU) x = P b in place for the right-hand sides selected by mask;
-// one workgroup per (rhs, frequency).  Blocked by kSolveBlock columns: wave 0
-// solves the diagonal block with the values in its lanes (shuffles), then all
-// work-items update the rows outside the block.
-constexpr int kSolveBlock = 32;
-__global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restrict__ A, size_t lda_f,
-                                                  const int *__restrict__ piv, double2 *__restrict__ X, size_t ldx_f,
-                                                  int ldx, int mask) {
+// Solve (LU) x = b in place, b already row-permuted, for the right-hand
+// sides selected by mask; one workgroup per (rhs, frequency).  Blocked by
+// kSolveBlock columns: the diagonal block is staged in LDS, wave 0 solves it
+// with the values in its lanes (shuffles), then all work-items update the
+// rows outside the block (one row each, the block's columns read together).
+constexpr int kSolveBlock = 32, kSolveThreads = 1024;
+
+// x[r] -= sum_j A[r][c0 + j] xb[j] for the rows r handled by this work-item
+__device__ inline void lu_solve_update(int n, const double2 *__restrict__ Af, int c0, int cb, int r0, int r1,
+                                       const double2 *xb, double2 *__restrict__ x) {
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += kSolveThreads) {
+        const double2 *a = Af + r + (size_t)n * c0;
+        double2 acc = make_double2(0, 0);
+#pragma unroll 8
+        for (int j = 0; j < cb; ++j) {
+            const double2 p = cmul(a[(size_t)n * j], xb[j]);
+            acc.x += p.x;
+            acc.y += p.y;
+        }
+        x[r].x -= acc.x;
+        x[r].y -= acc.y;
+    }
+}
+
+__global__ __launch_bounds__(kSolveThreads) void k_lu_solve(int n, const double2 *__restrict__ A, size_t lda_f,
+                                                            double2 *__restrict__ X, size_t ldx_f, int ldx,
+                                                            int mask) {
     const int f = blockIdx.z, rcol = blockIdx.x;
     if (!((mask >> rcol) & 1)) return;
     const double2 *Af = A + lda_f * f;
     double2 *x = X + ldx_f * f + (size_t)ldx * rcol;
     const int tid = threadIdx.x, lane = tid % 64;
     __shared__ double2 xb[kSolveBlock];
-    if (tid == 0)
-        for (int c = 0; c < n; ++c) {
-            const int p = piv[(size_t)n * f + c];
-            if (p != c) {
-                const double2 t = x[c];
-                x[c] = x[p];
-                x[p] = t;
-            }
+    __shared__ double2 Ab[kSolveBlock][kSolveBlock + 1];  // [col][row]
+    auto stage = [&](int c0, int cb) {
+        for (int i = tid; i < cb * cb; i += kSolveThreads) {
+            const int r = i % cb, col = i / cb;
+            Ab[col][r] = Af[(c0 + r) + (size_t)n * (c0 + col)];
         }
-    __syncthreads();
+        __syncthreads();
+    };
     // unit lower, blocks top to bottom
     for (int c0 = 0; c0 < n; c0 += kSolveBlock) {
         const int cb = min(kSolveBlock, n - c0);
+        stage(c0, cb);
         if (tid < 64) {
             double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
             for (int j = 0; j < cb; ++j) {
                 const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
                 if (lane > j && lane < cb) {
-                    const double2 p = cmul(Af[(c0 + lane) + (size_t)n * (c0 + j)], xj);
+                    const double2 p = cmul(Ab[j][lane], xj);
                     v.x -= p.x;
                     v.y -= p.y;
                 }
@@ -250,28 +321,20 @@ __global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restri
             }
         }
         __syncthreads();
-        for (int r = c0 + cb + tid; r < n; r += 256) {
-            double2 acc = make_double2(0, 0);
-            for (int j = 0; j < cb; ++j) {
-                const double2 p = cmul(Af[r + (size_t)n * (c0 + j)], xb[j]);
-                acc.x += p.x;
-                acc.y += p.y;
-            }
-            x[r].x -= acc.x;
-            x[r].y -= acc.y;
-        }
+        lu_solve_update(n, Af, c0, cb, c0 + cb, n, xb, x);
         __syncthreads();
     }
     // upper, blocks bottom to top
     for (int c1 = n; c1 > 0; c1 -= kSolveBlock) {
         const int c0 = max(0, c1 - kSolveBlock), cb = c1 - c0;
+        stage(c0, cb);
         if (tid < 64) {
             double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
             for (int j = cb - 1; j >= 0; --j) {
-                if (lane == j) v = cdiv(v, Af[(c0 + j) + (size_t)n * (c0 + j)]);
+                if (lane == j) v = cdiv(v, Ab[j][j]);
                 const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
                 if (lane < j) {
-                    const double2 p = cmul(Af[(c0 + lane) + (size_t)n * (c0 + j)], xj);
+                    const double2 p = cmul(Ab[j][lane], xj);
                     v.x -= p.x;
                     v.y -= p.y;
                 }
@@ -282,16 +345,7 @@ __global__ __launch_bounds__(256) void k_lu_solve(int n, const double2 *__restri
             }
         }
         __syncthreads();
-        for (int r = tid; r < c0; r += 256) {
-            double2 acc = make_double2(0, 0);
-            for (int j = 0; j < cb; ++j) {
-                const double2 p = cmul(Af[r + (size_t)n * (c0 + j)], xb[j]);
-                acc.x += p.x;
-                acc.y += p.y;
-            }
-            x[r].x -= acc.x;
-            x[r].y -= acc.y;
-        }
+        lu_solve_update(n, Af, c0, cb, 0, c0, xb, x);
         __syncthreads();
     }
 }
@@ -404,14 +458,49 @@ __global__ __launch_bounds__(256) void k_assemble_bem_ief(int n, const double *_
 }
 
 // Copy right-hand sides of one frequency: X_f(:, k) = rhs_f(:, k) for k in pol_mask, 0 otherwise.
-__global__ void k_copy_rhs(int n, int nrhs, int mask, const double2 *__restrict__ rhs, size_t ld_rhs_f,
-                           double2 *__restrict__ X, size_t ldx_f) {
+// The row interchanges of a factorisation as one permutation: perm[f][i] is
+// the row of b that P b holds at i (the interchanges applied in order to the
+// identity).  The pivots are staged through LDS in chunks so the serial pass
+// of work-item 0 touches LDS only; the permutation itself lives in LDS when
+// it fits.
+constexpr int kPermLds = 16384, kPivChunk = 2048;
+__global__ __launch_bounds__(256) void k_perm_from_piv(int n, const int *__restrict__ piv, int *__restrict__ perm) {
+    __shared__ int sp[kPermLds];
+    __shared__ int pc[kPivChunk];
+    const int f = blockIdx.x;
+    const int *pf = piv + (size_t)n * f;
+    int *out = perm + (size_t)n * f;
+    const bool in_lds = n <= kPermLds;
+    int *q = in_lds ? sp : out;
+    for (int i = threadIdx.x; i < n; i += 256) q[i] = i;
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += kPivChunk) {
+        const int cn = min(kPivChunk, n - c0);
+        for (int i = threadIdx.x; i < cn; i += 256) pc[i] = pf[c0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int c = 0; c < cn; ++c) {
+                const int p = pc[c];
+                const int t = q[c0 + c];
+                q[c0 + c] = q[p];
+                q[p] = t;
+            }
+        __syncthreads();
+    }
+    if (in_lds)
+        for (int i = threadIdx.x; i < n; i += 256) out[i] = sp[i];
+}
+
+// X[f][k] = P_f b_k for the selected right-hand sides (zero otherwise).
+__global__ void k_copy_rhs_perm(int n, int mask, const double2 *__restrict__ rhs, size_t ld_rhs_f,
+                                const int *__restrict__ perm, double2 *__restrict__ X, size_t ldx_f) {
     const int f = blockIdx.z;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (r >= n) return;
     const bool on = (mask >> k) & 1;
-    X[ldx_f * f + (size_t)n * k + r] = on ? rhs[ld_rhs_f * f + (size_t)n * k + r] : make_double2(0, 0);
+    X[ldx_f * f + (size_t)n * k + r] =
+        on ? rhs[ld_rhs_f * f + (size_t)n * k + perm[(size_t)n * f + r]] : make_double2(0, 0);
 }
 
 __global__ void k_fill_one(int n, double2 *__restrict__ X, size_t ldx_f) {
@@ -471,16 +560,16 @@ __global__ void k_dense_matvec(int n, const double2 *__restrict__ A, size_t lda_
     y[(size_t)ld * s + r] = acc;
 }
 
-void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, int *info) {
+void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, int *info) {
     hipStream_t st = c->stream;
     const size_t ldf = (size_t)n * n;
     for (int k = 0; k < n; k += NB) {
         const int nb = std::min(NB, n - k);
-        k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, info);
-        k_swap_rest<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, k, nb, A, ldf, piv);
+        k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        if (n > nb)
+            k_swap_trsm<<<dim3((n - nb + 127) / 128, 1, nf), 128, 0, st>>>(n, k, nb, A, ldf, swaps);
         const int rest = n - k - nb;
         if (rest > 0) {
-            k_trsm_u12<<<dim3((rest + 127) / 128, 1, nf), 128, 0, st>>>(n, k, nb, A, ldf);
             const int tiles = (rest + 63) / 64;
             k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, nb, A, ldf);
         }
@@ -576,12 +665,16 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
         if (c->atoms.hetero) dense_t(c, T);
     }
     const int chunk = (int)freq_chunk(c, n, n_freq);
-    DevBuf<double2> A, A1;
-    DevBuf<int> piv, info;
+    // the chunk's matrices and pivots stay with the context between calls
+    // (at most kDenseChunkBytes of HBM, held until the context is destroyed)
+    DevBuf<double2> &A = c->work_lu;
+    DevBuf<int> info;
     DevBuf<SysParam> par;
     DevBuf<double2> lam, X1;
     A.alloc((size_t)n * n * chunk);
-    piv.alloc((size_t)n * chunk);
+    c->work_piv.alloc((size_t)2 * n * chunk + sizeof(SwapSet) / sizeof(int) * chunk);
+    int *piv = c->work_piv.p, *perm = c->work_piv.p + (size_t)n * chunk;
+    SwapSet *swaps = reinterpret_cast<SwapSet *>(c->work_piv.p + (size_t)2 * n * chunk);
     info.alloc(chunk);
     const bool cc = bem && c->mesh.charge_constraint;
     if (cc) X1.alloc((size_t)n * chunk);
@@ -590,15 +683,16 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
         const int nf = std::min(chunk, n_freq - f0);
         assemble(c, f0, nf, freq, A.p, Mp, T, par, lam);
         PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, st));
-        lu_factor(c, n, nf, A.p, piv.p, info.p);
+        lu_factor(c, n, nf, A.p, piv, swaps, info.p);
         double2 *X = d_sol + (size_t)rhs_stride * f0;
-        k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, 3, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0,
-                                                                 (size_t)rhs_fstride, X, (size_t)rhs_stride);
-        k_lu_solve<<<dim3(3, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X, (size_t)rhs_stride, n,
-                                                   pol_mask);
+        k_perm_from_piv<<<nf, 256, 0, st>>>(n, piv, perm);
+        k_copy_rhs_perm<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(
+            n, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0, (size_t)rhs_fstride, perm, X, (size_t)rhs_stride);
+        k_lu_solve<<<dim3(3, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, X, (size_t)rhs_stride, n, pol_mask);
         if (cc) {
             k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, X1.p, (size_t)n);
-            k_lu_solve<<<dim3(1, 1, nf), 256, 0, st>>>(n, A.p, (size_t)n * n, piv.p, X1.p, (size_t)n, n, 1);
+            // P 1 = 1: the all-ones right-hand side needs no interchange
+            k_lu_solve<<<dim3(1, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, X1.p, (size_t)n, n, 1);
             k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, X, (size_t)rhs_stride, X1.p, (size_t)n);
         }
         PXH_HIP(hipGetLastError());

@@ -180,6 +180,8 @@ struct pxh_ctx {
     pxh::DevBuf<double> work_colmax;          // chunk maxima behind those scales
     pxh::DevBuf<double> work_pack;            // packed operand of the BEM matrix products
     pxh::DevBuf<double> work_gemm;            // split-K partial sums of those products
+    pxh::DevBuf<double2> work_lu;             // dense path: the factorised matrices of one frequency chunk
+    pxh::DevBuf<int> work_piv;                // their row interchanges, and the permutations built from them
     bool use_stored = false;                  // in-memory method: products use the stored T
     // IEF operator form used by the products: 0 the system matrix
     // -(D + lambda I) S; 1 the right-preconditioned -(D + lambda I) (unknown
