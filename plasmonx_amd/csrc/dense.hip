@@ -40,6 +40,43 @@ struct SwapSet {
     int m_o;
 };
 
+// Net permutation of the interchanges pv[0..w) (pivot rows, global indices)
+// of the window rows [w0, w0 + w), w <= 32, by one wave: lane i < w holds
+// window row w0 + i, lanes from 32 the rows below the window in order of
+// first touch; each lane carries the old row now at its position.  Writes
+// src_w[0..w), dst_o / src_o[0..m_o) and m_o as SwapSet describes.
+__device__ void net_permutation(int lane, int w0, int w, const int *pv, int *src_w, int *dst_o, int *src_o,
+                                int *m_o) {
+    int row = lane < w ? w0 + lane : -1;
+    int cont = row;
+    int extra = 0;
+    for (int i = 0; i < w; ++i) {
+        const int p = pv[i];
+        if (p == w0 + i) continue;
+        int slot;
+        if (p < w0 + w) {
+            slot = p - w0;
+        } else {
+            const unsigned long long hit = __ballot(lane >= 32 && row == p);
+            if (hit) {
+                slot = __ffsll((long long)hit) - 1;
+            } else {
+                slot = 32 + extra++;
+                if (lane == slot) row = cont = p;
+            }
+        }
+        const int ci = __shfl(cont, i, 64), cp = __shfl(cont, slot, 64);
+        if (lane == i) cont = cp;
+        if (lane == slot) cont = ci;
+    }
+    if (lane < w) src_w[lane] = cont;
+    if (lane >= 32 && lane < 32 + extra) {
+        dst_o[lane - 32] = row;
+        src_o[lane - 32] = cont;
+    }
+    if (lane == 0) *m_o = extra;
+}
+
 // Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
 // frequency.  Pivots (global row indices) go to piv[f][k..k+nb) and their net
 // permutation to swaps[f].  Row swaps are applied inside the panel only.  One
@@ -133,72 +170,286 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
         }
         __syncthreads();
     }
-    // net permutation of the touched rows, wave 0: lane i < nb holds window
-    // row k + i, lanes from 32 the rows below the window in order of first
-    // touch; each lane carries the old row now at its position
+    // net permutation of the panel's interchanges for the other columns
     if (tid < 64) {
-        const int lane = tid;
-        int row = lane < nb ? k + lane : -1;
-        int cont = row;
-        int extra = 0;
-        for (int i = 0; i < nb; ++i) {
-            const int p = spiv[i];
-            if (p == k + i) continue;
-            int slot;
-            if (p < k + nb) {
-                slot = p - k;
-            } else {
-                const unsigned long long hit = __ballot(lane >= 32 && row == p);
-                if (hit) {
-                    slot = __ffsll((long long)hit) - 1;
-                } else {
-                    slot = 32 + extra++;
-                    if (lane == slot) row = cont = p;
-                }
-            }
-            const int ci = __shfl(cont, i, 64), cp = __shfl(cont, slot, 64);
-            if (lane == i) cont = cp;
-            if (lane == slot) cont = ci;
-        }
         SwapSet &S = swaps[f];
-        if (lane < nb) S.src_w[lane] = cont;
-        if (lane >= 32 && lane < 32 + extra) {
-            S.dst_o[lane - 32] = row;
-            S.src_o[lane - 32] = cont;
-        }
-        if (lane == 0) S.m_o = extra;
+        net_permutation(tid, k, nb, spiv, S.src_w, S.dst_o, S.src_o, &S.m_o);
     }
 }
 
-// The panel's row interchanges applied to every column outside [k, k+nb),
+// The same panel factorisation with the panel held in registers: work-item t
+// owns rows k + t + kPanelThreads i (i < R), and the panel is processed in
+// sub-panels of SUB = min(8, 32 / R) columns (R SUB complex values per
+// work-item; the column loop is unrolled, so SUB stays small).
+// Per sub-panel: load it, bring it up to date with the sub-panels before it
+// (their U rows by forward substitution in LDS, the rows below by the rank
+// update with their L), factorise its columns one by one in registers (each
+// wave publishes its best candidate row in LDS together with row c, so one
+// barrier per column settles the pivot and the exchange), write it back, and
+// apply its interchanges to the other columns of the panel as one net
+// permutation.  Same pivots as k_panel; column c below the diagonal is scaled
+// by the reciprocal of the pivot, as LAPACK's ZGETF2 does.
+template <int R>
+__global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int nb, double2 *__restrict__ A,
+                                                             size_t lda_f, int *__restrict__ piv,
+                                                             SwapSet *__restrict__ swaps, int *__restrict__ info) {
+    constexpr int SUB = NB / R < 8 ? NB / R : 8;
+    constexpr int W = kPanelThreads / 64;
+    const int f = blockIdx.z;
+    double2 *Af = A + lda_f * f;
+    const int tid = threadIdx.x, lane = tid % 64, wid = tid / 64;
+    __shared__ double wv[2][W];
+    __shared__ int wi[2][W];
+    __shared__ double2 cand[2][W][SUB], crow[2][SUB];  // by column parity
+    __shared__ int spiv[NB];
+    __shared__ double2 Lw[NB][NB + 1];  // unit lower block of the finished sub-panels, [row][col]
+    __shared__ double2 Uw[NB][SUB];     // their U rows in the current sub-panel's columns
+    __shared__ int ps_src_w[SUB], ps_dst[NB], ps_src[NB], ps_m;
+    double2 v[R][SUB];
+    for (int s0 = k; s0 < k + nb; s0 += SUB) {
+        const int sb = min(SUB, k + nb - s0);
+        const int done = s0 - k;
+        // load the sub-panel (rows k..n)
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + tid + kPanelThreads * i;
+#pragma unroll
+            for (int j = 0; j < SUB; ++j)
+                v[i][j] = (r < n && j < sb) ? Af[r + (size_t)n * (s0 + j)] : make_double2(0, 0);
+        }
+        if (done > 0) {
+            // U rows [k, s0): forward substitution with the unit lower block
+            for (int e = tid; e < done * done; e += kPanelThreads) {
+                const int rr = e % done, cc = e / done;
+                Lw[rr][cc] = Af[(k + rr) + (size_t)n * (k + cc)];
+            }
+            if (tid < done)
+#pragma unroll
+                for (int j = 0; j < SUB; ++j) Uw[tid][j] = v[0][j];
+            __syncthreads();
+            // wave w solves column w, lane rr holding row rr
+            if (wid < sb) {
+                double2 val = lane < done ? Uw[lane][wid] : make_double2(0, 0);
+                for (int q = 0; q + 1 < done; ++q) {
+                    const double2 xq = make_double2(__shfl(val.x, q, 64), __shfl(val.y, q, 64));
+                    if (lane > q && lane < done) {
+                        const double2 pr = cmul(Lw[lane][q], xq);
+                        val.x -= pr.x;
+                        val.y -= pr.y;
+                    }
+                }
+                if (lane < done) Uw[lane][wid] = val;
+            }
+            __syncthreads();
+            // rows s0..n: rank update with the finished sub-panels' L
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = k + tid + kPanelThreads * i;
+                if (r >= s0 && r < n) {
+#pragma unroll 4
+                    for (int q = 0; q < done; ++q) {
+                        const double2 l = Af[r + (size_t)n * (k + q)];
+#pragma unroll
+                        for (int j = 0; j < SUB; ++j) {
+                            const double2 u = Uw[q][j];
+                            v[i][j].x -= l.x * u.x - l.y * u.y;
+                            v[i][j].y -= l.x * u.y + l.y * u.x;
+                        }
+                    }
+                }
+            }
+            if (tid < done)
+#pragma unroll
+                for (int j = 0; j < SUB; ++j) v[0][j] = Uw[tid][j];
+        }
+        // factorise the sub-panel's columns
+#pragma unroll
+        for (int j = 0; j < SUB; ++j) {
+            if (j < sb) {
+                const int c = s0 + j, b = j & 1;
+                double best = -1.0;
+                int bi = c;
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int r = k + tid + kPanelThreads * i;
+                    if (r >= c && r < n) {
+                        const double a = cabs2(v[i][j]);
+                        if (a > best) {
+                            best = a;
+                            bi = r;
+                        }
+                    }
+                }
+                // first maximum of the wave; its owner publishes the row
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    if (ob > best || (ob == best && oi < bi)) {
+                        best = ob;
+                        bi = oi;
+                    }
+                }
+                if (best >= 0.0) {
+                    const int tw = (bi - k) % kPanelThreads, iw = (bi - k) / kPanelThreads;
+#pragma unroll
+                    for (int i = 0; i < R; ++i)
+                        if (tid == tw && i == iw)
+#pragma unroll
+                            for (int jj = 0; jj < SUB; ++jj) cand[b][wid][jj] = v[i][jj];
+                }
+                if (lane == 0) {
+                    wv[b][wid] = best;
+                    wi[b][wid] = bi;
+                }
+                const int tc = (c - k) % kPanelThreads, ic = (c - k) / kPanelThreads;
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (tid == tc && i == ic)
+#pragma unroll
+                        for (int jj = 0; jj < SUB; ++jj) crow[b][jj] = v[i][jj];
+                __syncthreads();
+                // first maximum over the workgroup
+                best = wv[b][0];
+                bi = wi[b][0];
+                int ws = 0;
+#pragma unroll
+                for (int w = 1; w < W; ++w)
+                    if (wv[b][w] > best || (wv[b][w] == best && wi[b][w] < bi)) {
+                        best = wv[b][w];
+                        bi = wi[b][w];
+                        ws = w;
+                    }
+                const int p = bi;
+                if (tid == 0) {
+                    piv[(size_t)n * f + c] = p;
+                    spiv[c - k] = best == 0.0 ? c : p;
+                    if (best == 0.0 && info[f] == 0) info[f] = c + 1;
+                }
+                if (best != 0.0) {
+                    // exchange rows c and p
+                    const double2 *prow = cand[b][ws];
+                    const int tp = (p - k) % kPanelThreads, ip = (p - k) / kPanelThreads;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        if (tid == tc && i == ic)
+#pragma unroll
+                            for (int jj = 0; jj < SUB; ++jj) v[i][jj] = prow[jj];
+                        if (p != c && tid == tp && i == ip)
+#pragma unroll
+                            for (int jj = 0; jj < SUB; ++jj) v[i][jj] = crow[b][jj];
+                    }
+                    // scale column c below the diagonal, update the columns right of it
+                    const double2 d = prow[j];
+                    const double dd = d.x * d.x + d.y * d.y;
+                    const double2 dinv = make_double2(d.x / dd, -d.y / dd);
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const int r = k + tid + kPanelThreads * i;
+                        if (r > c && r < n) {
+                            const double2 l = cmul(v[i][j], dinv);
+                            v[i][j] = l;
+#pragma unroll
+                            for (int jj = j + 1; jj < SUB; ++jj) {
+                                const double2 u = prow[jj];
+                                v[i][jj].x -= l.x * u.x - l.y * u.y;
+                                v[i][jj].y -= l.x * u.y + l.y * u.x;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // write the sub-panel back
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = k + tid + kPanelThreads * i;
+            if (r < n)
+#pragma unroll
+                for (int j = 0; j < SUB; ++j)
+                    if (j < sb) Af[r + (size_t)n * (s0 + j)] = v[i][j];
+        }
+        __syncthreads();
+        // its interchanges on the other columns of the panel
+        if (nb > sb) {
+            if (tid < 64) net_permutation(tid, s0, sb, spiv + done, ps_src_w, ps_dst, ps_src, &ps_m);
+            __syncthreads();
+            if (tid < nb - sb) {
+                const int col = tid < done ? k + tid : k + tid + sb;
+                double2 *a = Af + (size_t)n * col;
+                double2 *x = v[0];  // the sub-panel is written back; its registers are free
+#pragma unroll
+                for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[ps_src_w[j]] : make_double2(0, 0);
+                for (int jj = 0; jj < ps_m; ++jj) a[ps_dst[jj]] = a[ps_src[jj]];
+#pragma unroll
+                for (int j = 0; j < SUB; ++j)
+                    if (j < sb) a[s0 + j] = x[j];
+            }
+            __syncthreads();
+        }
+    }
+    // net permutation of the panel's interchanges for the other columns
+    if (tid < 64) {
+        SwapSet &S = swaps[f];
+        net_permutation(tid, k, nb, spiv, S.src_w, S.dst_o, S.src_o, &S.m_o);
+    }
+}
+
+// The panel's row interchanges applied to the columns outside [k, k+nb),
 // and U12 = L11^-1 A12 (unit lower L11 of the panel) for the columns right of
 // it; one work-item per column, the window of the column in registers (loops
 // over the constant panel width, fully unrolled).  All old values a column
-// needs are read before any of its rows is written.
+// needs are read before any of its rows is written.  Columns by mode:
+//   kSwapAll    every column outside the panel;
+//   kSwapRight  the columns right of the panel, whose window rows first take
+//               the update by the previous panel [k - NB, k) (its L rows
+//               read at their places before this panel's interchanges, its
+//               U rows above the window), as the second panel of a pair;
+//   kSwapLeft   the columns left of the panel (interchanges only), after
+//               kSwapRight.
+enum { kSwapAll = 0, kSwapRight = 1, kSwapLeft = 2 };
 __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f,
-                                                   const SwapSet *__restrict__ swaps) {
+                                                   const SwapSet *__restrict__ swaps, int mode) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     __shared__ double2 L[NB][NB];
+    __shared__ double2 Lp[NB][NB + 1];  // kSwapRight: previous panel's L in this panel's window rows
     __shared__ SwapSet S;
-    const bool trsm = k + nb < n;  // then nb == NB
+    for (int e = threadIdx.x; e < (int)(sizeof(SwapSet) / sizeof(int)); e += blockDim.x)
+        reinterpret_cast<int *>(&S)[e] = reinterpret_cast<const int *>(swaps + f)[e];
+    __syncthreads();
+    const bool trsm = mode != kSwapLeft && k + nb < n;  // then nb == NB
     if (trsm)
         for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
             const int r = e % NB, c = e / NB;
             L[r][c] = Af[(k + r) + (size_t)n * (k + c)];
         }
-    for (int e = threadIdx.x; e < (int)(sizeof(SwapSet) / sizeof(int)); e += blockDim.x)
-        reinterpret_cast<int *>(&S)[e] = reinterpret_cast<const int *>(swaps + f)[e];
+    if (mode == kSwapRight)
+        for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+            const int r = e % NB, q = e / NB;
+            Lp[r][q] = Af[S.src_w[r] + (size_t)n * (k - NB + q)];
+        }
     __syncthreads();
     const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= n - nb) return;
-    const int col = cidx < k ? cidx : cidx + nb;
+    const int ncol = mode == kSwapAll ? n - nb : mode == kSwapRight ? n - k - nb : k;
+    if (cidx >= ncol) return;
+    const int col = mode == kSwapRight ? k + nb + cidx : (mode == kSwapLeft || cidx < k) ? cidx : cidx + nb;
     double2 *a = Af + (size_t)n * col;
     double2 x[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) x[i] = i < nb ? a[S.src_w[i]] : make_double2(0, 0);
     for (int j = 0; j < S.m_o; ++j) a[S.dst_o[j]] = a[S.src_o[j]];
-    if (col > k) {
+    if (mode == kSwapRight)
+        for (int q = 0; q < NB; ++q) {
+            const double2 u = a[k - NB + q];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const double2 p = cmul(Lp[i][q], u);
+                x[i].x -= p.x;
+                x[i].y -= p.y;
+            }
+        }
+    if (trsm && col > k) {
 #pragma unroll
         for (int r = 0; r < NB; ++r)
 #pragma unroll
@@ -214,11 +465,13 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
 }
 
 // A22 -= L21 U12, 64 x 64 output tile per workgroup of 256 (4 x 4 per item).
-__global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f) {
+// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 16.
+__global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, int c_end, double2 *__restrict__ A,
+                                                     size_t lda_f) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     const int r0 = k + nb + blockIdx.x * 64, c0 = k + nb + blockIdx.y * 64;
-    if (r0 >= n || c0 >= n) return;
+    if (r0 >= n || c0 >= c_end) return;
     constexpr int KC = 16;  // k-chunk staged in LDS (34 KB per workgroup)
     __shared__ double2 sL[64][KC + 1];
     __shared__ double2 sU[KC][64 + 1];
@@ -236,7 +489,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, doubl
         }
         for (int e = tid; e < KC * 64; e += 256) {
             const int q = e % KC, c = e / KC;
-            sU[q][c] = (c0 + c < n && q < kc) ? Af[(k + q0 + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
+            sU[q][c] = (c0 + c < c_end && q < kc) ? Af[(k + q0 + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
         }
         __syncthreads();
         for (int q = 0; q < kc; ++q) {
@@ -253,7 +506,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, doubl
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) {
             const int r = r0 + tr + 16 * a, c = c0 + tc + 16 * b;
-            if (r < n && c < n) {
+            if (r < n && c < c_end) {
                 Af[r + (size_t)n * c].x -= acc[a][b].x;
                 Af[r + (size_t)n * c].y -= acc[a][b].y;
             }
@@ -560,18 +813,51 @@ __global__ void k_dense_matvec(int n, const double2 *__restrict__ A, size_t lda_
     y[(size_t)ld * s + r] = acc;
 }
 
+// Right-looking blocked LU of nf matrices with partial pivoting.  Panels go in
+// pairs where a trailing part follows: the second panel's columns take the
+// first panel's update alone, the second panel's window rows of the columns
+// right of it take it while their U rows are solved, and the rest of the
+// trailing matrix takes both panels in one update of depth 2 NB, so it is
+// read and written once per 2 NB columns.
 void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, int *info) {
     hipStream_t st = c->stream;
     const size_t ldf = (size_t)n * n;
-    for (int k = 0; k < n; k += NB) {
+    auto panel = [&](int k, int nb) {
+        const int rows = n - k;
+        if (rows <= kPanelThreads)
+            k_panel_reg<1><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 2 * kPanelThreads)
+            k_panel_reg<2><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 4 * kPanelThreads)
+            k_panel_reg<4><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 8 * kPanelThreads)
+            k_panel_reg<8><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else
+            k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+    };
+    int k = 0;
+    while (k < n) {
         const int nb = std::min(NB, n - k);
-        k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        panel(k, nb);
         if (n > nb)
-            k_swap_trsm<<<dim3((n - nb + 127) / 128, 1, nf), 128, 0, st>>>(n, k, nb, A, ldf, swaps);
+            k_swap_trsm<<<dim3((n - nb + 127) / 128, 1, nf), 128, 0, st>>>(n, k, nb, A, ldf, swaps, kSwapAll);
         const int rest = n - k - nb;
-        if (rest > 0) {
-            const int tiles = (rest + 63) / 64;
-            k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, nb, A, ldf);
+        if (rest > NB) {
+            // second panel of the pair, with a trailing part after it
+            const int k2 = k + NB, rest2 = rest - NB;
+            k_gemm_update<<<dim3((rest + 63) / 64, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
+            panel(k2, NB);
+            k_swap_trsm<<<dim3((rest2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapRight);
+            k_swap_trsm<<<dim3((k2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapLeft);
+            const int tiles = (rest2 + 63) / 64;
+            k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, 2 * NB, n, A, ldf);
+            k += 2 * NB;
+        } else {
+            if (rest > 0) {
+                const int tiles = (rest + 63) / 64;
+                k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, nb, n, A, ldf);
+            }
+            k += NB;
         }
     }
     PXH_HIP(hipGetLastError());
