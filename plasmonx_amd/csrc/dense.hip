@@ -77,6 +77,28 @@ __device__ void net_permutation(int lane, int w0, int w, const int *pv, int *src
     if (lane == 0) *m_o = extra;
 }
 
+// a[dst[j]] = a[src[j]] for j < m, where no destination row is a source row
+// (a net permutation's rows below its window take old window rows): the
+// loads go in batches of 8 ahead of their stores, instead of one load-store
+// round trip per row.
+__device__ __forceinline__ void move_rows(double2 *a, const int *src, const int *dst, int m) {
+#pragma unroll
+    for (int j0 = 0; j0 < NB; j0 += 8) {
+        if (j0 >= m) break;
+        double2 t0 = a[src[j0]], t1 = a[src[min(j0 + 1, m - 1)]], t2 = a[src[min(j0 + 2, m - 1)]],
+                t3 = a[src[min(j0 + 3, m - 1)]], t4 = a[src[min(j0 + 4, m - 1)]], t5 = a[src[min(j0 + 5, m - 1)]],
+                t6 = a[src[min(j0 + 6, m - 1)]], t7 = a[src[min(j0 + 7, m - 1)]];
+        a[dst[j0]] = t0;
+        if (j0 + 1 < m) a[dst[j0 + 1]] = t1;
+        if (j0 + 2 < m) a[dst[j0 + 2]] = t2;
+        if (j0 + 3 < m) a[dst[j0 + 3]] = t3;
+        if (j0 + 4 < m) a[dst[j0 + 4]] = t4;
+        if (j0 + 5 < m) a[dst[j0 + 5]] = t5;
+        if (j0 + 6 < m) a[dst[j0 + 6]] = t6;
+        if (j0 + 7 < m) a[dst[j0 + 7]] = t7;
+    }
+}
+
 // Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
 // frequency.  Pivots (global row indices) go to piv[f][k..k+nb) and their net
 // permutation to swaps[f].  Row swaps are applied inside the panel only.  One
@@ -380,7 +402,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                 double2 *x = v[0];  // the sub-panel is written back; its registers are free
 #pragma unroll
                 for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[ps_src_w[j]] : make_double2(0, 0);
-                for (int jj = 0; jj < ps_m; ++jj) a[ps_dst[jj]] = a[ps_src[jj]];
+                move_rows(a, ps_src, ps_dst, ps_m);
 #pragma unroll
                 for (int j = 0; j < SUB; ++j)
                     if (j < sb) a[s0 + j] = x[j];
@@ -438,7 +460,7 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
     double2 x[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) x[i] = i < nb ? a[S.src_w[i]] : make_double2(0, 0);
-    for (int j = 0; j < S.m_o; ++j) a[S.dst_o[j]] = a[S.src_o[j]];
+    move_rows(a, S.src_o, S.dst_o, S.m_o);
     if (mode == kSwapRight)
         for (int q = 0; q < NB; ++q) {
             const double2 u = a[k - NB + q];
@@ -464,53 +486,106 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
         if (i < nb) a[k + i] = x[i];
 }
 
-// A22 -= L21 U12, 64 x 64 output tile per workgroup of 256 (4 x 4 per item).
-// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 16.
+// A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4_f64, FP64
+// throughout).  64 x 64 output tile per workgroup of 256; wave w takes the
+// 32 x 32 quarter (w & 1, w >> 1) as 2 x 2 blocks of 16 x 16.  Each block is
+// computed transposed, C^T = U^T L^T, so that a lane's accumulator elements
+// are 16 consecutive rows of one column of A (full 256-byte segments in the
+// read-modify-write); with complex operands split into parts,
+//   Re C^T += Ur^T Lr^T - Ui^T Li^T,   Im C^T += Ur^T Li^T + Ui^T Lr^T,
+// and one 16-byte load gives a lane both parts of its operand.  Operands come
+// straight from global memory (the L and U blocks are shared by many tiles
+// and stay in L2), 8 of the K columns loaded ahead of the products; the
+// products of one K step go block by block, so that eight independent
+// products separate two into the same accumulator.
+// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 8.
+typedef double d4v __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, int c_end, double2 *__restrict__ A,
                                                      size_t lda_f) {
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     const int r0 = k + nb + blockIdx.x * 64, c0 = k + nb + blockIdx.y * 64;
     if (r0 >= n || c0 >= c_end) return;
-    constexpr int KC = 16;  // k-chunk staged in LDS (34 KB per workgroup)
-    __shared__ double2 sL[64][KC + 1];
-    __shared__ double2 sU[KC][64 + 1];
-    const int tid = threadIdx.x;
-    const int tr = tid % 16, tc = tid / 16;
-    double2 acc[4][4];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0, 0);
-    for (int q0 = 0; q0 < nb; q0 += KC) {
-        const int kc = min(KC, nb - q0);
-        __syncthreads();
-        for (int e = tid; e < 64 * KC; e += 256) {
-            const int r = e % 64, q = e / 64;
-            sL[r][q] = (r0 + r < n && q < kc) ? Af[(r0 + r) + (size_t)n * (k + q0 + q)] : make_double2(0, 0);
-        }
-        for (int e = tid; e < KC * 64; e += 256) {
-            const int q = e % KC, c = e / KC;
-            sU[q][c] = (c0 + c < c_end && q < kc) ? Af[(k + q0 + q) + (size_t)n * (c0 + c)] : make_double2(0, 0);
-        }
-        __syncthreads();
-        for (int q = 0; q < kc; ++q) {
-            double2 l[4], u[4];
-            for (int a = 0; a < 4; ++a) l[a] = sL[tr + 16 * a][q];
-            for (int b = 0; b < 4; ++b) u[b] = sU[q][tc + 16 * b];
-            for (int a = 0; a < 4; ++a)
-                for (int b = 0; b < 4; ++b) {
-                    acc[a][b].x += l[a].x * u[b].x - l[a].y * u[b].y;
-                    acc[a][b].y += l[a].x * u[b].y + l[a].y * u[b].x;
-                }
-        }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int rw = r0 + 32 * (w & 1), cw = c0 + 32 * (w >> 1);
+    // operand rows / columns of this lane, clamped into the matrix (the
+    // products of clamped lanes land outside the stored range)
+    const double2 *lp[2], *up[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        lp[b] = Af + min(rw + 16 * b + i, n - 1) + (size_t)n * (k + kk);
+        up[b] = Af + (k + kk) + (size_t)n * min(cw + 16 * b + i, c_end - 1);
     }
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-            const int r = r0 + tr + 16 * a, c = c0 + tc + 16 * b;
-            if (r < n && c < c_end) {
-                Af[r + (size_t)n * c].x -= acc[a][b].x;
-                Af[r + (size_t)n * c].y -= acc[a][b].y;
-            }
+    d4v P[2][2], Q[2][2];  // [column block][row block]
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            P[cb][rb] = d4v{0, 0, 0, 0};
+            Q[cb][rb] = d4v{0, 0, 0, 0};
         }
+    constexpr int S = 2;  // K steps of 4 per chunk
+    double2 l[S][2], u[S][2];
+    auto load = [&](int kb, double2 (&lv)[S][2], double2 (&uv)[S][2]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                lv[s][b] = lp[b][(size_t)n * (kb + 4 * s)];
+                uv[s][b] = up[b][kb + 4 * s];
+            }
+    };
+    load(0, l, u);
+    for (int kb = 0; kb < nb; kb += 4 * S) {
+        double2 ln[S][2], un[S][2];
+        if (kb + 4 * S < nb) load(kb + 4 * S, ln, un);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].x, l[s][rb].x, P[cb][rb], 0, 0, 0);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].x, l[s][rb].y, Q[cb][rb], 0, 0, 0);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u[s][cb].y, l[s][rb].y, P[cb][rb], 0, 0, 0);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].y, l[s][rb].x, Q[cb][rb], 0, 0, 0);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                l[s][b] = ln[s][b];
+                u[s][b] = un[s][b];
+            }
+    }
+    // accumulator element v of a lane: C^T row lane / 16 + 4 v (a column of
+    // A), C^T column lane % 16 (a row of A)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rw + 16 * rb + i, c = cw + 16 * cb + kk + 4 * v;
+                if (r < n && c < c_end) {
+                    double2 &a = Af[r + (size_t)n * c];
+                    a.x -= P[cb][rb][v];
+                    a.y -= Q[cb][rb][v];
+                }
+            }
 }
 
 // Solve (LU) x = b in place, b already row-permuted, for the right-hand
