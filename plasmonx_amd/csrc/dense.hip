@@ -487,96 +487,89 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
 }
 
 // A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4_f64, FP64
-// throughout).  64 x 64 output tile per workgroup of 256; wave w takes the
-// 32 x 32 quarter (w & 1, w >> 1) as 2 x 2 blocks of 16 x 16.  Each block is
-// computed transposed, C^T = U^T L^T, so that a lane's accumulator elements
+// throughout).  Output tile of (32 BR) x (32 BC) per workgroup of 256; wave w
+// takes the quarter (w & 1, w >> 1) as BR x BC blocks of 16 x 16.  Each block
+// is computed transposed, C^T = U^T L^T, so that a lane's accumulator elements
 // are 16 consecutive rows of one column of A (full 256-byte segments in the
 // read-modify-write); with complex operands split into parts,
 //   Re C^T += Ur^T Lr^T - Ui^T Li^T,   Im C^T += Ur^T Li^T + Ui^T Lr^T,
 // and one 16-byte load gives a lane both parts of its operand.  Operands come
 // straight from global memory (the L and U blocks are shared by many tiles
-// and stay in L2), 8 of the K columns loaded ahead of the products; the
-// products of one K step go block by block, so that eight independent
-// products separate two into the same accumulator.
-// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 8.
+// and stay in L2), one K step of 4 loaded ahead of the products; the products
+// of one K step go block by block, so that 2 BR BC independent products
+// separate two into the same accumulator.
+// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 4.
 typedef double d4v __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void k_gemm_update(int n, int k, int nb, int c_end, double2 *__restrict__ A,
-                                                     size_t lda_f) {
+template <int BR, int BC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_update(
+    int n, int k, int nb, int c_end, double2 *__restrict__ A, size_t lda_f) {
+    constexpr int TR = 32 * BR, TC = 32 * BC;
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
-    const int r0 = k + nb + blockIdx.x * 64, c0 = k + nb + blockIdx.y * 64;
+    const int r0 = k + nb + blockIdx.x * TR, c0 = k + nb + blockIdx.y * TC;
     if (r0 >= n || c0 >= c_end) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = lane & 15, kk = lane >> 4;
-    const int rw = r0 + 32 * (w & 1), cw = c0 + 32 * (w >> 1);
+    const int rw = r0 + 16 * BR * (w & 1), cw = c0 + 16 * BC * (w >> 1);
     // operand rows / columns of this lane, clamped into the matrix (the
     // products of clamped lanes land outside the stored range)
-    const double2 *lp[2], *up[2];
+    const double2 *lp[BR], *up[BC];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        lp[b] = Af + min(rw + 16 * b + i, n - 1) + (size_t)n * (k + kk);
-        up[b] = Af + (k + kk) + (size_t)n * min(cw + 16 * b + i, c_end - 1);
-    }
-    d4v P[2][2], Q[2][2];  // [column block][row block]
+    for (int b = 0; b < BR; ++b) lp[b] = Af + min(rw + 16 * b + i, n - 1) + (size_t)n * (k + kk);
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int b = 0; b < BC; ++b) up[b] = Af + (k + kk) + (size_t)n * min(cw + 16 * b + i, c_end - 1);
+    d4v P[BC][BR], Q[BC][BR];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
+    for (int cb = 0; cb < BC; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < BR; ++rb) {
             P[cb][rb] = d4v{0, 0, 0, 0};
             Q[cb][rb] = d4v{0, 0, 0, 0};
         }
-    constexpr int S = 2;  // K steps of 4 per chunk
-    double2 l[S][2], u[S][2];
-    auto load = [&](int kb, double2 (&lv)[S][2], double2 (&uv)[S][2]) {
+    double2 l[BR], u[BC];
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+    for (int b = 0; b < BR; ++b) l[b] = lp[b][0];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                lv[s][b] = lp[b][(size_t)n * (kb + 4 * s)];
-                uv[s][b] = up[b][kb + 4 * s];
-            }
-    };
-    load(0, l, u);
-    for (int kb = 0; kb < nb; kb += 4 * S) {
-        double2 ln[S][2], un[S][2];
-        if (kb + 4 * S < nb) load(kb + 4 * S, ln, un);
+    for (int b = 0; b < BC; ++b) u[b] = up[b][0];
+    for (int kb = 0; kb < nb; kb += 4) {
+        double2 ln[BR], un[BC];
+        if (kb + 4 < nb) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+            for (int b = 0; b < BR; ++b) ln[b] = lp[b][(size_t)n * (kb + 4)];
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-                    P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].x, l[s][rb].x, P[cb][rb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-                    Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].x, l[s][rb].y, Q[cb][rb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-                    P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u[s][cb].y, l[s][rb].y, P[cb][rb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-                    Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[s][cb].y, l[s][rb].x, Q[cb][rb], 0, 0, 0);
+            for (int b = 0; b < BC; ++b) un[b] = up[b][kb + 4];
         }
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                l[s][b] = ln[s][b];
-                u[s][b] = un[s][b];
-            }
+            for (int rb = 0; rb < BR; ++rb)
+                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].x, l[rb].x, P[cb][rb], 0, 0, 0);
+#pragma unroll
+        for (int cb = 0; cb < BC; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < BR; ++rb)
+                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].x, l[rb].y, Q[cb][rb], 0, 0, 0);
+#pragma unroll
+        for (int cb = 0; cb < BC; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < BR; ++rb)
+                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u[cb].y, l[rb].y, P[cb][rb], 0, 0, 0);
+#pragma unroll
+        for (int cb = 0; cb < BC; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < BR; ++rb)
+                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].y, l[rb].x, Q[cb][rb], 0, 0, 0);
+#pragma unroll
+        for (int b = 0; b < BR; ++b) l[b] = ln[b];
+#pragma unroll
+        for (int b = 0; b < BC; ++b) u[b] = un[b];
     }
     // accumulator element v of a lane: C^T row lane / 16 + 4 v (a column of
     // A), C^T column lane % 16 (a row of A)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < BR; ++rb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = rw + 16 * rb + i, c = cw + 16 * cb + kk + 4 * v;
@@ -910,6 +903,12 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         else
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
     };
+    // trailing update of the rows and columns from k + depth
+    auto trailing = [&](int k, int depth) {
+        const int rest = n - k - depth;
+        const int t = (rest + 63) / 64;
+        k_gemm_update<2, 2><<<dim3(t, t, nf), 256, 0, st>>>(n, k, depth, n, A, ldf);
+    };
     int k = 0;
     while (k < n) {
         const int nb = std::min(NB, n - k);
@@ -920,18 +919,14 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         if (rest > NB) {
             // second panel of the pair, with a trailing part after it
             const int k2 = k + NB, rest2 = rest - NB;
-            k_gemm_update<<<dim3((rest + 63) / 64, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
+            k_gemm_update<4, 1><<<dim3((rest + 127) / 128, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
             panel(k2, NB);
             k_swap_trsm<<<dim3((rest2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapRight);
             k_swap_trsm<<<dim3((k2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapLeft);
-            const int tiles = (rest2 + 63) / 64;
-            k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, 2 * NB, n, A, ldf);
+            trailing(k, 2 * NB);
             k += 2 * NB;
         } else {
-            if (rest > 0) {
-                const int tiles = (rest + 63) / 64;
-                k_gemm_update<<<dim3(tiles, tiles, nf), 256, 0, st>>>(n, k, nb, n, A, ldf);
-            }
+            if (rest > 0) trailing(k, nb);
             k += NB;
         }
     }
