@@ -217,6 +217,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                                                              SwapSet *__restrict__ swaps, int *__restrict__ info) {
     constexpr int SUB = NB / R < 8 ? NB / R : 8;
     constexpr int W = kPanelThreads / 64;
+    constexpr int kUpdUnroll = R <= 8 ? 4 : 1;  // L loads in flight per row in the rank update
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
     const int tid = threadIdx.x, lane = tid % 64, wid = tid / 64;
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             for (int i = 0; i < R; ++i) {
                 const int r = k + tid + kPanelThreads * i;
                 if (r >= s0 && r < n) {
-#pragma unroll 4
+#pragma unroll kUpdUnroll
                     for (int q = 0; q < done; ++q) {
                         const double2 l = Af[r + (size_t)n * (k + q)];
 #pragma unroll
@@ -900,6 +901,10 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
             k_panel_reg<4><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
         else if (rows <= 8 * kPanelThreads)
             k_panel_reg<8><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 12 * kPanelThreads)
+            k_panel_reg<12><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 16 * kPanelThreads)
+            k_panel_reg<16><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
         else
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
     };
