@@ -77,31 +77,41 @@ __device__ void net_permutation(int lane, int w0, int w, const int *pv, int *src
     if (lane == 0) *m_o = extra;
 }
 
-// a[dst[j]] = a[src[j]] for j < m, where no destination row is a source row
-// (a net permutation's rows below its window take old window rows): the
+// a[ld dst[j]] = a[ld src[j]] for j < m, where no destination row is a source
+// row (a net permutation's rows below its window take old window rows): the
 // loads go in batches of 8 ahead of their stores, instead of one load-store
 // round trip per row.
-__device__ __forceinline__ void move_rows(double2 *a, const int *src, const int *dst, int m) {
+__device__ __forceinline__ void move_rows(double2 *a, size_t ld, const int *src, const int *dst, int m) {
 #pragma unroll
     for (int j0 = 0; j0 < NB; j0 += 8) {
         if (j0 >= m) break;
-        double2 t0 = a[src[j0]], t1 = a[src[min(j0 + 1, m - 1)]], t2 = a[src[min(j0 + 2, m - 1)]],
-                t3 = a[src[min(j0 + 3, m - 1)]], t4 = a[src[min(j0 + 4, m - 1)]], t5 = a[src[min(j0 + 5, m - 1)]],
-                t6 = a[src[min(j0 + 6, m - 1)]], t7 = a[src[min(j0 + 7, m - 1)]];
-        a[dst[j0]] = t0;
-        if (j0 + 1 < m) a[dst[j0 + 1]] = t1;
-        if (j0 + 2 < m) a[dst[j0 + 2]] = t2;
-        if (j0 + 3 < m) a[dst[j0 + 3]] = t3;
-        if (j0 + 4 < m) a[dst[j0 + 4]] = t4;
-        if (j0 + 5 < m) a[dst[j0 + 5]] = t5;
-        if (j0 + 6 < m) a[dst[j0 + 6]] = t6;
-        if (j0 + 7 < m) a[dst[j0 + 7]] = t7;
+        double2 t0 = a[ld * src[j0]], t1 = a[ld * src[min(j0 + 1, m - 1)]], t2 = a[ld * src[min(j0 + 2, m - 1)]],
+                t3 = a[ld * src[min(j0 + 3, m - 1)]], t4 = a[ld * src[min(j0 + 4, m - 1)]], t5 = a[ld * src[min(j0 + 5, m - 1)]],
+                t6 = a[ld * src[min(j0 + 6, m - 1)]], t7 = a[ld * src[min(j0 + 7, m - 1)]];
+        a[ld * dst[j0]] = t0;
+        if (j0 + 1 < m) a[ld * dst[j0 + 1]] = t1;
+        if (j0 + 2 < m) a[ld * dst[j0 + 2]] = t2;
+        if (j0 + 3 < m) a[ld * dst[j0 + 3]] = t3;
+        if (j0 + 4 < m) a[ld * dst[j0 + 4]] = t4;
+        if (j0 + 5 < m) a[ld * dst[j0 + 5]] = t5;
+        if (j0 + 6 < m) a[ld * dst[j0 + 6]] = t6;
+        if (j0 + 7 < m) a[ld * dst[j0 + 7]] = t7;
     }
 }
 
-// Panel factorisation of columns [k, k+nb) over rows [k, n), one workgroup per
-// frequency.  Pivots (global row indices) go to piv[f][k..k+nb) and their net
-// permutation to swaps[f].  Row swaps are applied inside the panel only.  One
+// Layout of the factorisation.  The matrices A(w) are assembled column-major
+// (element (i, j) at i + n j).  The kernels below factorise B = A^T, i.e. the
+// same memory read row-major (element (r, c) of B at c + n r), with row
+// interchanges on B: P B = L U.  A row of B is then contiguous, so the
+// interchanges, which move whole rows, move contiguous segments.  In A's
+// terms the memory ends up holding U^T in the lower triangle (with the
+// diagonal) and L^T (unit diagonal) strictly above it, and
+//   A x = b  <=>  U^T (L^T (P x)) = b,
+// solved by k_lu_solve and the scatter x[perm[i]] = (P x)[i].
+//
+// Panel factorisation of columns [k, k+nb) of B over rows [k, n), one
+// workgroup per frequency.  Pivots (global row indices) go to piv[f][k..k+nb)
+// and their net permutation to swaps[f].  Row swaps are applied inside the panel only.  One
 // pass over the rows per column: scale column c, update the remaining panel
 // columns with row c (staged in LDS), and find the pivot candidates of column
 // c + 1 on the way.
@@ -120,7 +130,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
     double best = -1.0;
     int bi = k;
     for (int r = k + tid; r < n; r += kPanelThreads) {
-        const double v = cabs2(Af[r + (size_t)n * k]);
+        const double v = cabs2(Af[k + (size_t)n * r]);
         if (v > best) {
             best = v;
             bi = r;
@@ -153,7 +163,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
             // singular column: nothing to eliminate; candidates of the next
             if (c + 1 < k + nb)
                 for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
-                    const double v = cabs2(Af[r + (size_t)n * (c + 1)]);
+                    const double v = cabs2(Af[(c + 1) + (size_t)n * r]);
                     if (v > best) {
                         best = v;
                         bi = r;
@@ -164,23 +174,23 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
         // swap rows c and p inside the panel, row c of the remaining columns to LDS
         if (p != c)
             for (int cc = k + tid; cc < k + nb; cc += kPanelThreads) {
-                const double2 t = Af[c + (size_t)n * cc];
-                Af[c + (size_t)n * cc] = Af[p + (size_t)n * cc];
-                Af[p + (size_t)n * cc] = t;
+                const double2 t = Af[cc + (size_t)n * c];
+                Af[cc + (size_t)n * c] = Af[cc + (size_t)n * p];
+                Af[cc + (size_t)n * p] = t;
             }
         __syncthreads();
-        for (int cc = c + 1 + tid; cc < k + nb; cc += kPanelThreads) urow[cc - k] = Af[c + (size_t)n * cc];
+        for (int cc = c + 1 + tid; cc < k + nb; cc += kPanelThreads) urow[cc - k] = Af[cc + (size_t)n * c];
         __syncthreads();
         const double2 d = Af[c + (size_t)n * c];
         for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
-            const double2 l = cdiv(Af[r + (size_t)n * c], d);
-            Af[r + (size_t)n * c] = l;
+            const double2 l = cdiv(Af[c + (size_t)n * r], d);
+            Af[c + (size_t)n * r] = l;
             for (int cc = c + 1; cc < k + nb; ++cc) {
                 const double2 pr = cmul(l, urow[cc - k]);
-                double2 a = Af[r + (size_t)n * cc];
+                double2 a = Af[cc + (size_t)n * r];
                 a.x -= pr.x;
                 a.y -= pr.y;
-                Af[r + (size_t)n * cc] = a;
+                Af[cc + (size_t)n * r] = a;
                 if (cc == c + 1) {
                     const double v = cabs2(a);
                     if (v > best) {
@@ -238,13 +248,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             const int r = k + tid + kPanelThreads * i;
 #pragma unroll
             for (int j = 0; j < SUB; ++j)
-                v[i][j] = (r < n && j < sb) ? Af[r + (size_t)n * (s0 + j)] : make_double2(0, 0);
+                v[i][j] = (r < n && j < sb) ? Af[(s0 + j) + (size_t)n * r] : make_double2(0, 0);
         }
         if (done > 0) {
             // U rows [k, s0): forward substitution with the unit lower block
             for (int e = tid; e < done * done; e += kPanelThreads) {
                 const int rr = e % done, cc = e / done;
-                Lw[rr][cc] = Af[(k + rr) + (size_t)n * (k + cc)];
+                Lw[rr][cc] = Af[(k + cc) + (size_t)n * (k + rr)];
             }
             if (tid < done)
 #pragma unroll
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                 if (r >= s0 && r < n) {
 #pragma unroll kUpdUnroll
                     for (int q = 0; q < done; ++q) {
-                        const double2 l = Af[r + (size_t)n * (k + q)];
+                        const double2 l = Af[(k + q) + (size_t)n * r];
 #pragma unroll
                         for (int j = 0; j < SUB; ++j) {
                             const double2 u = Uw[q][j];
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             if (r < n)
 #pragma unroll
                 for (int j = 0; j < SUB; ++j)
-                    if (j < sb) Af[r + (size_t)n * (s0 + j)] = v[i][j];
+                    if (j < sb) Af[(s0 + j) + (size_t)n * r] = v[i][j];
         }
         __syncthreads();
         // its interchanges on the other columns of the panel
@@ -399,14 +409,14 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             __syncthreads();
             if (tid < nb - sb) {
                 const int col = tid < done ? k + tid : k + tid + sb;
-                double2 *a = Af + (size_t)n * col;
+                double2 *a = Af + col;
                 double2 *x = v[0];  // the sub-panel is written back; its registers are free
 #pragma unroll
-                for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[ps_src_w[j]] : make_double2(0, 0);
-                move_rows(a, ps_src, ps_dst, ps_m);
+                for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[(size_t)n * ps_src_w[j]] : make_double2(0, 0);
+                move_rows(a, n, ps_src, ps_dst, ps_m);
 #pragma unroll
                 for (int j = 0; j < SUB; ++j)
-                    if (j < sb) a[s0 + j] = x[j];
+                    if (j < sb) a[(size_t)n * (s0 + j)] = x[j];
             }
             __syncthreads();
         }
@@ -445,26 +455,26 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
     if (trsm)
         for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
             const int r = e % NB, c = e / NB;
-            L[r][c] = Af[(k + r) + (size_t)n * (k + c)];
+            L[r][c] = Af[(k + c) + (size_t)n * (k + r)];
         }
     if (mode == kSwapRight)
         for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
             const int r = e % NB, q = e / NB;
-            Lp[r][q] = Af[S.src_w[r] + (size_t)n * (k - NB + q)];
+            Lp[r][q] = Af[(k - NB + q) + (size_t)n * S.src_w[r]];
         }
     __syncthreads();
     const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = mode == kSwapAll ? n - nb : mode == kSwapRight ? n - k - nb : k;
     if (cidx >= ncol) return;
     const int col = mode == kSwapRight ? k + nb + cidx : (mode == kSwapLeft || cidx < k) ? cidx : cidx + nb;
-    double2 *a = Af + (size_t)n * col;
+    double2 *a = Af + col;  // column col of B, rows n apart
     double2 x[NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) x[i] = i < nb ? a[S.src_w[i]] : make_double2(0, 0);
-    move_rows(a, S.src_o, S.dst_o, S.m_o);
+    for (int i = 0; i < NB; ++i) x[i] = i < nb ? a[(size_t)n * S.src_w[i]] : make_double2(0, 0);
+    move_rows(a, n, S.src_o, S.dst_o, S.m_o);
     if (mode == kSwapRight)
         for (int q = 0; q < NB; ++q) {
-            const double2 u = a[k - NB + q];
+            const double2 u = a[(size_t)n * (k - NB + q)];
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
                 const double2 p = cmul(Lp[i][q], u);
@@ -484,16 +494,17 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i)
-        if (i < nb) a[k + i] = x[i];
+        if (i < nb) a[(size_t)n * (k + i)] = x[i];
 }
 
-// A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4_f64, FP64
+// B22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4_f64, FP64
 // throughout).  Output tile of (32 BR) x (32 BC) per workgroup of 256; wave w
-// takes the quarter (w & 1, w >> 1) as BR x BC blocks of 16 x 16.  Each block
-// is computed transposed, C^T = U^T L^T, so that a lane's accumulator elements
-// are 16 consecutive rows of one column of A (full 256-byte segments in the
-// read-modify-write); with complex operands split into parts,
-//   Re C^T += Ur^T Lr^T - Ui^T Li^T,   Im C^T += Ur^T Li^T + Ui^T Lr^T,
+// takes the quarter (w & 1, w >> 1) as BR x BC blocks of 16 x 16.  In the
+// accumulator layout a lane holds 4 elements of one column of a block, and
+// the 16 lanes of a row group hold 16 consecutive columns: with B row-major
+// these are full 256-byte segments in the read-modify-write.  With complex
+// operands split into parts,
+//   Re C += Lr Ur - Li Ui,   Im C += Lr Ui + Li Ur,
 // and one 16-byte load gives a lane both parts of its operand.  Operands come
 // straight from global memory (the L and U blocks are shared by many tiles
 // and stay in L2), one K step of 4 loaded ahead of the products; the products
@@ -516,9 +527,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // products of clamped lanes land outside the stored range)
     const double2 *lp[BR], *up[BC];
 #pragma unroll
-    for (int b = 0; b < BR; ++b) lp[b] = Af + min(rw + 16 * b + i, n - 1) + (size_t)n * (k + kk);
+    for (int b = 0; b < BR; ++b) lp[b] = Af + (k + kk) + (size_t)n * min(rw + 16 * b + i, n - 1);
 #pragma unroll
-    for (int b = 0; b < BC; ++b) up[b] = Af + (k + kk) + (size_t)n * min(cw + 16 * b + i, c_end - 1);
+    for (int b = 0; b < BC; ++b) up[b] = Af + min(cw + 16 * b + i, c_end - 1) + (size_t)n * (k + kk);
     d4v P[BC][BR], Q[BC][BR];
 #pragma unroll
     for (int cb = 0; cb < BC; ++cb)
@@ -536,57 +547,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         double2 ln[BR], un[BC];
         if (kb + 4 < nb) {
 #pragma unroll
-            for (int b = 0; b < BR; ++b) ln[b] = lp[b][(size_t)n * (kb + 4)];
+            for (int b = 0; b < BR; ++b) ln[b] = lp[b][kb + 4];
 #pragma unroll
-            for (int b = 0; b < BC; ++b) un[b] = up[b][kb + 4];
+            for (int b = 0; b < BC; ++b) un[b] = up[b][(size_t)n * (kb + 4)];
         }
 #pragma unroll
         for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
             for (int rb = 0; rb < BR; ++rb)
-                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].x, l[rb].x, P[cb][rb], 0, 0, 0);
+                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].x, u[cb].x, P[cb][rb], 0, 0, 0);
 #pragma unroll
         for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
             for (int rb = 0; rb < BR; ++rb)
-                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].x, l[rb].y, Q[cb][rb], 0, 0, 0);
+                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].y, u[cb].x, Q[cb][rb], 0, 0, 0);
 #pragma unroll
         for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
             for (int rb = 0; rb < BR; ++rb)
-                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u[cb].y, l[rb].y, P[cb][rb], 0, 0, 0);
+                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-l[rb].y, u[cb].y, P[cb][rb], 0, 0, 0);
 #pragma unroll
         for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
             for (int rb = 0; rb < BR; ++rb)
-                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(u[cb].y, l[rb].x, Q[cb][rb], 0, 0, 0);
+                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].x, u[cb].y, Q[cb][rb], 0, 0, 0);
 #pragma unroll
         for (int b = 0; b < BR; ++b) l[b] = ln[b];
 #pragma unroll
         for (int b = 0; b < BC; ++b) u[b] = un[b];
     }
-    // accumulator element v of a lane: C^T row lane / 16 + 4 v (a column of
-    // A), C^T column lane % 16 (a row of A)
+    // accumulator element v of a lane: block row lane / 16 + 4 v, block
+    // column lane % 16
 #pragma unroll
     for (int cb = 0; cb < BC; ++cb)
 #pragma unroll
         for (int rb = 0; rb < BR; ++rb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = rw + 16 * rb + i, c = cw + 16 * cb + kk + 4 * v;
+                const int r = rw + 16 * rb + kk + 4 * v, c = cw + 16 * cb + i;
                 if (r < n && c < c_end) {
-                    double2 &a = Af[r + (size_t)n * c];
+                    double2 &a = Af[c + (size_t)n * r];
                     a.x -= P[cb][rb][v];
                     a.y -= Q[cb][rb][v];
                 }
             }
 }
 
-// Solve (LU) x = b in place, b already row-permuted, for the right-hand
-// sides selected by mask; one workgroup per (rhs, frequency).  Blocked by
-// kSolveBlock columns: the diagonal block is staged in LDS, wave 0 solves it
-// with the values in its lanes (shuffles), then all work-items update the
-// rows outside the block (one row each, the block's columns read together).
+// Solve U^T (L^T y) = b in place (column-major view: U^T is the lower
+// triangle with the diagonal, L^T the unit upper triangle) for the
+// right-hand sides selected by mask; one workgroup per (rhs, frequency).
+// Blocked by kSolveBlock columns: the diagonal block is staged in LDS, wave 0
+// solves it with the values in its lanes (shuffles), then all work-items
+// update the rows outside the block (one row each, the block's columns read
+// together).
 constexpr int kSolveBlock = 32, kSolveThreads = 1024;
 
 // x[r] -= sum_j A[r][c0 + j] xb[j] for the rows r handled by this work-item
@@ -623,13 +636,14 @@ __global__ __launch_bounds__(kSolveThreads) void k_lu_solve(int n, const double2
         }
         __syncthreads();
     };
-    // unit lower, blocks top to bottom
+    // lower with its diagonal, blocks top to bottom
     for (int c0 = 0; c0 < n; c0 += kSolveBlock) {
         const int cb = min(kSolveBlock, n - c0);
         stage(c0, cb);
         if (tid < 64) {
             double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
             for (int j = 0; j < cb; ++j) {
+                if (lane == j) v = cdiv(v, Ab[j][j]);
                 const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
                 if (lane > j && lane < cb) {
                     const double2 p = cmul(Ab[j][lane], xj);
@@ -646,14 +660,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_lu_solve(int n, const double2
         lu_solve_update(n, Af, c0, cb, c0 + cb, n, xb, x);
         __syncthreads();
     }
-    // upper, blocks bottom to top
+    // unit upper, blocks bottom to top
     for (int c1 = n; c1 > 0; c1 -= kSolveBlock) {
         const int c0 = max(0, c1 - kSolveBlock), cb = c1 - c0;
         stage(c0, cb);
         if (tid < 64) {
             double2 v = lane < cb ? x[c0 + lane] : make_double2(0, 0);
             for (int j = cb - 1; j >= 0; --j) {
-                if (lane == j) v = cdiv(v, Ab[j][j]);
                 const double2 xj = make_double2(__shfl(v.x, j, 64), __shfl(v.y, j, 64));
                 if (lane < j) {
                     const double2 p = cmul(Ab[j][lane], xj);
@@ -779,10 +792,9 @@ __global__ __launch_bounds__(256) void k_assemble_bem_ief(int n, const double *_
         }
 }
 
-// Copy right-hand sides of one frequency: X_f(:, k) = rhs_f(:, k) for k in pol_mask, 0 otherwise.
-// The row interchanges of a factorisation as one permutation: perm[f][i] is
-// the row of b that P b holds at i (the interchanges applied in order to the
-// identity).  The pivots are staged through LDS in chunks so the serial pass
+// The row interchanges of a factorisation of B as one permutation: perm[f][i]
+// is the row of B that P B holds at i (the interchanges applied in order to
+// the identity), so x = P^T z puts z[i] at x[perm[i]].  The pivots are staged through LDS in chunks so the serial pass
 // of work-item 0 touches LDS only; the permutation itself lives in LDS when
 // it fits.
 constexpr int kPermLds = 16384, kPivChunk = 2048;
@@ -813,16 +825,25 @@ __global__ __launch_bounds__(256) void k_perm_from_piv(int n, const int *__restr
         for (int i = threadIdx.x; i < n; i += 256) out[i] = sp[i];
 }
 
-// X[f][k] = P_f b_k for the selected right-hand sides (zero otherwise).
-__global__ void k_copy_rhs_perm(int n, int mask, const double2 *__restrict__ rhs, size_t ld_rhs_f,
-                                const int *__restrict__ perm, double2 *__restrict__ X, size_t ldx_f) {
+// Z[f][k] = b_k for the selected right-hand sides (zero otherwise).
+__global__ void k_copy_rhs(int n, int mask, const double2 *__restrict__ rhs, size_t ld_rhs_f,
+                           double2 *__restrict__ Z, size_t ldz_f) {
     const int f = blockIdx.z;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (r >= n) return;
     const bool on = (mask >> k) & 1;
-    X[ldx_f * f + (size_t)n * k + r] =
-        on ? rhs[ld_rhs_f * f + (size_t)n * k + perm[(size_t)n * f + r]] : make_double2(0, 0);
+    Z[ldz_f * f + (size_t)n * k + r] = on ? rhs[ld_rhs_f * f + (size_t)n * k + r] : make_double2(0, 0);
+}
+
+// x = P^T z: X[f][k][perm_f[i]] = Z[f][k][i].
+__global__ void k_scatter_rows(int n, const int *__restrict__ perm, const double2 *__restrict__ Z, size_t ldz_f,
+                               double2 *__restrict__ X, size_t ldx_f) {
+    const int f = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (r >= n) return;
+    X[ldx_f * f + (size_t)n * k + perm[(size_t)n * f + r]] = Z[ldz_f * f + (size_t)n * k + r];
 }
 
 __global__ void k_fill_one(int n, double2 *__restrict__ X, size_t ldx_f) {
@@ -1031,14 +1052,18 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     DevBuf<double2> &A = c->work_lu;
     DevBuf<int> info;
     DevBuf<SysParam> par;
-    DevBuf<double2> lam, X1;
+    DevBuf<double2> lam, X1, Z, Z1;
     A.alloc((size_t)n * n * chunk);
     c->work_piv.alloc((size_t)2 * n * chunk + sizeof(SwapSet) / sizeof(int) * chunk);
     int *piv = c->work_piv.p, *perm = c->work_piv.p + (size_t)n * chunk;
     SwapSet *swaps = reinterpret_cast<SwapSet *>(c->work_piv.p + (size_t)2 * n * chunk);
     info.alloc(chunk);
     const bool cc = bem && c->mesh.charge_constraint;
-    if (cc) X1.alloc((size_t)n * chunk);
+    Z.alloc((size_t)3 * n * chunk);
+    if (cc) {
+        X1.alloc((size_t)n * chunk);
+        Z1.alloc((size_t)n * chunk);
+    }
     int status = 0;
     for (int f0 = 0; f0 < n_freq; f0 += chunk) {
         const int nf = std::min(chunk, n_freq - f0);
@@ -1046,14 +1071,17 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
         PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, st));
         lu_factor(c, n, nf, A.p, piv, swaps, info.p);
         double2 *X = d_sol + (size_t)rhs_stride * f0;
+        // U^T (L^T z) = b, then x = P^T z
         k_perm_from_piv<<<nf, 256, 0, st>>>(n, piv, perm);
-        k_copy_rhs_perm<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(
-            n, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0, (size_t)rhs_fstride, perm, X, (size_t)rhs_stride);
-        k_lu_solve<<<dim3(3, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, X, (size_t)rhs_stride, n, pol_mask);
+        k_copy_rhs<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(
+            n, pol_mask, d_rhs_all + (size_t)rhs_fstride * f0, (size_t)rhs_fstride, Z.p, (size_t)3 * n);
+        k_lu_solve<<<dim3(3, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, Z.p, (size_t)3 * n, n, pol_mask);
+        k_scatter_rows<<<dim3((n + 255) / 256, 3, nf), 256, 0, st>>>(n, perm, Z.p, (size_t)3 * n, X,
+                                                                     (size_t)rhs_stride);
         if (cc) {
-            k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, X1.p, (size_t)n);
-            // P 1 = 1: the all-ones right-hand side needs no interchange
-            k_lu_solve<<<dim3(1, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, X1.p, (size_t)n, n, 1);
+            k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, Z1.p, (size_t)n);
+            k_lu_solve<<<dim3(1, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, Z1.p, (size_t)n, n, 1);
+            k_scatter_rows<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, perm, Z1.p, (size_t)n, X1.p, (size_t)n);
             k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, X, (size_t)rhs_stride, X1.p, (size_t)n);
         }
         PXH_HIP(hipGetLastError());
