@@ -926,6 +926,10 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
             k_panel_reg<12><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
         else if (rows <= 16 * kPanelThreads)
             k_panel_reg<16><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 20 * kPanelThreads)
+            k_panel_reg<20><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        else if (rows <= 24 * kPanelThreads)
+            k_panel_reg<24><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
         else
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
     };
@@ -1048,7 +1052,7 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     }
     const int chunk = (int)freq_chunk(c, n, n_freq);
     // the chunk's matrices and pivots stay with the context between calls
-    // (at most kDenseChunkBytes of HBM, held until the context is destroyed)
+    // when they take at most kKeepLuBytes (released at the end otherwise)
     DevBuf<double2> &A = c->work_lu;
     DevBuf<int> info;
     DevBuf<SysParam> par;
@@ -1090,6 +1094,10 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
         PXH_HIP(hipStreamSynchronize(st));
         for (int f = 0; f < nf; ++f)
             if (hi[f] != 0) status = PXH_ESINGULAR;
+    }
+    if (A.n * sizeof(double2) > kKeepLuBytes) {
+        A.release();
+        c->work_piv.release();
     }
     return status;
 }

@@ -616,23 +616,39 @@ __global__ void k_dense_t(int n, int nt, int fmu, int kern, const double4 *__res
 }
 
 // M(q rows) = K-bar T(q rows):  M(i, c) = sum_j K_ij (T(j, c) - T(i, c)).
-__global__ void k_kbar_dense(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
-                             const double *__restrict__ kij, int nv, const double *__restrict__ T,
-                             double *__restrict__ M) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (c >= nv) return;
-    const double tic = T[i + (size_t)nv * c];
-    double acc = 0.0;
-    for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) acc += kij[z] * (T[col[z] + (size_t)nv * c] - tic);
-    M[i + (size_t)nv * c] = acc;
+// M[i][c] = sum_z K_iz (B[z][c] - B[i][c]) for the q rows i, with B = T_qq
+// (c < n) or T_muq^T (c >= n), i.e. B[z][c] = T[c][z].  A workgroup of 256
+// computes a 64 x 64 tile: wave w takes rows i0 + w + 4k (its K-bar rows read
+// uniformly), its lanes consecutive columns (contiguous reads of T), and the
+// tile goes out through LDS with consecutive lanes on consecutive rows
+// (contiguous writes of column-major M).
+__global__ __launch_bounds__(256) void k_kbar_dense(int n, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, const double *__restrict__ kij,
+                                                    int nv, const double *__restrict__ T, double *__restrict__ M) {
+    __shared__ double tile[64][65];  // [row][column]
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int c0 = blockIdx.x * 64, i0 = blockIdx.y * 64;
+    const int c = c0 + lane;
+    for (int k = 0; k < 16; ++k) {
+        const int ii = w + 4 * k, i = i0 + ii;
+        double acc = 0.0;
+        if (i < n && c < nv) {
+            const double tic = T[c + (size_t)nv * i];
+            for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) acc += kij[z] * (T[c + (size_t)nv * col[z]] - tic);
+        }
+        tile[ii][lane] = acc;
+    }
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) {
+        const int cc = w + 4 * k, i = i0 + lane;
+        if (i < n && c0 + cc < nv) M[i + (size_t)nv * (c0 + cc)] = tile[lane][cc];
+    }
 }
 
 __global__ void k_copy_mu_rows(int n, int nv, const double *__restrict__ T, double *__restrict__ M) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;  // mu row index 0..3n-1
-    if (c >= nv) return;
-    M[(n + r) + (size_t)nv * c] = T[(n + r) + (size_t)nv * c];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // mu row index 0..3n-1
+    if (r >= 3 * n) return;
+    for (int c = blockIdx.y; c < nv; c += gridDim.y) M[(n + r) + (size_t)nv * c] = T[(n + r) + (size_t)nv * c];
 }
 
 template <typename R, int KV, bool FMU>
@@ -986,11 +1002,12 @@ void constant_matrix_atoms(pxh_ctx *c, double *d_m) {
     T.alloc((size_t)nv * nv);
     dense_t_atoms(c, T.p, false);
     PXH_HIP(hipMemsetAsync(d_m, 0, sizeof(double) * (size_t)nv * nv, c->stream));
+    const int gy = std::min(nv, 65535);
     if (!A.hetero)
-        k_kbar_dense<<<dim3((nv + 127) / 128, n), 128, 0, c->stream>>>(n, A.kb_rowptr.p, A.kb_col.p, A.kb_kij.p, nv,
-                                                                       T.p, d_m);
+        k_kbar_dense<<<dim3((nv + 63) / 64, (n + 63) / 64), 256, 0, c->stream>>>(n, A.kb_rowptr.p, A.kb_col.p,
+                                                                                  A.kb_kij.p, nv, T.p, d_m);
     if (A.model == PXH_WFQFMU)
-        k_copy_mu_rows<<<dim3((nv + 127) / 128, 3 * n), 128, 0, c->stream>>>(n, nv, T.p, d_m);
+        k_copy_mu_rows<<<dim3((3 * n + 127) / 128, gy), 128, 0, c->stream>>>(n, nv, T.p, d_m);
     PXH_HIP(hipGetLastError());
     PXH_HIP(hipStreamSynchronize(c->stream));
 }

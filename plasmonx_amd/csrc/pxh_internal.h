@@ -33,8 +33,9 @@ struct Error {
 // Maximum number of atom types handled by the pair kernels.
 constexpr int kMaxTypes = 8;
 // Frequencies whose dense matrices are factorised together (inversion).
-constexpr size_t kMaxFreqChunk = 64;                  // dense path: frequencies factorised together ...
-constexpr size_t kDenseChunkBytes = size_t(8) << 30;  // ... within this many bytes of matrices
+constexpr size_t kMaxFreqChunk = 128;                  // dense path: frequencies factorised together ...
+constexpr size_t kDenseChunkBytes = size_t(32) << 30;  // ... within this many bytes of matrices
+constexpr size_t kKeepLuBytes = size_t(4) << 30;       // their workspace kept with the context up to this
 constexpr size_t kKeepMconstBytes = size_t(2) << 30;  // dense constant matrices kept with the target up to this
 
 // Per type-pair constants of the interaction kernels (index ti + nt*tj).
