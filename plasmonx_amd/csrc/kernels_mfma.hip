@@ -387,18 +387,20 @@ void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam 
 namespace {
 
 // Time one 48-system product for each candidate split count (the pass over
-// the prepared first group plus the reduction of the split partials) and
-// keep the fastest.
+// the prepared first group plus the reduction of the split partials): a
+// warm-up round, then the best of three timed rounds per candidate.  The
+// first candidate (the heuristic) is kept unless another is more than 3 %
+// faster, so that timing noise does not move it.
 template <int K>
 int tune_splits(pxh_ctx *c, const std::vector<int> &cand, int iblocks, int jsteps, float rfar2, float tsc, int ns,
                 int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y, int ld) {
     hipEvent_t e0, e1;
     PXH_HIP(hipEventCreate(&e0));
     PXH_HIP(hipEventCreate(&e1));
-    int best = cand[0];
-    float best_ms = 1e30f;
-    for (int rep = 0; rep < 2; ++rep)  // the first round warms the caches
-        for (int sp : cand) {
+    std::vector<float> t(cand.size(), 1e30f);
+    for (int rep = 0; rep < 4; ++rep)  // the first round warms the caches
+        for (size_t q = 0; q < cand.size(); ++q) {
+            const int sp = cand[q];
             const int per = (jsteps + sp - 1) / sp;
             PXH_HIP(hipEventRecord(e0, c->stream));
             launch_sweep<K>(c, iblocks, sp, per, rfar2, tsc, ns, nsys, 0);
@@ -407,10 +409,14 @@ int tune_splits(pxh_ctx *c, const std::vector<int> &cand, int iblocks, int jstep
             PXH_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
             PXH_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (rep == 1 && ms < best_ms) {
-                best_ms = ms;
-                best = sp;
-            }
+            if (rep > 0) t[q] = std::min(t[q], ms);
+        }
+    int best = cand[0];
+    float best_ms = t[0] * 0.97f;
+    for (size_t q = 1; q < cand.size(); ++q)
+        if (t[q] < best_ms) {
+            best_ms = t[q];
+            best = cand[q];
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
