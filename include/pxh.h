@@ -107,7 +107,12 @@ int pxh_set_permittivity(pxh_ctx *ctx, int n_freq, const double *eps_w);
  * of algorithm%solve_dynamic_field (algorithm_module.f90:150-179) with its
  * three loops solve_w_inversion / solve_w_iter_mem / solve_w_iter_on_the_fly
  * (:189-465).
- *   method:      PXH_INVERSION | PXH_ITERATIVE | PXH_ITERATIVE_OTF;
+ *   method:      PXH_INVERSION | PXH_ITERATIVE | PXH_ITERATIVE_OTF
+ *                (PXH_INVERSION factorises up to 128 frequencies at once,
+ *                within 32 GB of matrices, half the free HBM and the
+ *                context's budget; the factorisation is of A^T, so its
+ *                partial pivoting picks along rows of A, a rounding-level
+ *                difference from ZGETRF);
  *   tol_abs:     absolute GMRES threshold, already scaled by sqrt(n_var) as
  *                input_module.f90:251 does; the true residual
  *                ||b - A x|| < tol_abs is required on return;
