@@ -27,7 +27,9 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
     const double d = b.x * b.x + b.y * b.y;
     return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
-__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+// Pivot magnitude |re| + |im|, as LAPACK's IZAMAX (via DCABS1) ranks the
+// candidates of ZGETRF.
+__device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 
 // The panel's row interchanges as one permutation of the rows they touch,
 // for the other columns: window row k + i takes the old row src_w[i]; rows
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
     double best = -1.0;
     int bi = k;
     for (int r = k + tid; r < n; r += kPanelThreads) {
-        const double v = cabs2(Af[k + (size_t)n * r]);
+        const double v = cabs1(Af[k + (size_t)n * r]);
         if (v > best) {
             best = v;
             bi = r;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
             // singular column: nothing to eliminate; candidates of the next
             if (c + 1 < k + nb)
                 for (int r = c + 1 + tid; r < n; r += kPanelThreads) {
-                    const double v = cabs2(Af[(c + 1) + (size_t)n * r]);
+                    const double v = cabs1(Af[(c + 1) + (size_t)n * r]);
                     if (v > best) {
                         best = v;
                         bi = r;
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
                 a.y -= pr.y;
                 Af[cc + (size_t)n * r] = a;
                 if (cc == c + 1) {
-                    const double v = cabs2(a);
+                    const double v = cabs1(a);
                     if (v > best) {
                         best = v;
                         bi = r;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                 for (int i = 0; i < R; ++i) {
                     const int r = k + tid + kPanelThreads * i;
                     if (r >= c && r < n) {
-                        const double a = cabs2(v[i][j]);
+                        const double a = cabs1(v[i][j]);
                         if (a > best) {
                             best = a;
                             bi = r;
@@ -375,7 +377,8 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                     // scale column c below the diagonal, update the columns right of it
                     const double2 d = prow[j];
                     const double dd = d.x * d.x + d.y * d.y;
-                    const double2 dinv = make_double2(d.x / dd, -d.y / dd);
+                    const double rdd = 1.0 / dd;
+                    const double2 dinv = make_double2(d.x * rdd, -d.y * rdd);
 #pragma unroll
                     for (int i = 0; i < R; ++i) {
                         const int r = k + tid + kPanelThreads * i;
