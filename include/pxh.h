@@ -148,6 +148,20 @@ int pxh_constant_matrix(pxh_ctx *ctx, double *m);
  * 329-428; rhs_module.f90:43-430; bem_module.f90:788-873). */
 int pxh_dynamic_rhs(pxh_ctx *ctx, double freq_au, double e0, int rhs_form, double *rhs);
 
+/* Field at n_points points for the variables of one frequency and one field
+ * direction i_pol (0, 1, 2) of an atomistic target with the Gaussian kernel:
+ * the incident e0 along i_pol plus the fields of the Gaussian charges and,
+ * for omegaFQFmu, dipoles (widths R_q, R_mu of each atom).  points_bohr
+ * 3 x n_points; variables n_var complex; field 3 x n_points complex
+ * (point-major).  |field| / e0 on a grid is the plasmonX_analysis field
+ * cube.  Replaces calculate_induced_field_at_point
+ * (wfq_module.f90:592-617, wfqfmu_module.f90:973-1003) with
+ * calculate_induced_field_q / _mu (matrix_module.f90:1140-1246) as the
+ * per-point loop of calculate_field_3d / _2d calls it
+ * (src/plasmonX_analysis/grid_module.f90:200-388). */
+int pxh_induced_field(pxh_ctx *ctx, int n_points, const double *points_bohr, const double *variables, int i_pol,
+                      double e0, double *field);
+
 /* Copy the message of the last failure into buf (NUL-terminated). */
 int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
 

@@ -72,6 +72,8 @@ void pxo_matvec_otf_rows(const pxo_atoms *a, double freq,
 void pxo_polar(const pxo_atoms *a, double e0, const pxo_cplx *var,
                pxo_cplx *polar);
 void pxo_results_row(double freq, const pxo_cplx *polar, double *res20);
+void pxo_induced_field(const pxo_atoms *a, int n_points, const double *points,
+                       const pxo_cplx *var, int i_pol, double e0, pxo_cplx *E);
 void pxo_neighbours(int n, int nt, const double *coord, const int *type,
                     const double *r0, int *nb_counts, int *nb_total);
 

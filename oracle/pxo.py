@@ -97,6 +97,20 @@ def matvec_otf(s, freq, alpha_t, x):
     return y
 
 
+def induced_field(s, points_bohr, var, i_pol, e0):
+    """Field (n_points, 3) complex at points (n_points, 3) bohr for the
+    variables var (n_var complex) of the field direction i_pol (0..2)."""
+    lib = load()
+    oa = OracleAtoms(s)
+    pts = np.ascontiguousarray(points_bohr, dtype=np.float64)
+    v = np.ascontiguousarray(var, dtype=np.complex128)
+    assert v.shape == (oa.n_var,)
+    E = np.zeros((pts.shape[0], 3), dtype=np.complex128)
+    lib.pxo_induced_field(oa.ref, ctypes.c_int(pts.shape[0]), P(pts), P(v), ctypes.c_int(i_pol), ctypes.c_double(e0),
+                          P(E))
+    return E
+
+
 def constant_matrix(s):
     lib = load()
     oa = OracleAtoms(s)

@@ -659,3 +659,55 @@ void pxo_gold_etchegoin(int polar, int n, const double *freq, pxo_cplx *out) {
         }
     }
 }
+
+/* Field at n_points points (bohr, 3 per point) for the variables var of one
+ * field direction: the incident e0 along i_pol plus, for every atom i with
+ * d = r_i - p and |d| > 0.1 bohr,
+ *   the field of its Gaussian charge (width R_q of atom i alone)
+ *     E -= (erf(|d|/R) - 2|d|/(sqrt(pi) R) exp(-|d|^2/R^2)) q_i d / |d|^3
+ *   (src/matrix/matrix_module.f90:1140-1172), and for omegaFQFmu the field
+ *   of its Gaussian dipole (width R_mu of atom i), E += T mu_i with
+ *     T_ab = (-3 d_a d_b / |d|^5 + delta_ab / |d|^3) fD1 + fD2 d_a d_b / |d|^2,
+ *     fD1 = erf(|d|/R) - 2|d|/(sqrt(pi) R) exp(-|d|^2/R^2),
+ *     fD2 = 4 / (sqrt(pi) R^3) exp(-|d|^2/R^2)
+ *   (:1174-1246),
+ * summed over the atoms as calculate_induced_field_at_point of
+ * src/wfq/wfq_module.f90:592-617 and src/wfq/wfqfmu_module.f90:973-1003
+ * (dipole of atom i at var[n + 3i + c]).  E is 3 x n_points, point-major. */
+void pxo_induced_field(const pxo_atoms *a, int n_points, const double *points, const pxo_cplx *var, int i_pol,
+                       double e0, pxo_cplx *E) {
+    const int n = a->n_atoms;
+#pragma omp parallel for
+    for (int p = 0; p < n_points; ++p) {
+        pxo_cplx e[3] = {0, 0, 0};
+        e[i_pol] = e0;
+        for (int i = 0; i < n; ++i) {
+            double d[3];
+            for (int c = 0; c < 3; ++c) d[c] = a->coord[3 * i + c] - points[3 * p + c];
+            const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (!(r > 1.0e-1)) continue;
+            const double r2 = r * r, r3 = r * r2;
+            const int t = a->type[i];
+            const double Rq = a->r_q[t];
+            const double fq = erf(r / Rq) - (2.0 / (sqrt(PI) * Rq)) * r * exp(-r2 / (Rq * Rq));
+            for (int c = 0; c < 3; ++c) e[c] -= fq * d[c] * var[i] / r3;
+            if (a->model == PXO_WFQFMU) {
+                const double R = a->r_mu[t], r5 = r2 * r3;
+                const double ex = exp(-r2 / (R * R));
+                const double f1 = erf(r / R) - ((2.0 * r) / (sqrt(PI) * R)) * ex;
+                const double f2 = (4.0 / (sqrt(PI) * R * R * R)) * ex;
+                const pxo_cplx *mu = var + n + 3 * i;
+                for (int u = 0; u < 3; ++u) {
+                    pxo_cplx s = 0;
+                    for (int v = 0; v < 3; ++v) {
+                        const double tuv = ((-3.0 * d[u] * d[v]) / r5 + (u == v ? 1.0 / r3 : 0.0)) * f1 +
+                                           f2 * (d[u] * d[v]) / r2;
+                        s += mu[v] * tuv;
+                    }
+                    e[u] += s;
+                }
+            }
+        }
+        for (int c = 0; c < 3; ++c) E[3 * p + c] = e[c];
+    }
+}

@@ -27,6 +27,7 @@ EXPORTS = [
     "pxh_set_permittivity", "pxh_solve_spectrum", "pxh_matvec", "pxh_constant_matrix",
     "pxh_dynamic_rhs", "pxh_last_error", "pxh_set_option", "pxh_get_stats",
     "pxh_server_create", "pxh_server_submit", "pxh_server_wait", "pxh_server_last_error", "pxh_server_destroy",
+    "pxh_induced_field",
 ]
 
 
@@ -58,6 +59,7 @@ def load():
     lib.pxh_set_permittivity.argtypes = [P, I, P]
     lib.pxh_solve_spectrum.argtypes = [P, I, P, D, I, I, I, D, I, I, P, P, P, P, P]
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
+    lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
     lib.pxh_constant_matrix.argtypes = [P, P]
     lib.pxh_dynamic_rhs.argtypes = [P, D, D, I, P]
     lib.pxh_last_error.argtypes = [P, ctypes.c_char_p, I]
@@ -184,6 +186,16 @@ class Context:
         y = np.zeros_like(x)
         self._check(self.lib.pxh_matvec(self._h, float(freq_au), int(i_freq), x.shape[0], _ptr(x), _ptr(y)))
         return y
+
+    def induced_field(self, points_bohr, variables, i_pol, e0):
+        """Field (n_points, 3) complex at points (n_points, 3) bohr for the
+        variables (n_var complex) of field direction i_pol (0, 1, 2)."""
+        pts = np.ascontiguousarray(points_bohr, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(variables, dtype=np.complex128).reshape(-1)
+        e = np.zeros((pts.shape[0], 3), dtype=np.complex128)
+        self._check(self.lib.pxh_induced_field(self._h, pts.shape[0], _ptr(pts), _ptr(v), int(i_pol), float(e0),
+                                               _ptr(e)))
+        return e
 
     def constant_matrix(self, n_var):
         m = np.zeros((n_var, n_var), dtype=np.float64, order="F")

@@ -1,0 +1,72 @@
+"""Induced field on a grid: the field-enhancement cubes of plasmonX_analysis.
+
+The reference's post-processing evaluates, at the points of a cubic grid
+around the target, the incident field plus the field of the solved Gaussian
+charges (and, for omegaFQFmu, dipoles) of one frequency and one field
+direction, and writes |E| / E0 as a Gaussian cube
+(src/plasmonX_analysis/grid_module.f90:64-88 grid, :200-242 field,
+src/plasmonX_analysis/plot_module.f90:496-562 cube).  The point sums run on
+the GPU through pxh_induced_field; this module builds the grid and writes
+the cube.
+"""
+import numpy as np
+
+from .model import TOBOHR
+
+
+def cubic_grid(coord_ang, n=(30, 30, 30), offset_ang=10.0):
+    """Grid of create_cubic_grid (grid_module.f90:64-88): the atoms' extent
+    +- offset on each axis, n points per axis.  Returns (min_ang, step_ang,
+    points_ang) with the points in cube order (x outermost, z innermost,
+    plot_module.f90:544-558), shape (nx*ny*nz, 3)."""
+    c = np.asarray(coord_ang, dtype=np.float64)
+    lo = c.min(axis=0) - offset_ang
+    hi = c.max(axis=0) + offset_ang
+    n = np.asarray(n, dtype=np.int64)
+    step = (hi - lo) / (n - 1)
+    i, j, k = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    pts = np.stack([lo[0] + i * step[0], lo[1] + j * step[1], lo[2] + k * step[2]], axis=-1).reshape(-1, 3)
+    return lo, step, pts
+
+
+def field_enhancement(ctx, points_ang, variables, i_pol, e0):
+    """|E(p)| / E0 at the points (Angstrom) for the variables (n_var complex)
+    of the field direction i_pol (0, 1, 2), as calculate_field_3d
+    (grid_module.f90:200-242)."""
+    e = ctx.induced_field(np.asarray(points_ang, dtype=np.float64) * TOBOHR, variables, i_pol, e0)
+    return np.sqrt((np.abs(e) ** 2).sum(axis=1)) / e0
+
+
+def fortran_e(x, w=14, d=6):
+    """Fortran Ew.d edit of a real: 0.dddddd E+ee, right-aligned in w."""
+    if x == 0.0 or not np.isfinite(x):
+        m, e = 0.0, 0
+    else:
+        e = int(np.floor(np.log10(abs(x)))) + 1
+        m = x / 10.0 ** e
+        if round(abs(m), d) >= 1.0:  # rounding carried into the exponent
+            m /= 10.0
+            e += 1
+    s = ("-" if m < 0 else "") + ("0.%0*d" % (d, int(round(abs(m) * 10 ** d)))) + "E" + ("%+03d" % e)
+    return s.rjust(w)
+
+
+def write_cube(path, atomic_numbers, coord_ang, lo_ang, step_ang, n, values):
+    """Gaussian cube in the layout of write_cube_file (plot_module.f90:496-562):
+    header in bohr, atoms, then the values x outermost / z innermost, six
+    per line (E14.6)."""
+    v = np.asarray(values, dtype=np.float64).reshape(n[0], n[1], n[2])
+    with open(path, "w") as f:
+        f.write("Generated by plasmonx_amd analysis\n\n")
+        f.write("%8d%12.6f%12.6f%12.6f\n" % (len(atomic_numbers), *(np.asarray(lo_ang) * TOBOHR)))
+        for a in range(3):
+            ax = [0.0, 0.0, 0.0]
+            ax[a] = step_ang[a] * TOBOHR
+            f.write("%8d%12.6f%12.6f%12.6f\n" % (n[a], *ax))
+        for z, c in zip(atomic_numbers, coord_ang):
+            f.write("%5d %10.6f%12.6f%12.6f%12.6f\n" % (int(z), 0.0, *(np.asarray(c) * TOBOHR)))
+        for i in range(n[0]):
+            for j in range(n[1]):
+                row = v[i, j]
+                for s in range(0, n[2], 6):
+                    f.write("".join(" " + fortran_e(x if abs(x) > 1e-99 else 0.0) for x in row[s:s + 6]) + "\n")

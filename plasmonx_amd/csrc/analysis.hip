@@ -1,0 +1,123 @@
+// Induced field at points: the post-processing sums of plasmonX_analysis.
+//
+// For the variables of one frequency and one field direction, the field at
+// a point p is the incident e0 along that direction plus, for every atom i
+// with d = r_i - p and |d| > 0.1 bohr,
+//   the field of its Gaussian charge (width R_q of atom i)
+//     E -= (erf(|d|/R) - 2|d|/(sqrt(pi) R) exp(-|d|^2/R^2)) q_i d / |d|^3
+//   (src/matrix/matrix_module.f90:1140-1172), and for omegaFQFmu of its
+//   Gaussian dipole (width R_mu of atom i): E += T mu_i,
+//     T_ab = (-3 d_a d_b / |d|^5 + delta_ab / |d|^3) fD1 + fD2 d_a d_b / |d|^2
+//   (:1174-1246),
+// summed over the atoms as src/wfq/wfq_module.f90:592-617 and
+// src/wfq/wfqfmu_module.f90:973-1003.  The reference evaluates one point at
+// a time on the host (grid_module.f90:200-242); here a work-item owns a
+// point and the atoms stream through LDS in tiles, FP64 throughout.
+#include "pxh_internal.h"
+
+namespace pxh {
+
+namespace {
+
+constexpr int kFieldTile = 256;
+constexpr double kInvSqrtPi = 0.56418958354775628;  // 1 / sqrt(pi)
+
+struct TypeRadii {
+    double rq[kMaxTypes], rmu[kMaxTypes];
+};
+
+__global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, const double *__restrict__ pts, int n,
+                                                              const double4 *__restrict__ pos, TypeRadii radii,
+                                                              int fmu, const double2 *__restrict__ var, int i_pol,
+                                                              double e0, double2 *__restrict__ E) {
+    __shared__ double4 sp[kFieldTile];
+    __shared__ double2 sq[kFieldTile];
+    __shared__ double2 smu[3][kFieldTile];
+    __shared__ double sr[2][kFieldTile];
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * kFieldTile + tid;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (p < n_points) {
+        px = pts[3 * (size_t)p];
+        py = pts[3 * (size_t)p + 1];
+        pz = pts[3 * (size_t)p + 2];
+    }
+    double2 ex = make_double2(i_pol == 0 ? e0 : 0.0, 0.0);
+    double2 ey = make_double2(i_pol == 1 ? e0 : 0.0, 0.0);
+    double2 ez = make_double2(i_pol == 2 ? e0 : 0.0, 0.0);
+    for (int t0 = 0; t0 < n; t0 += kFieldTile) {
+        const int j = t0 + tid;
+        if (j < n) {
+            const double4 a = pos[j];
+            const int t = (int)a.w;
+            sp[tid] = a;
+            sq[tid] = var[j];
+            sr[0][tid] = radii.rq[t];
+            sr[1][tid] = radii.rmu[t];
+            if (fmu)
+                for (int c = 0; c < 3; ++c) smu[c][tid] = var[n + 3 * (size_t)j + c];
+        }
+        __syncthreads();
+        const int m = min(kFieldTile, n - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            const double4 a = sp[jj];
+            const double dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double r = sqrt(r2);
+            if (!(r > 1.0e-1)) continue;
+            const double inv_r = 1.0 / r, inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
+            const double R = sr[0][jj];
+            const double fq = erf(r / R) - 2.0 * kInvSqrtPi / R * r * exp(-r2 / (R * R));
+            const double2 q = sq[jj];
+            const double cq = fq * inv_r3;
+            ex.x -= cq * dx * q.x;
+            ex.y -= cq * dx * q.y;
+            ey.x -= cq * dy * q.x;
+            ey.y -= cq * dy * q.y;
+            ez.x -= cq * dz * q.x;
+            ez.y -= cq * dz * q.y;
+            if (fmu) {
+                const double Rm = sr[1][jj];
+                const double em = exp(-r2 / (Rm * Rm));
+                const double f1 = erf(r / Rm) - 2.0 * kInvSqrtPi * r / Rm * em;
+                const double f2 = 4.0 * kInvSqrtPi / (Rm * Rm * Rm) * em;
+                // T = (f2 / r^2 - 3 f1 / r^5) d d^T + (f1 / r^3) I
+                const double g = f2 * inv_r2 - 3.0 * f1 * inv_r3 * inv_r2;
+                const double h = f1 * inv_r3;
+                const double2 mx = smu[0][jj], my = smu[1][jj], mz = smu[2][jj];
+                const double2 dm = make_double2(dx * mx.x + dy * my.x + dz * mz.x, dx * mx.y + dy * my.y + dz * mz.y);
+                ex.x += g * dx * dm.x + h * mx.x;
+                ex.y += g * dx * dm.y + h * mx.y;
+                ey.x += g * dy * dm.x + h * my.x;
+                ey.y += g * dy * dm.y + h * my.y;
+                ez.x += g * dz * dm.x + h * mz.x;
+                ez.y += g * dz * dm.y + h * mz.y;
+            }
+        }
+        __syncthreads();
+    }
+    if (p < n_points) {
+        E[3 * (size_t)p] = ex;
+        E[3 * (size_t)p + 1] = ey;
+        E[3 * (size_t)p + 2] = ez;
+    }
+}
+
+}  // namespace
+
+void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,
+                   double2 *d_E) {
+    const Atoms &A = c->atoms;
+    TypeRadii radii{};
+    for (int t = 0; t < A.nt; ++t) {
+        radii.rq[t] = A.rq[t];
+        radii.rmu[t] = A.model == PXH_WFQFMU ? A.rmu[t] : 1.0;
+    }
+    const int blocks = (n_points + kFieldTile - 1) / kFieldTile;
+    if (blocks > 0)
+        k_induced_field<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii,
+                                                              A.model == PXH_WFQFMU ? 1 : 0, d_var, i_pol, e0, d_E);
+    PXH_HIP(hipGetLastError());
+}
+
+}  // namespace pxh

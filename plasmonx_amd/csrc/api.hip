@@ -250,6 +250,28 @@ int pxh_solve_spectrum(pxh_ctx *c, int n_freq, const double *freq_au, double e0,
     return g != PXH_OK ? g : status;
 }
 
+int pxh_induced_field(pxh_ctx *c, int n_points, const double *points_bohr, const double *variables, int i_pol,
+                      double e0, double *field) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU, PXH_EINVAL, "no atomistic target set");
+        PXH_CHECK(c->atoms.kernel == PXH_GAUSSIAN, PXH_EINVAL, "the induced field needs the Gaussian kernel");
+        PXH_CHECK(n_points >= 0 && i_pol >= 0 && i_pol < 3, PXH_EINVAL, "bad point count or field direction");
+        PXH_CHECK(n_points == 0 || (points_bohr && variables && field), PXH_EINVAL, "null array");
+        if (n_points == 0) return;
+        const int nv = c->atoms.n_var();
+        pxh::DevBuf<double> dp;
+        pxh::DevBuf<double2> dv, de;
+        dp.upload(points_bohr, (size_t)3 * n_points, c->stream);
+        dv.upload(reinterpret_cast<const double2 *>(variables), (size_t)nv, c->stream);
+        de.alloc((size_t)3 * n_points);
+        pxh::induced_field(c, n_points, dp.p, dv.p, i_pol, e0, de.p);
+        PXH_HIP(hipMemcpyAsync(field, de.p, sizeof(double2) * 3 * (size_t)n_points, hipMemcpyDeviceToHost,
+                               c->stream));
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *x, double *y) {
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {

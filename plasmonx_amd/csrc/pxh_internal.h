@@ -207,6 +207,8 @@ struct SysParam {
 };
 // y[s] = A(omega_s) x[s] for the listed systems; x, y are [nsys_total][nv]
 // complex (double2); sys_list holds the systems to process (device).
+void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,
+                   double2 *d_E);
 void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld);
 void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,

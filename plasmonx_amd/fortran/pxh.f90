@@ -106,6 +106,17 @@ module pxh
          complex(c_double_complex), intent(out) :: rhs(*)
       end function
 
+      integer(c_int) function pxh_induced_field(ctx, n_points, points_bohr, variables, i_pol, e0, field) &
+            bind(C, name="pxh_induced_field")
+         import :: c_int, c_double, c_double_complex, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n_points, i_pol
+         real(c_double), intent(in) :: points_bohr(*)
+         complex(c_double_complex), intent(in) :: variables(*)
+         real(c_double), value :: e0
+         complex(c_double_complex), intent(out) :: field(*)
+      end function
+
       integer(c_int) function pxh_last_error(ctx, buf, len) bind(C, name="pxh_last_error")
          import :: c_int, c_char, c_ptr
          type(c_ptr), value :: ctx

@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Fixtures for the induced-field grid from the reference's analysis tests.
+
+Run in a checkout that has the reference tree (default /root/reference):
+
+    python tests/golden/make_analysis_fixtures.py [--ref /root/reference]
+
+Only data is read: each test's input archive (the solver's ``.info`` target
+description and the ``.freq`` complex variables of one frequency) and the
+``reference/*/*.cube`` grids that plasmonX_analysis printed for it.  Nothing
+from the reference is imported or executed.
+
+The cube holds |E(r)| / E0 on the cubic grid of
+src/plasmonX_analysis/grid_module.f90:64-88 (atoms' extent +- the offset,
+10 Angstrom here, n points per axis), x outermost and z innermost
+(plot_module.f90:544-558, E14.6), with E the incident field plus the field
+of the Gaussian charges (matrix_module.f90:1140-1172) and, for omegaFQFmu,
+of the Gaussian dipoles (:1174-1246), summed over atoms
+(wfq_module.f90:592-617, wfqfmu_module.f90:973-1003).
+
+Each fixture keeps the target (coordinates, per-type R_q / R_mu and the
+dynamic parameters of the .info), E0, the variables of each field direction,
+the grid (origin, step, points per axis, from the cube header) and a
+deterministic sample of the cube's values with their flat indices.
+"""
+import argparse
+import json
+import os
+import re
+import tarfile
+
+TOBOHR = 1.8897261254578281
+R0_BOHR = {"Ag": 2.885 * TOBOHR, "Au": 2.885 * TOBOHR, "Na": 6.9226100144600462}
+SYMBOL = {11: "Na", 47: "Ag", 79: "Au"}
+
+CASES = {
+    # case: (frequency tag, field directions with a cube)
+    "analysis_field_3d_x_wfq_sodium": ("1.5800000000", ["x"]),
+    "analysis_field_3d_x_y_z_wfqfmu_ag": ("3.0000000000", ["x", "y", "z"]),
+}
+SAMPLE = 600  # cube values kept per direction
+
+
+def read_info(text):
+    lines = text.splitlines()
+    head = {}
+    for ln in lines[:12]:
+        if ":" in ln:
+            k, v = ln.split(":", 1)
+            head[k.strip()] = v.strip()
+    n = int(head["NAtoms"])
+    i0 = next(i for i, ln in enumerate(lines) if ln.startswith("Input Geometry"))
+    symbols, coord, tidx = [], [], []
+    for ln in lines[i0 + 1:i0 + 1 + n]:
+        z, t, x, y, w = ln.split()
+        symbols.append(SYMBOL[int(float(z))])
+        tidx.append(int(t) - 1)
+        coord.append([float(x), float(y), float(w)])
+    j = next(i for i, ln in enumerate(lines) if ln.startswith("Parameters:"))
+    nt = int(lines[j + 1].split(":")[1])
+    types = []
+    for ln in lines[j + 2:j + 2 + nt]:
+        f = ln.split()
+        # fq: name chi eta r_q ; fqfmu: name chi eta alpha r_q r_mu
+        if len(f) == 4:
+            types.append({"name": f[0], "chi": float(f[1]), "eta": float(f[2]), "rq": float(f[3]), "rmu": 0.0})
+        else:
+            types.append({"name": f[0], "chi": float(f[1]), "eta": float(f[2]), "alpha": float(f[3]),
+                          "rq": float(f[4]), "rmu": float(f[5])})
+    k = next(i for i, ln in enumerate(lines) if ln.startswith("Dynamic parameters"))
+    ndyn = int(lines[k].split(":")[1])
+    dyn = [float(lines[k + 1 + q].split()[1]) for q in range(ndyn)]
+    return {"model": head["Model"], "kernel": head["Kernel"], "e0": float(head["Input |E|"]), "symbols": symbols,
+            "coord_ang": coord, "type_index": tidx, "types": types, "dynamic": dyn}
+
+
+def read_freq(text, n_var):
+    out, cur = {}, None
+    for ln in text.splitlines():
+        m = re.match(r"complex .*: ([xyz])\. freq", ln)
+        if m:
+            cur = m.group(1)
+            out[cur] = []
+            continue
+        f = ln.split()
+        if cur and len(f) == 3:
+            out[cur].append([float(f[1]), float(f[2])])
+    for d in out:
+        assert len(out[d]) == n_var, (d, len(out[d]), n_var)
+    return out
+
+
+def read_cube(text):
+    lines = text.splitlines()
+    n_at, *org = lines[2].split()
+    axes = [lines[3 + a].split() for a in range(3)]
+    npts = [int(a[0]) for a in axes]
+    step = [float(axes[a][1 + a]) for a in range(3)]
+    vals = " ".join(lines[6 + int(n_at):]).split()
+    vals = [float(v) for v in vals]
+    assert len(vals) == npts[0] * npts[1] * npts[2]
+    return {"origin_bohr": [float(o) for o in org], "step_bohr": step, "n": npts}, vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    here = os.path.dirname(os.path.abspath(__file__))
+    for case, (tag, dirs) in CASES.items():
+        tdir = os.path.join(a.ref, "tests", case)
+        with tarfile.open(os.path.join(tdir, case + ".tar.gz")) as tf:
+            info = read_info(tf.extractfile(case + ".info").read().decode())
+            n = len(info["symbols"])
+            n_var = 4 * n if info["model"] == "wfqfmu" else n
+            var = read_freq(tf.extractfile(f"{case}-{tag}.freq").read().decode(), n_var)
+        refdir = os.path.join(tdir, "reference")
+        sub = [d for d in os.listdir(refdir) if os.path.isdir(os.path.join(refdir, d))][0]
+        cubes, grid = {}, None
+        for d in dirs:
+            with open(os.path.join(refdir, sub, f"{case}-E{d}-{tag}.cube")) as f:
+                g, vals = read_cube(f.read())
+            grid = grid or g
+            # deterministic sample: evenly spaced flat indices plus the maximum
+            total = len(vals)
+            idx = sorted(set(list(range(0, total, max(1, total // SAMPLE))) + [max(range(total), key=vals.__getitem__)]))
+            cubes[d] = {"idx": idx, "values": [vals[i] for i in idx]}
+        fx = {"case": case, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
+              "symbols": info["symbols"], "coord_ang": info["coord_ang"], "types": info["types"],
+              "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
+              "grid_offset_ang": 10.0, "grid": grid,
+              "variables": {d: var[d] for d in dirs}, "cube": cubes}
+        with open(os.path.join(here, case + ".json"), "w") as f:
+            json.dump(fx, f)
+        print(case, n, "atoms", {d: len(cubes[d]["idx"]) for d in dirs}, "values")
+
+
+if __name__ == "__main__":
+    main()
