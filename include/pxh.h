@@ -162,6 +162,21 @@ int pxh_dynamic_rhs(pxh_ctx *ctx, double freq_au, double e0, int rhs_form, doubl
 int pxh_induced_field(pxh_ctx *ctx, int n_points, const double *points_bohr, const double *variables, int i_pol,
                       double e0, double *field);
 
+/* Plasmon density at n_points points for the variables of one frequency and
+ * one field direction of an atomistic target with the Gaussian kernel: the
+ * normalised Gaussians (width R_q) of the charges and, for omegaFQFmu, minus
+ * the derivatives along the dipoles of the Gaussians of width R_mu.
+ * points_bohr 3 x n_points; variables n_var complex; rho n_points complex;
+ * rho_q and rho_mu (the charge and dipole parts, rho = rho_q + rho_mu) are
+ * written when not NULL.  Replaces calculate_density_at_point and
+ * calculate_density_at_point_q_mu_separated (wfq_module.f90:625-648,
+ * wfqfmu_module.f90:1012-1081) with calculate_density_q / _mu
+ * (matrix_module.f90:1258-1324) as the per-point loop of
+ * calculate_density_3d / _2d calls it
+ * (src/plasmonX_analysis/grid_module.f90:248-301, 395-453). */
+int pxh_plasmon_density(pxh_ctx *ctx, int n_points, const double *points_bohr, const double *variables, double *rho,
+                        double *rho_q, double *rho_mu);
+
 /* Copy the message of the last failure into buf (NUL-terminated). */
 int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
 

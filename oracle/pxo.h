@@ -74,6 +74,9 @@ void pxo_polar(const pxo_atoms *a, double e0, const pxo_cplx *var,
 void pxo_results_row(double freq, const pxo_cplx *polar, double *res20);
 void pxo_induced_field(const pxo_atoms *a, int n_points, const double *points,
                        const pxo_cplx *var, int i_pol, double e0, pxo_cplx *E);
+void pxo_plasmon_density(const pxo_atoms *a, int n_points, const double *points,
+                         const pxo_cplx *var, pxo_cplx *rho, pxo_cplx *rho_q,
+                         pxo_cplx *rho_mu);
 void pxo_neighbours(int n, int nt, const double *coord, const int *type,
                     const double *r0, int *nb_counts, int *nb_total);
 

@@ -111,6 +111,19 @@ def induced_field(s, points_bohr, var, i_pol, e0):
     return E
 
 
+def plasmon_density(s, points_bohr, var):
+    """(rho, rho_q, rho_mu), each (n_points,) complex, at points (n_points, 3)
+    bohr for the variables var (n_var complex); rho = rho_q + rho_mu."""
+    lib = load()
+    oa = OracleAtoms(s)
+    pts = np.ascontiguousarray(points_bohr, dtype=np.float64)
+    v = np.ascontiguousarray(var, dtype=np.complex128)
+    assert v.shape == (oa.n_var,)
+    out = [np.zeros(pts.shape[0], dtype=np.complex128) for _ in range(3)]
+    lib.pxo_plasmon_density(oa.ref, ctypes.c_int(pts.shape[0]), P(pts), P(v), *[P(o) for o in out])
+    return tuple(out)
+
+
 def constant_matrix(s):
     lib = load()
     oa = OracleAtoms(s)

@@ -711,3 +711,41 @@ void pxo_induced_field(const pxo_atoms *a, int n_points, const double *points, c
         for (int c = 0; c < 3; ++c) E[3 * p + c] = e[c];
     }
 }
+
+/* Plasmon density at n_points points (bohr, 3 per point): for every atom i
+ * with d = r_i - p,
+ *   the normalised Gaussian of its charge,
+ *     rho_q += q_i exp(-|d|^2/R_q^2) / (pi^{3/2} R_q^3)
+ *   (src/matrix/matrix_module.f90:1258-1284), and for omegaFQFmu minus the
+ *   derivative of the Gaussian of R_mu along its dipole,
+ *     rho_mu -= 2 exp(-|d|^2/R_mu^2) / (pi^{3/2} R_mu^5) (mu_i . d)
+ *   (:1295-1324),
+ * summed over the atoms as src/wfq/wfq_module.f90:625-648 and
+ * src/wfq/wfqfmu_module.f90:1012-1081.  rho = rho_q + rho_mu; rho_q and
+ * rho_mu may be NULL. */
+void pxo_plasmon_density(const pxo_atoms *a, int n_points, const double *points, const pxo_cplx *var, pxo_cplx *rho,
+                         pxo_cplx *rho_q, pxo_cplx *rho_mu) {
+    const int n = a->n_atoms;
+    const double pi32 = PI * sqrt(PI);
+#pragma omp parallel for
+    for (int p = 0; p < n_points; ++p) {
+        pxo_cplx sq = 0, smu = 0;
+        for (int i = 0; i < n; ++i) {
+            double d[3];
+            for (int c = 0; c < 3; ++c) d[c] = a->coord[3 * i + c] - points[3 * p + c];
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const int t = a->type[i];
+            const double Rq = a->r_q[t];
+            sq += var[i] * (exp(-r2 / (Rq * Rq)) / (pi32 * Rq * Rq * Rq));
+            if (a->model == PXO_WFQFMU) {
+                const double R = a->r_mu[t];
+                const double g = exp(-r2 / (R * R)) / (pi32 * R * R * R);
+                const pxo_cplx *mu = var + n + 3 * i;
+                smu -= (2.0 * g / (R * R)) * (mu[0] * d[0] + mu[1] * d[1] + mu[2] * d[2]);
+            }
+        }
+        rho[p] = sq + smu;
+        if (rho_q) rho_q[p] = sq;
+        if (rho_mu) rho_mu[p] = smu;
+    }
+}

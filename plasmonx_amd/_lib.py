@@ -28,6 +28,7 @@ EXPORTS = [
     "pxh_dynamic_rhs", "pxh_last_error", "pxh_set_option", "pxh_get_stats",
     "pxh_server_create", "pxh_server_submit", "pxh_server_wait", "pxh_server_last_error", "pxh_server_destroy",
     "pxh_induced_field",
+    "pxh_plasmon_density",
 ]
 
 
@@ -60,6 +61,7 @@ def load():
     lib.pxh_solve_spectrum.argtypes = [P, I, P, D, I, I, I, D, I, I, P, P, P, P, P]
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
     lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
+    lib.pxh_plasmon_density.argtypes = [P, I, P, P, P, P, P]
     lib.pxh_constant_matrix.argtypes = [P, P]
     lib.pxh_dynamic_rhs.argtypes = [P, D, D, I, P]
     lib.pxh_last_error.argtypes = [P, ctypes.c_char_p, I]
@@ -196,6 +198,16 @@ class Context:
         self._check(self.lib.pxh_induced_field(self._h, pts.shape[0], _ptr(pts), _ptr(v), int(i_pol), float(e0),
                                                _ptr(e)))
         return e
+
+    def plasmon_density(self, points_bohr, variables, parts=False):
+        """Plasmon density (n_points,) complex at points (n_points, 3) bohr for
+        the variables (n_var complex); with parts=True, (rho, rho_q, rho_mu)."""
+        pts = np.ascontiguousarray(points_bohr, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(variables, dtype=np.complex128).reshape(-1)
+        out = [np.zeros(pts.shape[0], dtype=np.complex128) for _ in range(3 if parts else 1)]
+        extra = [_ptr(o) for o in out[1:]] + [None] * (3 - len(out))
+        self._check(self.lib.pxh_plasmon_density(self._h, pts.shape[0], _ptr(pts), _ptr(v), _ptr(out[0]), *extra))
+        return tuple(out) if parts else out[0]
 
     def constant_matrix(self, n_var):
         m = np.zeros((n_var, n_var), dtype=np.float64, order="F")

@@ -1,13 +1,14 @@
-"""Induced field on a grid: the field-enhancement cubes of plasmonX_analysis.
+"""Induced field and plasmon density on a grid: the cubes of plasmonX_analysis.
 
 The reference's post-processing evaluates, at the points of a cubic grid
 around the target, the incident field plus the field of the solved Gaussian
 charges (and, for omegaFQFmu, dipoles) of one frequency and one field
-direction, and writes |E| / E0 as a Gaussian cube
+direction, and writes |E| / E0 (or the real and imaginary parts of the
+plasmon density, grid_module.f90:248-301) as a Gaussian cube
 (src/plasmonX_analysis/grid_module.f90:64-88 grid, :200-242 field,
 src/plasmonX_analysis/plot_module.f90:496-562 cube).  The point sums run on
-the GPU through pxh_induced_field; this module builds the grid and writes
-the cube.
+the GPU through pxh_induced_field and pxh_plasmon_density; this module builds
+the grid and writes the cube.
 """
 import numpy as np
 
@@ -35,6 +36,14 @@ def field_enhancement(ctx, points_ang, variables, i_pol, e0):
     (grid_module.f90:200-242)."""
     e = ctx.induced_field(np.asarray(points_ang, dtype=np.float64) * TOBOHR, variables, i_pol, e0)
     return np.sqrt((np.abs(e) ** 2).sum(axis=1)) / e0
+
+
+def plasmon_density(ctx, points_ang, variables, parts=False):
+    """Plasmon density at the points (Angstrom) for the variables (n_var
+    complex), as calculate_density_3d (grid_module.f90:248-301); its real and
+    imaginary parts are the densityRe / densityIm cubes.  parts=True returns
+    (rho, rho_q, rho_mu) as the separate_q_mu option does."""
+    return ctx.plasmon_density(np.asarray(points_ang, dtype=np.float64) * TOBOHR, variables, parts)
 
 
 def fortran_e(x, w=14, d=6):

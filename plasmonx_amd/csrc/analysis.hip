@@ -1,4 +1,5 @@
-// Induced field at points: the post-processing sums of plasmonX_analysis.
+// Induced field and plasmon density at points: the post-processing sums of
+// plasmonX_analysis.
 //
 // For the variables of one frequency and one field direction, the field at
 // a point p is the incident e0 along that direction plus, for every atom i
@@ -103,7 +104,93 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
     }
 }
 
+// Plasmon density at a point p: for every atom i with d = r_i - p, the
+// normalised Gaussian of its charge, q_i exp(-|d|^2/R_q^2) / (pi^{3/2} R_q^3)
+// (src/matrix/matrix_module.f90:1258-1284), and for omegaFQFmu minus the
+// derivative of the Gaussian of R_mu along its dipole,
+// -2 exp(-|d|^2/R_mu^2) / (pi^{3/2} R_mu^5) (mu_i . d) (:1295-1324), summed
+// as src/wfq/wfq_module.f90:625-648 and src/wfq/wfqfmu_module.f90:1012-1081.
+// The per-atom constants are formed once per tile in LDS.  rho_q and rho_mu
+// (the two parts) are written when not null.
+__global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, const double *__restrict__ pts, int n,
+                                                                const double4 *__restrict__ pos, TypeRadii radii,
+                                                                int fmu, const double2 *__restrict__ var,
+                                                                double2 *__restrict__ rho, double2 *__restrict__ rho_q,
+                                                                double2 *__restrict__ rho_mu) {
+    __shared__ double4 sp[kFieldTile];
+    __shared__ double2 sq[kFieldTile];
+    __shared__ double2 smu[3][kFieldTile];
+    __shared__ double sc[4][kFieldTile];  // 1/R_q^2, 1/(pi^1.5 R_q^3), 1/R_mu^2, 2/(pi^1.5 R_mu^5)
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * kFieldTile + tid;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (p < n_points) {
+        px = pts[3 * (size_t)p];
+        py = pts[3 * (size_t)p + 1];
+        pz = pts[3 * (size_t)p + 2];
+    }
+    constexpr double kInvPi32 = 0.17958712212516656;  // 1 / pi^{3/2}
+    double2 aq = make_double2(0.0, 0.0), am = make_double2(0.0, 0.0);
+    for (int t0 = 0; t0 < n; t0 += kFieldTile) {
+        const int j = t0 + tid;
+        if (j < n) {
+            const double4 a = pos[j];
+            const int t = (int)a.w;
+            sp[tid] = a;
+            sq[tid] = var[j];
+            const double R = radii.rq[t];
+            sc[0][tid] = 1.0 / (R * R);
+            sc[1][tid] = kInvPi32 / (R * R * R);
+            if (fmu) {
+                const double Rm = radii.rmu[t];
+                sc[2][tid] = 1.0 / (Rm * Rm);
+                sc[3][tid] = 2.0 * kInvPi32 / (Rm * Rm * Rm * Rm * Rm);
+                for (int c = 0; c < 3; ++c) smu[c][tid] = var[n + 3 * (size_t)j + c];
+            }
+        }
+        __syncthreads();
+        const int m = min(kFieldTile, n - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            const double4 a = sp[jj];
+            const double dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double gq = sc[1][jj] * exp(-r2 * sc[0][jj]);
+            const double2 q = sq[jj];
+            aq.x += gq * q.x;
+            aq.y += gq * q.y;
+            if (fmu) {
+                const double gm = sc[3][jj] * exp(-r2 * sc[2][jj]);
+                const double2 mx = smu[0][jj], my = smu[1][jj], mz = smu[2][jj];
+                am.x -= gm * (dx * mx.x + dy * my.x + dz * mz.x);
+                am.y -= gm * (dx * mx.y + dy * my.y + dz * mz.y);
+            }
+        }
+        __syncthreads();
+    }
+    if (p < n_points) {
+        rho[p] = make_double2(aq.x + am.x, aq.y + am.y);
+        if (rho_q) rho_q[p] = aq;
+        if (rho_mu) rho_mu[p] = am;
+    }
+}
+
 }  // namespace
+
+void plasmon_density(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, double2 *d_rho,
+                     double2 *d_rho_q, double2 *d_rho_mu) {
+    const Atoms &A = c->atoms;
+    TypeRadii radii{};
+    for (int t = 0; t < A.nt; ++t) {
+        radii.rq[t] = A.rq[t];
+        radii.rmu[t] = A.model == PXH_WFQFMU ? A.rmu[t] : 1.0;
+    }
+    const int blocks = (n_points + kFieldTile - 1) / kFieldTile;
+    if (blocks > 0)
+        k_plasmon_density<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii,
+                                                                A.model == PXH_WFQFMU ? 1 : 0, d_var, d_rho, d_rho_q,
+                                                                d_rho_mu);
+    PXH_HIP(hipGetLastError());
+}
 
 void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,
                    double2 *d_E) {

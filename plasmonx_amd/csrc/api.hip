@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -256,7 +257,8 @@ int pxh_induced_field(pxh_ctx *c, int n_points, const double *points_bohr, const
     return guarded(c, [&] {
         PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU, PXH_EINVAL, "no atomistic target set");
         PXH_CHECK(c->atoms.kernel == PXH_GAUSSIAN, PXH_EINVAL, "the induced field needs the Gaussian kernel");
-        PXH_CHECK(n_points >= 0 && i_pol >= 0 && i_pol < 3, PXH_EINVAL, "bad point count or field direction");
+        PXH_CHECK(n_points >= 0 && n_points <= INT_MAX - 256 && i_pol >= 0 && i_pol < 3, PXH_EINVAL,
+                  "bad point count or field direction");
         PXH_CHECK(n_points == 0 || (points_bohr && variables && field), PXH_EINVAL, "null array");
         if (n_points == 0) return;
         const int nv = c->atoms.n_var();
@@ -268,6 +270,32 @@ int pxh_induced_field(pxh_ctx *c, int n_points, const double *points_bohr, const
         pxh::induced_field(c, n_points, dp.p, dv.p, i_pol, e0, de.p);
         PXH_HIP(hipMemcpyAsync(field, de.p, sizeof(double2) * 3 * (size_t)n_points, hipMemcpyDeviceToHost,
                                c->stream));
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pxh_plasmon_density(pxh_ctx *c, int n_points, const double *points_bohr, const double *variables, double *rho,
+                        double *rho_q, double *rho_mu) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU, PXH_EINVAL, "no atomistic target set");
+        PXH_CHECK(c->atoms.kernel == PXH_GAUSSIAN, PXH_EINVAL, "the plasmon density needs the Gaussian kernel");
+        PXH_CHECK(n_points >= 0 && n_points <= INT_MAX - 256, PXH_EINVAL, "bad point count");
+        PXH_CHECK(n_points == 0 || (points_bohr && variables && rho), PXH_EINVAL, "null array");
+        if (n_points == 0) return;
+        const int nv = c->atoms.n_var();
+        const size_t np = (size_t)n_points;
+        pxh::DevBuf<double> dp;
+        pxh::DevBuf<double2> dv, dr, dq, dm;
+        dp.upload(points_bohr, 3 * np, c->stream);
+        dv.upload(reinterpret_cast<const double2 *>(variables), (size_t)nv, c->stream);
+        dr.alloc(np);
+        if (rho_q) dq.alloc(np);
+        if (rho_mu) dm.alloc(np);
+        pxh::plasmon_density(c, n_points, dp.p, dv.p, dr.p, rho_q ? dq.p : nullptr, rho_mu ? dm.p : nullptr);
+        PXH_HIP(hipMemcpyAsync(rho, dr.p, sizeof(double2) * np, hipMemcpyDeviceToHost, c->stream));
+        if (rho_q) PXH_HIP(hipMemcpyAsync(rho_q, dq.p, sizeof(double2) * np, hipMemcpyDeviceToHost, c->stream));
+        if (rho_mu) PXH_HIP(hipMemcpyAsync(rho_mu, dm.p, sizeof(double2) * np, hipMemcpyDeviceToHost, c->stream));
         PXH_HIP(hipStreamSynchronize(c->stream));
     });
 }

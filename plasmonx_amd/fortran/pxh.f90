@@ -117,6 +117,17 @@ module pxh
          complex(c_double_complex), intent(out) :: field(*)
       end function
 
+      integer(c_int) function pxh_plasmon_density(ctx, n_points, points_bohr, variables, rho, rho_q, rho_mu) &
+            bind(C, name="pxh_plasmon_density")
+         import :: c_int, c_double, c_double_complex, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n_points
+         real(c_double), intent(in) :: points_bohr(*)
+         complex(c_double_complex), intent(in) :: variables(*)
+         complex(c_double_complex), intent(out) :: rho(*)
+         type(c_ptr), value :: rho_q, rho_mu
+      end function
+
       integer(c_int) function pxh_last_error(ctx, buf, len) bind(C, name="pxh_last_error")
          import :: c_int, c_char, c_ptr
          type(c_ptr), value :: ctx

@@ -16,7 +16,12 @@ src/plasmonX_analysis/grid_module.f90:64-88 (atoms' extent +- the offset,
 (plot_module.f90:544-558, E14.6), with E the incident field plus the field
 of the Gaussian charges (matrix_module.f90:1140-1172) and, for omegaFQFmu,
 of the Gaussian dipoles (:1174-1246), summed over atoms
-(wfq_module.f90:592-617, wfqfmu_module.f90:973-1003).
+(wfq_module.f90:592-617, wfqfmu_module.f90:973-1003).  The density cubes
+hold the real and imaginary parts of the plasmon density on the same grid:
+the normalised Gaussian of each charge (matrix_module.f90:1258-1284) and,
+for omegaFQFmu, minus the dipole's derivative of the Gaussian of R_mu
+(:1295-1324), summed over atoms (wfq_module.f90:625-648,
+wfqfmu_module.f90:1012-1040).
 
 Each fixture keeps the target (coordinates, per-type R_q / R_mu and the
 dynamic parameters of the .info), E0, the variables of each field direction,
@@ -34,9 +39,11 @@ R0_BOHR = {"Ag": 2.885 * TOBOHR, "Au": 2.885 * TOBOHR, "Na": 6.9226100144600462}
 SYMBOL = {11: "Na", 47: "Ag", 79: "Au"}
 
 CASES = {
-    # case: (frequency tag, field directions with a cube)
-    "analysis_field_3d_x_wfq_sodium": ("1.5800000000", ["x"]),
-    "analysis_field_3d_x_y_z_wfqfmu_ag": ("3.0000000000", ["x", "y", "z"]),
+    # case: (frequency tag, field directions with a cube, quantity)
+    "analysis_field_3d_x_wfq_sodium": ("1.5800000000", ["x"], "field"),
+    "analysis_field_3d_x_y_z_wfqfmu_ag": ("3.0000000000", ["x", "y", "z"], "field"),
+    "analysis_density_3d_x_wfq_sodium": ("1.5800000000", ["x"], "density"),
+    "analysis_density_3d_y_wfqfmu_au": ("3.0000000000", ["y"], "density"),
 }
 SAMPLE = 600  # cube values kept per direction
 
@@ -107,7 +114,7 @@ def main():
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
     here = os.path.dirname(os.path.abspath(__file__))
-    for case, (tag, dirs) in CASES.items():
+    for case, (tag, dirs, what) in CASES.items():
         tdir = os.path.join(a.ref, "tests", case)
         with tarfile.open(os.path.join(tdir, case + ".tar.gz")) as tf:
             info = read_info(tf.extractfile(case + ".info").read().decode())
@@ -118,14 +125,19 @@ def main():
         sub = [d for d in os.listdir(refdir) if os.path.isdir(os.path.join(refdir, d))][0]
         cubes, grid = {}, None
         for d in dirs:
-            with open(os.path.join(refdir, sub, f"{case}-E{d}-{tag}.cube")) as f:
-                g, vals = read_cube(f.read())
-            grid = grid or g
-            # deterministic sample: evenly spaced flat indices plus the maximum
-            total = len(vals)
-            idx = sorted(set(list(range(0, total, max(1, total // SAMPLE))) + [max(range(total), key=vals.__getitem__)]))
-            cubes[d] = {"idx": idx, "values": [vals[i] for i in idx]}
-        fx = {"case": case, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
+            parts = {"values": ""} if what == "field" else {"re": "-densityRe", "im": "-densityIm"}
+            read = {}
+            for key, suffix in parts.items():
+                with open(os.path.join(refdir, sub, f"{case}-E{d}-{tag}{suffix}.cube")) as f:
+                    g, read[key] = read_cube(f.read())
+                grid = grid or g
+            # deterministic sample: evenly spaced flat indices plus each part's extremes
+            total = len(next(iter(read.values())))
+            extra = [max(range(total), key=v.__getitem__) for v in read.values()]
+            extra += [min(range(total), key=v.__getitem__) for v in read.values()]
+            idx = sorted(set(list(range(0, total, max(1, total // SAMPLE))) + extra))
+            cubes[d] = {"idx": idx, **{k: [v[i] for i in idx] for k, v in read.items()}}
+        fx = {"case": case, "quantity": what, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
               "symbols": info["symbols"], "coord_ang": info["coord_ang"], "types": info["types"],
               "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
               "grid_offset_ang": 10.0, "grid": grid,

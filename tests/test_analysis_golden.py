@@ -1,11 +1,15 @@
-"""Induced field on a grid: the oracle against the reference's cubes.
+"""Induced field and plasmon density on a grid: the oracle against the
+reference's cubes.
 
 The fixtures (tests/golden/make_analysis_fixtures.py) hold the solved
 variables of the reference's analysis tests and a sample of the |E|/E0 cubes
 plasmonX_analysis printed for them (E14.6, i.e. 6 significant digits).  The
 grid is rebuilt here the reference's way (atoms' extent +- 10 Angstrom, 30
 points per axis) and checked against the cube header; the oracle's field at
-the sampled points must reproduce the printed values to their precision.
+the sampled points must reproduce the printed values to their precision,
+and so must its plasmon density (real and imaginary cubes).  The density of
+one atom is further pinned by its moments: it integrates to the charge and
+its first moment is q r_i + mu_i.
 """
 import json
 import os
@@ -19,6 +23,13 @@ from plasmonx_amd.model import TOBOHR, AtomSystem, AtomType
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = ["analysis_field_3d_x_wfq_sodium", "analysis_field_3d_x_y_z_wfqfmu_ag"]
+DENSITY_CASES = ["analysis_density_3d_x_wfq_sodium", "analysis_density_3d_y_wfqfmu_au"]
+
+
+def half_ulp(val):
+    """Half a unit of the sixth significant digit of E14.6 values."""
+    v = np.maximum(np.abs(val), 1e-300)
+    return 0.5 * 10.0 ** (np.floor(np.log10(v)) + 1 - 6)
 
 
 def load(case):
@@ -46,7 +57,7 @@ def grid_points(fx):
     return lo, step, pts
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", CASES + DENSITY_CASES)
 def test_grid_matches_cube_header(case):
     fx = load(case)
     lo, step, pts = grid_points(fx)
@@ -91,3 +102,41 @@ def test_oracle_field_far_limit():
     r = np.linalg.norm(rc)
     dip = (3.0 * rc * (rc @ p) / r ** 5 - p / r ** 3)
     assert np.allclose(e, dip, rtol=1e-3, atol=1e-3 * np.abs(dip).max())
+
+
+@pytest.mark.parametrize("case", DENSITY_CASES)
+def test_oracle_density_matches_reference_cube(case):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    _, _, pts = grid_points(fx)
+    for d, cube in fx["cube"].items():
+        var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+        idx = np.array(cube["idx"])
+        rho, rq, rm = pxo.plasmon_density(s, pts[idx] * TOBOHR, var)
+        assert np.allclose(rq + rm, rho, rtol=1e-13, atol=0)
+        for part, got in (("re", rho.real), ("im", rho.imag)):
+            ref = np.array(cube[part])
+            assert np.all(np.abs(got - ref) <= 1.01 * half_ulp(ref)), (d, part, np.max(np.abs(got - ref) / half_ulp(ref)))
+            assert [fortran_e(x) for x in got[:50]] == [fortran_e(x) for x in ref[:50]]
+
+
+@pytest.mark.parametrize("case", DENSITY_CASES)
+def test_oracle_density_moments_of_one_atom(case):
+    """The charge's Gaussian is normalised and the dipole term is -mu . grad
+    of the R_mu Gaussian: integrated on a fine grid, rho gives q and its first
+    moment q r_i + mu (the charge term alone when mu = 0)."""
+    fx = dict(load(case))
+    fx["coord_ang"] = [[0.31, -0.22, 0.13]]
+    fx["symbols"] = fx["symbols"][:1]
+    s = system_from_analysis_fixture(fx)
+    r = s.coord_bohr[0]
+    h = 0.3
+    ax = np.arange(-17.0, 17.0 + h / 2, h)
+    g = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3) + r
+    var = np.array([0.7 - 0.4j] + ([0.3, -0.25j, 0.5 + 0.1j] if s.model == "wfqfmu" else []))
+    rho = pxo.plasmon_density(s, g, var)[0]
+    w = h ** 3
+    assert abs(rho.sum() * w - var[0]) < 1e-8
+    mu = np.zeros(3, dtype=complex) if s.model != "wfqfmu" else var[1:4]
+    first = (g * rho[:, None]).sum(axis=0) * w
+    assert np.allclose(first, var[0] * r + mu, rtol=0, atol=1e-7)

@@ -1,10 +1,13 @@
-"""pxh_induced_field (HIP) against the oracle and the reference's cubes.
+"""pxh_induced_field and pxh_plasmon_density (HIP) against the oracle and
+the reference's cubes.
 
 Bars: the GPU field equals the oracle's to 1e-12 relative (FP64 on both
 sides, summation order aside); |E|/E0 at the sampled grid points equals the
 reference's printed cube values to their 6 digits; on a 15683-atom omegaFQFmu
 sphere (the C2 target) with random variables the field at 64 points matches
-the oracle, and the throughput of a full 100k-point grid is reported.
+the oracle, and the throughput of a full 100k-point grid is reported.  The
+density is held to the same bars (real and imaginary cubes, parts summing to
+the total).
 """
 import time
 
@@ -13,9 +16,9 @@ import pytest
 
 from oracle import pxo
 from plasmonx_amd import Context
-from plasmonx_amd.analysis import cubic_grid, field_enhancement
+from plasmonx_amd.analysis import cubic_grid, field_enhancement, plasmon_density
 from plasmonx_amd.model import TOBOHR
-from test_analysis_golden import CASES, load, system_from_analysis_fixture
+from test_analysis_golden import CASES, DENSITY_CASES, half_ulp, load, system_from_analysis_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -66,3 +69,48 @@ def test_field_large_target():
     ref = pxo.induced_field(s, pts[sel], var, 0, 1e-4)
     err = np.max(np.abs(got[sel] - ref)) / np.max(np.abs(ref))
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("case", DENSITY_CASES)
+def test_density_matches_oracle_and_cube(case):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    _, _, pts = cubic_grid(fx["coord_ang"], fx["grid"]["n"], fx["grid_offset_ang"])
+    with Context(0) as ctx:
+        ctx.set_atoms(s)
+        for d, cube in fx["cube"].items():
+            var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+            got, gq, gm = ctx.plasmon_density(pts * TOBOHR, var, parts=True)
+            ref, rq, rm = pxo.plasmon_density(s, pts * TOBOHR, var)
+            scale = np.max(np.abs(ref))
+            err = max(np.max(np.abs(a - b)) for a, b in ((got, ref), (gq, rq), (gm, rm))) / scale
+            print(f"{case} E{d}: {pts.shape[0]} points, density max relative difference to the oracle {err:.1e}",
+                  flush=True)
+            assert err < 1e-12
+            assert np.array_equal(ctx.plasmon_density(pts * TOBOHR, var), got)
+            idx = np.array(cube["idx"])
+            rho = plasmon_density(ctx, pts[idx], var)
+            for part, v in (("re", rho.real), ("im", rho.imag)):
+                val = np.array(cube[part])
+                assert np.all(np.abs(v - val) <= 1.01 * half_ulp(val))
+
+
+def test_density_large_target():
+    from plasmonx_amd.workloads import c2_silver_sphere
+    s, _ = c2_silver_sphere(n_freq=2)
+    rng = np.random.default_rng(6)
+    var = rng.normal(size=s.n_var) + 1j * rng.normal(size=s.n_var)
+    lo = s.coord_bohr.min(axis=0) - 10.0
+    hi = s.coord_bohr.max(axis=0) + 10.0
+    pts = lo + rng.random((100_000, 3)) * (hi - lo)
+    with Context(0) as ctx:
+        ctx.set_atoms(s)
+        ctx.plasmon_density(pts[:256], var)  # warm-up
+        t = time.perf_counter()
+        got = ctx.plasmon_density(pts, var)
+        dt = time.perf_counter() - t
+    print(f"15683-atom omegaFQFmu target, {pts.shape[0]} points: density {dt * 1e3:.1f} ms with the copies, "
+          f"{pts.shape[0] * s.n_atoms / dt / 1e9:.1f} G point-atom pairs/s", flush=True)
+    sel = np.arange(0, pts.shape[0], pts.shape[0] // 64)
+    ref = pxo.plasmon_density(s, pts[sel], var)[0]
+    assert np.max(np.abs(got[sel] - ref)) / np.max(np.abs(ref)) < 1e-11
