@@ -47,17 +47,76 @@ def plasmon_density(ctx, points_ang, variables, parts=False):
 
 
 def fortran_e(x, w=14, d=6):
-    """Fortran Ew.d edit of a real: 0.dddddd E+ee, right-aligned in w."""
+    """Fortran Ew.d edit of a real: 0.ddd...d E+ee (d significant digits,
+    correctly rounded), right-aligned in w."""
     if x == 0.0 or not np.isfinite(x):
-        m, e = 0.0, 0
+        digits, e, neg = "0" * d, 0, False
     else:
-        e = int(np.floor(np.log10(abs(x)))) + 1
-        m = x / 10.0 ** e
-        if round(abs(m), d) >= 1.0:  # rounding carried into the exponent
-            m /= 10.0
-            e += 1
-    s = ("-" if m < 0 else "") + ("0.%0*d" % (d, int(round(abs(m) * 10 ** d)))) + "E" + ("%+03d" % e)
-    return s.rjust(w)
+        mant, ex = ("%.*e" % (d - 1, abs(x))).split("e")
+        digits, e, neg = mant.replace(".", ""), int(ex) + 1, x < 0
+    return (("-" if neg else "") + "0." + digits + "E" + "%+03d" % e).rjust(w)
+
+
+# plane name -> (first in-plane axis, second in-plane axis, normal axis)
+PLANE_AXES = {"xy": (0, 1, 2), "xz": (0, 2, 1), "yz": (1, 2, 0)}
+
+
+def plane_grid(coord_ang, plane, n, offset_ang=10.0, lo_ang=None, hi_ang=None):
+    """Grid of create_plane_grid (grid_module.f90:93-154): along the plane's
+    two axes the atoms' extent +- offset (or the given min / max), n = (n1,
+    n2) points.  Returns (lo, step), length 3, zero on the normal axis."""
+    a1, a2, _ = PLANE_AXES[plane]
+    c = np.asarray(coord_ang, dtype=np.float64)
+    lo = c.min(axis=0) - offset_ang if lo_ang is None else np.asarray(lo_ang, dtype=np.float64)
+    hi = c.max(axis=0) + offset_ang if hi_ang is None else np.asarray(hi_ang, dtype=np.float64)
+    lo3, step = np.zeros(3), np.zeros(3)
+    for a, m in ((a1, n[0]), (a2, n[1])):
+        lo3[a] = lo[a]
+        step[a] = (hi[a] - lo[a]) / (m - 1)
+    return lo3, step
+
+
+def plane_points(lo, step, plane, n, third_ang):
+    """Points (n1*n2, 3) Angstrom of one plane at third_ang on the normal
+    axis, in the row order of the reference's CSV (first in-plane axis
+    outermost, plot_module.f90:597-608)."""
+    a1, a2, an = PLANE_AXES[plane]
+    i, j = np.meshgrid(np.arange(n[0]), np.arange(n[1]), indexing="ij")
+    pts = np.zeros((n[0] * n[1], 3))
+    pts[:, a1] = lo[a1] + i.reshape(-1) * step[a1]
+    pts[:, a2] = lo[a2] + j.reshape(-1) * step[a2]
+    pts[:, an] = third_ang
+    return pts
+
+
+def write_csv(path, plane, points_ang, n, values):
+    """Plane CSV in the layout of write_csv_file (plot_module.f90:568-611):
+    two in-plane coordinates and the value, E25.16 three spaces apart (the
+    trailing 3X of the format prints nothing), a blank line after each run of
+    the second coordinate."""
+    a1, a2, _ = PLANE_AXES[plane]
+    v = np.asarray(values, dtype=np.float64).reshape(-1)
+    with open(path, "w") as f:
+        for k, (p, x) in enumerate(zip(points_ang, v)):
+            x = x if abs(x) > 1e-99 else 0.0
+            f.write("   ".join(fortran_e(y, 25, 16) for y in (p[a1], p[a2], x)) + "\n")
+            if (k + 1) % n[1] == 0:
+                f.write("\n")
+
+
+def effective_volume(enhancement, lo_ang, hi_ang, step_ang):
+    """Effective volume and area of the field in a box, as
+    plot_field_or_density with print_effective_volume_area
+    (plot_module.f90:202-215, 680-735): V = sum(f^2) dx dy dz / max(f)^2 with
+    f = |E| / E0 on the box's grid, A = V / (the box's smallest side).
+    Returns (V in nm^3, A in nm^2, smallest side in Angstrom, its axis)."""
+    f = np.asarray(enhancement, dtype=np.float64)
+    side = np.asarray(hi_ang, dtype=np.float64) - np.asarray(lo_ang, dtype=np.float64)
+    k = int(np.argmin(side))
+    if np.sum(side == side[k]) > 1 or side[k] < 1e-14:
+        raise ValueError("the box has no single smallest side")
+    vol = np.sum(f ** 2) * np.prod(step_ang) / np.max(f) ** 2
+    return vol / 1e3, vol / side[k] / 1e2, side[k], "XYZ"[k]
 
 
 def write_cube(path, atomic_numbers, coord_ang, lo_ang, step_ang, n, values):

@@ -26,7 +26,13 @@ wfqfmu_module.f90:1012-1040).
 Each fixture keeps the target (coordinates, per-type R_q / R_mu and the
 dynamic parameters of the .info), E0, the variables of each field direction,
 the grid (origin, step, points per axis, from the cube header) and a
-deterministic sample of the cube's values with their flat indices.
+deterministic sample of the cube's values with their flat indices.  Plane
+fixtures keep the CSV rows (two in-plane coordinates in Angstrom and the
+value, E25.16) of calculate_field_2d / calculate_density_2d
+(grid_module.f90:310-453, plot_module.f90:568-611) on the planes of
+create_plane_grid (grid_module.f90:93-154); volume fixtures keep the
+effective volume and area printed by print_effective_volume_area
+(plot_module.f90:680-735) and the maximum of the field.
 """
 import argparse
 import json
@@ -109,26 +115,80 @@ def read_cube(text):
     return {"origin_bohr": [float(o) for o in org], "step_bohr": step, "n": npts}, vals
 
 
+PLANE_CASES = {
+    # case: (frequency tag, field direction, quantity, plane, third coordinates, user min/max or None)
+    "analysis_field_2d_x_yz_wfq_sodium": ("1.5800000000", "x", "field", "yz", ["6.35"], None),
+    "analysis_field_2d_z_xz_wfqfmu_ag": ("3.0000000000", "z", "field", "xz", ["0.00"], None),
+    "analysis_field_2d_z_xz_wfqfmu_ag_reduced": ("3.0000000000", "z", "field", "xz", ["0.00"],
+                                                 ([-5.0, 0.0, -5.0], [5.0, 0.0, 5.0])),
+    "analysis_density_2d_y_xy_wfqfmu_ag": ("3.0000000000", "y", "field", "xy", ["0.00"], None),
+    "analysis_density_2d_x_wfq_sodium": ("1.5800000000", "x", "density", "xy", ["6.35"], None),
+    "analysis_density_2d_x_wfq_sodium_n_planes": ("1.5800000000", "x", "density", "xy", ["6.35", "6.85", "7.35"],
+                                                  None),
+}
+VOLUME_CASES = {
+    # case: (frequency tag, field direction, points per axis, user min, user max)
+    "analysis_field_volume_x_wfq_sodium": ("1.5800000000", "x", 30, [53.0, 0.0, 0.0], [55.0, 10.0, 10.0]),
+    "analysis_field_volume_z_wfqfmu_au": ("3.0000000000", "z", 30, [10.0, -4.0, -4.0], [12.0, 4.0, 4.0]),
+}
+PLANE_KEEP = 1000  # planes with more points keep an even sample of SAMPLE
+
+
+def read_target(ref, case, tag):
+    tdir = os.path.join(ref, "tests", case)
+    with tarfile.open(os.path.join(tdir, case + ".tar.gz")) as tf:
+        info = read_info(tf.extractfile(case + ".info").read().decode())
+        n = len(info["symbols"])
+        n_var = 4 * n if info["model"] == "wfqfmu" else n
+        var = read_freq(tf.extractfile(f"{case}-{tag}.freq").read().decode(), n_var)
+    refdir = os.path.join(tdir, "reference")
+    sub = [d for d in os.listdir(refdir) if os.path.isdir(os.path.join(refdir, d))][0]
+    return info, var, os.path.join(refdir, sub)
+
+
+def target_fields(case, info):
+    return {"case": case, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
+            "symbols": info["symbols"], "coord_ang": info["coord_ang"], "types": info["types"],
+            "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
+            "grid_offset_ang": 10.0}
+
+
+def read_csv(text):
+    """Rows of (coordinate 1, coordinate 2, value), E25.16; a blank line ends
+    each run of the second coordinate."""
+    rows, n2, run = [], None, 0
+    for ln in text.splitlines():
+        f = ln.split()
+        if not f:
+            n2 = n2 or run
+            run = 0
+            continue
+        rows.append([float(v) for v in f])
+        run += 1
+    return rows, n2
+
+
+def sample(total, *values):
+    if total <= PLANE_KEEP:
+        return list(range(total))
+    extra = [max(range(total), key=v.__getitem__) for v in values] + [min(range(total), key=v.__getitem__)
+                                                                      for v in values]
+    return sorted(set(list(range(0, total, max(1, total // SAMPLE))) + extra))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
     here = os.path.dirname(os.path.abspath(__file__))
     for case, (tag, dirs, what) in CASES.items():
-        tdir = os.path.join(a.ref, "tests", case)
-        with tarfile.open(os.path.join(tdir, case + ".tar.gz")) as tf:
-            info = read_info(tf.extractfile(case + ".info").read().decode())
-            n = len(info["symbols"])
-            n_var = 4 * n if info["model"] == "wfqfmu" else n
-            var = read_freq(tf.extractfile(f"{case}-{tag}.freq").read().decode(), n_var)
-        refdir = os.path.join(tdir, "reference")
-        sub = [d for d in os.listdir(refdir) if os.path.isdir(os.path.join(refdir, d))][0]
+        info, var, rdir = read_target(a.ref, case, tag)
         cubes, grid = {}, None
         for d in dirs:
             parts = {"values": ""} if what == "field" else {"re": "-densityRe", "im": "-densityIm"}
             read = {}
             for key, suffix in parts.items():
-                with open(os.path.join(refdir, sub, f"{case}-E{d}-{tag}{suffix}.cube")) as f:
+                with open(os.path.join(rdir, f"{case}-E{d}-{tag}{suffix}.cube")) as f:
                     g, read[key] = read_cube(f.read())
                 grid = grid or g
             # deterministic sample: evenly spaced flat indices plus each part's extremes
@@ -137,14 +197,48 @@ def main():
             extra += [min(range(total), key=v.__getitem__) for v in read.values()]
             idx = sorted(set(list(range(0, total, max(1, total // SAMPLE))) + extra))
             cubes[d] = {"idx": idx, **{k: [v[i] for i in idx] for k, v in read.items()}}
-        fx = {"case": case, "quantity": what, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
-              "symbols": info["symbols"], "coord_ang": info["coord_ang"], "types": info["types"],
-              "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
-              "grid_offset_ang": 10.0, "grid": grid,
+        fx = {**target_fields(case, info), "quantity": what, "grid": grid,
               "variables": {d: var[d] for d in dirs}, "cube": cubes}
         with open(os.path.join(here, case + ".json"), "w") as f:
             json.dump(fx, f)
-        print(case, n, "atoms", {d: len(cubes[d]["idx"]) for d in dirs}, "values")
+        print(case, len(info["symbols"]), "atoms", {d: len(cubes[d]["idx"]) for d in dirs}, "values")
+
+    planes = {}
+    for case, (tag, d, what, plane, thirds, user) in PLANE_CASES.items():
+        info, var, rdir = read_target(a.ref, case, tag)
+        out = []
+        for third in thirds:
+            parts = {"values": ""} if what == "field" else {"re": "-densityRe", "im": "-densityIm"}
+            read, coords, n2 = {}, None, None
+            for key, suffix in parts.items():
+                with open(os.path.join(rdir, "planes", plane, f"{case}-E{d}-{tag}-p-{third}{suffix}.csv")) as f:
+                    rows, n2 = read_csv(f.read())
+                read[key] = [r[2] for r in rows]
+                coords = coords or [r[:2] for r in rows]
+            idx = sample(len(coords), *read.values())
+            out.append({"third": float(third), "n": [len(coords) // n2, n2], "idx": idx,
+                        "coords": [coords[i] for i in idx], **{k: [v[i] for i in idx] for k, v in read.items()}})
+        fx = {**target_fields(case, info), "quantity": what, "plane": plane, "user_grid": user,
+              "field_dir": d, "variables": {d: var[d]}, "planes": out}
+        planes[case] = fx
+        with open(os.path.join(here, case + ".json"), "w") as f:
+            json.dump(fx, f)
+        print(case, len(info["symbols"]), "atoms", [len(p["idx"]) for p in out], "plane values")
+
+    for case, (tag, d, n, lo, hi) in VOLUME_CASES.items():
+        info, var, rdir = read_target(a.ref, case, tag)
+        with open(os.path.join(rdir, case + "_analysis.log")) as f:
+            log = f.read().splitlines()
+        vol = next(float(ln.split(":")[1].split()[0]) for ln in log if ln.strip().startswith("Effective Volume"))
+        area = next(float(ln.split(":")[1].split()[0]) for ln in log if ln.strip().startswith("Effective Area"))
+        i = next(k for k, ln in enumerate(log) if ln.split()[:2] == ["Field", "X"])
+        f = log[i + 1].split()
+        fx = {**target_fields(case, info), "quantity": "volume", "field_dir": d, "n": n, "min_ang": lo, "max_ang": hi,
+              "variables": {d: var[d]}, "effective_volume_nm3": vol, "effective_area_nm2": area,
+              "max_point_ang": [float(v) for v in f[1:4]], "max_value": float(f[4])}
+        with open(os.path.join(here, case + ".json"), "w") as fo:
+            json.dump(fx, fo)
+        print(case, len(info["symbols"]), "atoms", vol, area, f)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,123 @@
+"""Planes and effective volumes of the analysis: the oracle against the
+reference's CSVs (E25.16) and logs.
+
+The plane grid is rebuilt as create_plane_grid does and must give the CSV's
+in-plane coordinates; the oracle's |E| / E0 or density on it must give the
+CSV's values to 1e-10 relative (16 printed digits; the reference sums in a
+different order and from Angstrom coordinates).  The effective volume and
+area of a field box must give the printed figures (5 decimals) and the
+maximum of the field its printed value and grid point.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from oracle import pxo
+from plasmonx_amd.analysis import (cubic_grid, effective_volume, fortran_e, plane_grid, plane_points, write_csv)
+from plasmonx_amd.model import TOBOHR
+from test_analysis_golden import load, system_from_analysis_fixture
+
+PLANE_CASES = ["analysis_field_2d_x_yz_wfq_sodium", "analysis_field_2d_z_xz_wfqfmu_ag",
+               "analysis_field_2d_z_xz_wfqfmu_ag_reduced", "analysis_density_2d_y_xy_wfqfmu_ag",
+               "analysis_density_2d_x_wfq_sodium", "analysis_density_2d_x_wfq_sodium_n_planes"]
+VOLUME_CASES = ["analysis_field_volume_x_wfq_sodium", "analysis_field_volume_z_wfqfmu_au"]
+
+
+def plane_setup(fx, pl):
+    user = fx["user_grid"]
+    lo, step = plane_grid(fx["coord_ang"], fx["plane"], pl["n"], fx["grid_offset_ang"],
+                          *(user if user else (None, None)))
+    return plane_points(lo, step, fx["plane"], pl["n"], pl["third"])
+
+
+def plane_values(fx, pl, pts, field_fn, density_fn):
+    """{part: values} at pts for the fixture's quantity, from the given field /
+    density callables (points in bohr, variables, i_pol)."""
+    d = fx["field_dir"]
+    var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+    if fx["quantity"] == "field":
+        e = field_fn(pts * TOBOHR, var, "xyz".index(d), fx["e0"])
+        return {"values": np.sqrt((np.abs(e) ** 2).sum(axis=1)) / fx["e0"]}
+    rho = density_fn(pts * TOBOHR, var)
+    return {"re": rho.real, "im": rho.imag}
+
+
+def check_plane(fx, pl, got):
+    for part, v in got.items():
+        ref = np.array(pl[part])
+        scale = np.max(np.abs(ref))
+        assert np.allclose(v, ref, rtol=1e-10, atol=1e-12 * scale), (part, np.max(np.abs(v - ref) / scale))
+
+
+def volume_setup(fx):
+    n = (fx["n"],) * 3
+    lo, hi = np.array(fx["min_ang"]), np.array(fx["max_ang"])
+    step = (hi - lo) / (np.array(n) - 1)
+    i, j, k = np.meshgrid(*(np.arange(m) for m in n), indexing="ij")
+    pts = np.stack([lo[0] + i * step[0], lo[1] + j * step[1], lo[2] + k * step[2]], axis=-1).reshape(-1, 3)
+    return lo, hi, step, pts
+
+
+def check_volume(fx, lo, hi, step, pts, f):
+    vol, area, side, axis = effective_volume(f, lo, hi, step)
+    assert axis == "X" and abs(side - 2.0) < 1e-12
+    assert abs(vol - fx["effective_volume_nm3"]) <= 0.51e-5
+    assert abs(area - fx["effective_area_nm2"]) <= 0.51e-5
+    assert fortran_e(np.max(f)) == fortran_e(fx["max_value"])
+    assert np.allclose(pts[np.argmax(f)], fx["max_point_ang"], atol=0.51e-5)
+
+
+@pytest.mark.parametrize("case", PLANE_CASES)
+def test_plane_grid_and_oracle_values(case):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    a1, a2 = {"xy": (0, 1), "xz": (0, 2), "yz": (1, 2)}[fx["plane"]]
+    for pl in fx["planes"]:
+        pts = plane_setup(fx, pl)
+        idx = np.array(pl["idx"])
+        assert np.allclose(pts[idx][:, [a1, a2]], pl["coords"], rtol=0, atol=1e-13)
+        got = plane_values(fx, pl, pts[idx], lambda p, v, i, e0: pxo.induced_field(s, p, v, i, e0),
+                           lambda p, v: pxo.plasmon_density(s, p, v)[0])
+        check_plane(fx, pl, got)
+
+
+def test_plane_csv_layout(tmp_path):
+    """write_csv prints the reference's rows for the reference's numbers."""
+    fx = load("analysis_density_2d_x_wfq_sodium_n_planes")
+    pl = fx["planes"][1]
+    assert len(pl["idx"]) == pl["n"][0] * pl["n"][1]  # every row kept
+    pts = plane_setup(fx, pl)
+    path = tmp_path / "plane.csv"
+    write_csv(path, fx["plane"], pts, pl["n"], pl["re"])
+    lines = path.read_text().splitlines()
+    assert len(lines) == pl["n"][0] * (pl["n"][1] + 1)
+    rows = [ln for ln in lines if ln]
+    assert all(len(ln) == 81 for ln in rows)
+    for ln, c, x in zip(rows, pl["coords"], pl["re"]):
+        cols = [ln[0:25], ln[28:53], ln[56:81]]
+        assert all(re.fullmatch(r" *-?0\.\d{16}E[+-]\d\d", t) for t in cols), ln
+        assert np.allclose([float(t) for t in cols], [*c, x], rtol=1e-15, atol=1e-12)  # grid rounding near 0
+    assert lines[pl["n"][1]] == ""
+
+
+@pytest.mark.parametrize("case", VOLUME_CASES)
+def test_effective_volume_oracle(case):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    lo, hi, step, pts = volume_setup(fx)
+    d = fx["field_dir"]
+    var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+    e = pxo.induced_field(s, pts * TOBOHR, var, "xyz".index(d), fx["e0"])
+    check_volume(fx, lo, hi, step, pts, np.sqrt((np.abs(e) ** 2).sum(axis=1)) / fx["e0"])
+
+
+def test_effective_volume_of_a_constant_field():
+    """A uniform enhancement fills the box: V is the box volume."""
+    lo, hi = np.array([0.0, 0.0, 0.0]), np.array([2.0, 10.0, 10.0])
+    step = (hi - lo) / 29
+    f = np.full(30 ** 3, 3.0)
+    vol, area, side, axis = effective_volume(f, lo, hi, step)
+    box = np.prod(step) * 30 ** 3
+    assert abs(vol * 1e3 - box) < 1e-9 and abs(area * 1e2 - box / 2.0) < 1e-9 and axis == "X"
+    _ = cubic_grid  # the cubic grid of the default box is covered by the cube tests

@@ -7,7 +7,9 @@ reference's printed cube values to their 6 digits; on a 15683-atom omegaFQFmu
 sphere (the C2 target) with random variables the field at 64 points matches
 the oracle, and the throughput of a full 100k-point grid is reported.  The
 density is held to the same bars (real and imaginary cubes, parts summing to
-the total).
+the total).  The planes of the reference's plane tests are computed whole and
+must give its CSV values (16 digits) to 1e-10, and the effective volume of
+its field boxes the printed figures.
 """
 import time
 
@@ -19,6 +21,8 @@ from plasmonx_amd import Context
 from plasmonx_amd.analysis import cubic_grid, field_enhancement, plasmon_density
 from plasmonx_amd.model import TOBOHR
 from test_analysis_golden import CASES, DENSITY_CASES, half_ulp, load, system_from_analysis_fixture
+from test_analysis_planes import (PLANE_CASES, VOLUME_CASES, check_plane, check_volume, plane_setup, plane_values,
+                                  volume_setup)
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +118,35 @@ def test_density_large_target():
     sel = np.arange(0, pts.shape[0], pts.shape[0] // 64)
     ref = pxo.plasmon_density(s, pts[sel], var)[0]
     assert np.max(np.abs(got[sel] - ref)) / np.max(np.abs(ref)) < 1e-11
+
+
+@pytest.mark.parametrize("case", PLANE_CASES)
+def test_planes_match_reference_csv(case):
+    """Every plane of the reference's plane tests, whole, on the GPU: the
+    sampled CSV values to 1e-10 and the whole plane to the oracle."""
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    with Context(0) as ctx:
+        ctx.set_atoms(s)
+        for pl in fx["planes"]:
+            pts = plane_setup(fx, pl)
+            got = plane_values(fx, pl, pts, ctx.induced_field, ctx.plasmon_density)
+            ref = plane_values(fx, pl, pts, lambda p, v, i, e0: pxo.induced_field(s, p, v, i, e0),
+                               lambda p, v: pxo.plasmon_density(s, p, v)[0])
+            for part in got:
+                assert np.max(np.abs(got[part] - ref[part])) <= 1e-12 * np.max(np.abs(ref[part]))
+            idx = np.array(pl["idx"])
+            check_plane(fx, pl, {k: v[idx] for k, v in got.items()})
+
+
+@pytest.mark.parametrize("case", VOLUME_CASES)
+def test_effective_volume_gpu(case):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    lo, hi, step, pts = volume_setup(fx)
+    d = fx["field_dir"]
+    var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+    with Context(0) as ctx:
+        ctx.set_atoms(s)
+        f = field_enhancement(ctx, pts, var, "xyz".index(d), fx["e0"])
+    check_volume(fx, lo, hi, step, pts, f)
