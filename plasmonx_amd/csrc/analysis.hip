@@ -14,6 +14,8 @@
 // src/wfq/wfqfmu_module.f90:973-1003.  The reference evaluates one point at
 // a time on the host (grid_module.f90:200-242); here a work-item owns a
 // point and the atoms stream through LDS in tiles, FP64 throughout.
+#include <algorithm>
+
 #include "pxh_internal.h"
 
 namespace pxh {
@@ -30,11 +32,14 @@ struct TypeRadii {
 __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, const double *__restrict__ pts, int n,
                                                               const double4 *__restrict__ pos, TypeRadii radii,
                                                               int fmu, const double2 *__restrict__ var, int i_pol,
-                                                              double e0, double2 *__restrict__ E) {
+                                                              double e0, int slice, size_t part_stride,
+                                                              double2 *__restrict__ E) {
     __shared__ double4 sp[kFieldTile];
     __shared__ double2 sq[kFieldTile];
     __shared__ double2 smu[3][kFieldTile];
-    __shared__ double sr[2][kFieldTile];
+    // per-atom constants: 1/R_q, 1/R_q^2, 2/(sqrt(pi) R_q), 1/R_mu, 1/R_mu^2,
+    // 2/(sqrt(pi) R_mu), 4/(sqrt(pi) R_mu^3): one division per pair remains
+    __shared__ double sr[7][kFieldTile];
     const int tid = threadIdx.x;
     const int p = blockIdx.x * kFieldTile + tid;
     double px = 0.0, py = 0.0, pz = 0.0;
@@ -43,23 +48,31 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
         py = pts[3 * (size_t)p + 1];
         pz = pts[3 * (size_t)p + 2];
     }
-    double2 ex = make_double2(i_pol == 0 ? e0 : 0.0, 0.0);
-    double2 ey = make_double2(i_pol == 1 ? e0 : 0.0, 0.0);
-    double2 ez = make_double2(i_pol == 2 ? e0 : 0.0, 0.0);
-    for (int t0 = 0; t0 < n; t0 += kFieldTile) {
+    const double inc = blockIdx.y == 0 ? e0 : 0.0;  // the incident field, once
+    double2 ex = make_double2(i_pol == 0 ? inc : 0.0, 0.0);
+    double2 ey = make_double2(i_pol == 1 ? inc : 0.0, 0.0);
+    double2 ez = make_double2(i_pol == 2 ? inc : 0.0, 0.0);
+    const int a0 = blockIdx.y * slice, a1 = min(n, a0 + slice);
+    for (int t0 = a0; t0 < a1; t0 += kFieldTile) {
         const int j = t0 + tid;
-        if (j < n) {
+        if (j < a1) {
             const double4 a = pos[j];
             const int t = (int)a.w;
             sp[tid] = a;
             sq[tid] = var[j];
-            sr[0][tid] = radii.rq[t];
-            sr[1][tid] = radii.rmu[t];
+            const double iq = 1.0 / radii.rq[t], im = 1.0 / radii.rmu[t];
+            sr[0][tid] = iq;
+            sr[1][tid] = iq * iq;
+            sr[2][tid] = 2.0 * kInvSqrtPi * iq;
+            sr[3][tid] = im;
+            sr[4][tid] = im * im;
+            sr[5][tid] = 2.0 * kInvSqrtPi * im;
+            sr[6][tid] = 4.0 * kInvSqrtPi * im * im * im;
             if (fmu)
                 for (int c = 0; c < 3; ++c) smu[c][tid] = var[n + 3 * (size_t)j + c];
         }
         __syncthreads();
-        const int m = min(kFieldTile, n - t0);
+        const int m = min(kFieldTile, a1 - t0);
         for (int jj = 0; jj < m; ++jj) {
             const double4 a = sp[jj];
             const double dx = a.x - px, dy = a.y - py, dz = a.z - pz;
@@ -67,8 +80,7 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
             const double r = sqrt(r2);
             if (!(r > 1.0e-1)) continue;
             const double inv_r = 1.0 / r, inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
-            const double R = sr[0][jj];
-            const double fq = erf(r / R) - 2.0 * kInvSqrtPi / R * r * exp(-r2 / (R * R));
+            const double fq = erf(r * sr[0][jj]) - sr[2][jj] * r * exp(-r2 * sr[1][jj]);
             const double2 q = sq[jj];
             const double cq = fq * inv_r3;
             ex.x -= cq * dx * q.x;
@@ -78,10 +90,9 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
             ez.x -= cq * dz * q.x;
             ez.y -= cq * dz * q.y;
             if (fmu) {
-                const double Rm = sr[1][jj];
-                const double em = exp(-r2 / (Rm * Rm));
-                const double f1 = erf(r / Rm) - 2.0 * kInvSqrtPi * r / Rm * em;
-                const double f2 = 4.0 * kInvSqrtPi / (Rm * Rm * Rm) * em;
+                const double em = exp(-r2 * sr[4][jj]);
+                const double f1 = erf(r * sr[3][jj]) - sr[5][jj] * r * em;
+                const double f2 = sr[6][jj] * em;
                 // T = (f2 / r^2 - 3 f1 / r^5) d d^T + (f1 / r^3) I
                 const double g = f2 * inv_r2 - 3.0 * f1 * inv_r3 * inv_r2;
                 const double h = f1 * inv_r3;
@@ -98,9 +109,10 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
         __syncthreads();
     }
     if (p < n_points) {
-        E[3 * (size_t)p] = ex;
-        E[3 * (size_t)p + 1] = ey;
-        E[3 * (size_t)p + 2] = ez;
+        double2 *out = E + blockIdx.y * part_stride;
+        out[3 * (size_t)p] = ex;
+        out[3 * (size_t)p + 1] = ey;
+        out[3 * (size_t)p + 2] = ez;
     }
 }
 
@@ -114,8 +126,9 @@ __global__ __launch_bounds__(kFieldTile) void k_induced_field(int n_points, cons
 // (the two parts) are written when not null.
 __global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, const double *__restrict__ pts, int n,
                                                                 const double4 *__restrict__ pos, TypeRadii radii,
-                                                                int fmu, const double2 *__restrict__ var,
-                                                                double2 *__restrict__ rho, double2 *__restrict__ rho_q,
+                                                                int fmu, const double2 *__restrict__ var, int slice,
+                                                                size_t part_stride, double2 *__restrict__ rho,
+                                                                double2 *__restrict__ rho_q,
                                                                 double2 *__restrict__ rho_mu) {
     __shared__ double4 sp[kFieldTile];
     __shared__ double2 sq[kFieldTile];
@@ -131,9 +144,10 @@ __global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, co
     }
     constexpr double kInvPi32 = 0.17958712212516656;  // 1 / pi^{3/2}
     double2 aq = make_double2(0.0, 0.0), am = make_double2(0.0, 0.0);
-    for (int t0 = 0; t0 < n; t0 += kFieldTile) {
+    const int a0 = blockIdx.y * slice, a1 = min(n, a0 + slice);
+    for (int t0 = a0; t0 < a1; t0 += kFieldTile) {
         const int j = t0 + tid;
-        if (j < n) {
+        if (j < a1) {
             const double4 a = pos[j];
             const int t = (int)a.w;
             sp[tid] = a;
@@ -149,7 +163,7 @@ __global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, co
             }
         }
         __syncthreads();
-        const int m = min(kFieldTile, n - t0);
+        const int m = min(kFieldTile, a1 - t0);
         for (int jj = 0; jj < m; ++jj) {
             const double4 a = sp[jj];
             const double dx = a.x - px, dy = a.y - py, dz = a.z - pz;
@@ -168,10 +182,56 @@ __global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, co
         __syncthreads();
     }
     if (p < n_points) {
-        rho[p] = make_double2(aq.x + am.x, aq.y + am.y);
-        if (rho_q) rho_q[p] = aq;
-        if (rho_mu) rho_mu[p] = am;
+        if (rho) rho[p] = make_double2(aq.x + am.x, aq.y + am.y);
+        if (rho_q) rho_q[blockIdx.y * part_stride + p] = aq;
+        if (rho_mu) rho_mu[blockIdx.y * part_stride + p] = am;
     }
+}
+
+// Sum of the per-slice partial results, in slice order (deterministic):
+// parts[s * count + i] for s < n_slices.
+__global__ void k_sum_slices(size_t count, int n_slices, const double2 *__restrict__ parts,
+                             double2 *__restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double2 acc = parts[i];
+    for (int s = 1; s < n_slices; ++s) {
+        const double2 v = parts[s * count + i];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    out[i] = acc;
+}
+
+// rho = q part + mu part, each summed over the slices; the parts on request.
+__global__ void k_sum_density(int n_points, int n_slices, const double2 *__restrict__ pq,
+                              const double2 *__restrict__ pm, double2 *__restrict__ rho,
+                              double2 *__restrict__ rho_q, double2 *__restrict__ rho_mu) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_points) return;
+    double2 q = pq[p], m = pm[p];
+    for (int s = 1; s < n_slices; ++s) {
+        const double2 a = pq[(size_t)s * n_points + p], b = pm[(size_t)s * n_points + p];
+        q.x += a.x;
+        q.y += a.y;
+        m.x += b.x;
+        m.y += b.y;
+    }
+    rho[p] = make_double2(q.x + m.x, q.y + m.y);
+    if (rho_q) rho_q[p] = q;
+    if (rho_mu) rho_mu[p] = m;
+}
+
+// Atom slices per point block so that a launch holds about 8 workgroups per
+// CU: a 30^3 grid is 106 blocks of points, fewer than the 256 CUs.
+int atom_slices(int point_blocks, int n) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int want = (8 * cus + point_blocks - 1) / point_blocks;
+    const int max_slices = (n + kFieldTile - 1) / kFieldTile;  // at least one tile per slice
+    return std::max(1, std::min(want, max_slices));
 }
 
 }  // namespace
@@ -185,11 +245,26 @@ void plasmon_density(pxh_ctx *c, int n_points, const double *d_pts, const double
         radii.rmu[t] = A.model == PXH_WFQFMU ? A.rmu[t] : 1.0;
     }
     const int blocks = (n_points + kFieldTile - 1) / kFieldTile;
-    if (blocks > 0)
-        k_plasmon_density<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii,
-                                                                A.model == PXH_WFQFMU ? 1 : 0, d_var, d_rho, d_rho_q,
-                                                                d_rho_mu);
+    if (blocks == 0) return;
+    const int fmu = A.model == PXH_WFQFMU ? 1 : 0;
+    const int S = A.n == 0 ? 1 : atom_slices(blocks, A.n);
+    const int slice = A.n == 0 ? 1 : (A.n + S - 1) / S;
+    if (S == 1) {
+        k_plasmon_density<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii, fmu, d_var, slice,
+                                                                0, d_rho, d_rho_q, d_rho_mu);
+        PXH_HIP(hipGetLastError());
+        return;
+    }
+    DevBuf<double2> part;
+    const size_t np = (size_t)n_points;
+    part.alloc(2 * (size_t)S * np);
+    k_plasmon_density<<<dim3(blocks, S), kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii, fmu, d_var,
+                                                                     slice, np, nullptr, part.p, part.p + S * np);
     PXH_HIP(hipGetLastError());
+    k_sum_density<<<blocks, kFieldTile, 0, c->stream>>>(n_points, S, part.p, part.p + S * np, d_rho, d_rho_q,
+                                                        d_rho_mu);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));  // before the partials are freed
 }
 
 void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,
@@ -201,10 +276,26 @@ void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 
         radii.rmu[t] = A.model == PXH_WFQFMU ? A.rmu[t] : 1.0;
     }
     const int blocks = (n_points + kFieldTile - 1) / kFieldTile;
-    if (blocks > 0)
-        k_induced_field<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii,
-                                                              A.model == PXH_WFQFMU ? 1 : 0, d_var, i_pol, e0, d_E);
+    if (blocks == 0) return;
+    const int fmu = A.model == PXH_WFQFMU ? 1 : 0;
+    const int S = A.n == 0 ? 1 : atom_slices(blocks, A.n);
+    const int slice = A.n == 0 ? 1 : (A.n + S - 1) / S;
+    if (S == 1) {
+        k_induced_field<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii, fmu, d_var, i_pol,
+                                                              e0, slice, 0, d_E);
+        PXH_HIP(hipGetLastError());
+        return;
+    }
+    DevBuf<double2> part;
+    const size_t count = 3 * (size_t)n_points;
+    part.alloc((size_t)S * count);
+    k_induced_field<<<dim3(blocks, S), kFieldTile, 0, c->stream>>>(n_points, d_pts, A.n, A.pos.p, radii, fmu, d_var,
+                                                                   i_pol, e0, slice, count, part.p);
     PXH_HIP(hipGetLastError());
+    k_sum_slices<<<(unsigned)((count + kFieldTile - 1) / kFieldTile), kFieldTile, 0, c->stream>>>(count, S, part.p,
+                                                                                                   d_E);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));  // before the partials are freed
 }
 
 }  // namespace pxh
