@@ -162,6 +162,18 @@ int pxh_dynamic_rhs(pxh_ctx *ctx, double freq_au, double e0, int rhs_form, doubl
 int pxh_induced_field(pxh_ctx *ctx, int n_points, const double *points_bohr, const double *variables, int i_pol,
                       double e0, double *field);
 
+/* Field at n_points points of BEM surface charges (n_tess complex, point
+ * charges at the tessera centroids, bohr, 3 x n_tess) for field direction
+ * i_pol (0, 1, 2): zero at points inside the particle (not farther from the
+ * centroids' mean than the nearest centroid is, plus 0.5 Angstrom), else the
+ * incident e0 along i_pol plus the Coulomb fields of the charges.  Needs no
+ * target in the context.  field 3 x n_points complex.  Replaces
+ * calculate_induced_field_at_point_bem (bem_module.f90:1151-1213) with
+ * calculate_induced_field_q_bem (matrix_module.f90:1444-1464) in the
+ * per-point loops of grid_module.f90:200-388. */
+int pxh_bem_induced_field(pxh_ctx *ctx, int n_tess, const double *centroids_bohr, const double *charges, int n_points,
+                          const double *points_bohr, int i_pol, double e0, double *field);
+
 /* Plasmon density at n_points points for the variables of one frequency and
  * one field direction of an atomistic target with the Gaussian kernel: the
  * normalised Gaussians (width R_q) of the charges and, for omegaFQFmu, minus

@@ -121,6 +121,9 @@ void pxo_tesserae(int n_tess, const double *verts9, double normal_factor,
                   double *coord, double *normal, double *tang1, double *tang2,
                   double *area);
 void pxo_lgl(int npol, double *points, double *weights);
+void pxo_bem_induced_field(int n_tess, const double *c, const pxo_cplx *q,
+                           int n_points, const double *points, int i_pol,
+                           double e0, pxo_cplx *E);
 void pxo_bem_d(const pxo_mesh *m, int accurate, double sphere_r, double *D,
                double *D_ief /* may be NULL: D with unrefined diagonal - 2pi */,
                double *SA /* may be NULL: IEF S A^-1 (real) */);

@@ -256,3 +256,18 @@ def bem_solve_spectrum(mesh, freq_au, eps_w, e0, method=0, tol=1e-9, m=1000, wan
     if want_variables:
         out["variables"] = var
     return out
+
+
+def bem_induced_field(centroids_bohr, charges, points_bohr, i_pol, e0):
+    """Field (n_points, 3) complex of the BEM charges at the centroids, zero
+    inside the particle."""
+    lib = load()
+    c = np.ascontiguousarray(centroids_bohr, dtype=np.float64)
+    q = np.ascontiguousarray(charges, dtype=np.complex128)
+    pts = np.ascontiguousarray(points_bohr, dtype=np.float64)
+    assert c.shape == (q.shape[0], 3)
+    E = np.zeros((pts.shape[0], 3), dtype=np.complex128)
+    lib.pxo_bem_induced_field(ctypes.c_int(q.shape[0]), P(c), P(q), ctypes.c_int(pts.shape[0]), P(pts),
+                              ctypes.c_int(i_pol), ctypes.c_double(e0), P(E))
+    return E
+

@@ -409,3 +409,49 @@ int pxo_bem_solve_spectrum(const pxo_mesh *m, int ief, int accurate, double sphe
     free(one);
     return status;
 }
+
+/* Field at n_points points (bohr) of the BEM surface charges q (n_tess
+ * complex) at the tessera centroids c (bohr), as
+ * calculate_induced_field_at_point_bem (src/bem/bem_module.f90:1151-1213):
+ * with m the mean of the centroids and l the first centroid nearest to the
+ * point p, p is outside when |p - m| > |c_l - m| + 0.5 Angstrom; outside,
+ * E = e0 along i_pol minus sum_i q_i d / |d|^3 over |d| > 0.1 bohr,
+ * d = c_i - p (src/matrix/matrix_module.f90:1444-1464); inside, E = 0. */
+void pxo_bem_induced_field(int n_tess, const double *c, const pxo_cplx *q, int n_points, const double *points,
+                           int i_pol, double e0, pxo_cplx *E) {
+    double m[3] = {0, 0, 0};
+    for (int i = 0; i < n_tess; ++i)
+        for (int k = 0; k < 3; ++k) m[k] += c[3 * i + k] / n_tess;
+#pragma omp parallel for
+    for (int p = 0; p < n_points; ++p) {
+        const double *x = points + 3 * p;
+        int l = 0;
+        double best = 1.0e6 * 1.8897261254578281;
+        for (int i = 0; i < n_tess; ++i) {
+            const double dx = c[3 * i] - x[0], dy = c[3 * i + 1] - x[1], dz = c[3 * i + 2] - x[2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (best > r) {
+                best = r;
+                l = i;
+            }
+        }
+        const double rl = sqrt((c[3 * l] - m[0]) * (c[3 * l] - m[0]) + (c[3 * l + 1] - m[1]) * (c[3 * l + 1] - m[1]) +
+                               (c[3 * l + 2] - m[2]) * (c[3 * l + 2] - m[2])) +
+                          0.5 * 1.8897261254578281;
+        const double rp = sqrt((x[0] - m[0]) * (x[0] - m[0]) + (x[1] - m[1]) * (x[1] - m[1]) +
+                               (x[2] - m[2]) * (x[2] - m[2]));
+        pxo_cplx e[3] = {0, 0, 0};
+        if (rp > rl) {
+            e[i_pol] = e0;
+            for (int i = 0; i < n_tess; ++i) {
+                double d[3];
+                for (int k = 0; k < 3; ++k) d[k] = c[3 * i + k] - x[k];
+                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (!(r > 1.0e-1)) continue;
+                const double r3 = r * r * r;
+                for (int k = 0; k < 3; ++k) e[k] -= d[k] * q[i] / r3;
+            }
+        }
+        for (int k = 0; k < 3; ++k) E[3 * p + k] = e[k];
+    }
+}

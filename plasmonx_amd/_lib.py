@@ -29,6 +29,7 @@ EXPORTS = [
     "pxh_server_create", "pxh_server_submit", "pxh_server_wait", "pxh_server_last_error", "pxh_server_destroy",
     "pxh_induced_field",
     "pxh_plasmon_density",
+    "pxh_bem_induced_field",
 ]
 
 
@@ -62,6 +63,7 @@ def load():
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
     lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
     lib.pxh_plasmon_density.argtypes = [P, I, P, P, P, P, P]
+    lib.pxh_bem_induced_field.argtypes = [P, I, P, P, I, P, I, D, P]
     lib.pxh_constant_matrix.argtypes = [P, P]
     lib.pxh_dynamic_rhs.argtypes = [P, D, D, I, P]
     lib.pxh_last_error.argtypes = [P, ctypes.c_char_p, I]
@@ -197,6 +199,19 @@ class Context:
         e = np.zeros((pts.shape[0], 3), dtype=np.complex128)
         self._check(self.lib.pxh_induced_field(self._h, pts.shape[0], _ptr(pts), _ptr(v), int(i_pol), float(e0),
                                                _ptr(e)))
+        return e
+
+    def bem_induced_field(self, centroids_bohr, charges, points_bohr, i_pol, e0):
+        """Field (n_points, 3) complex of BEM charges at the tessera centroids
+        (n_tess, 3) bohr, zero inside the particle."""
+        c = np.ascontiguousarray(centroids_bohr, dtype=np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(charges, dtype=np.complex128).reshape(-1)
+        if q.shape[0] != c.shape[0]:
+            raise ValueError("one charge per centroid")
+        pts = np.ascontiguousarray(points_bohr, dtype=np.float64).reshape(-1, 3)
+        e = np.zeros((pts.shape[0], 3), dtype=np.complex128)
+        self._check(self.lib.pxh_bem_induced_field(self._h, c.shape[0], _ptr(c), _ptr(q), pts.shape[0], _ptr(pts),
+                                                   int(i_pol), float(e0), _ptr(e)))
         return e
 
     def plasmon_density(self, points_bohr, variables, parts=False):

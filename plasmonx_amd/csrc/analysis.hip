@@ -15,6 +15,8 @@
 // a time on the host (grid_module.f90:200-242); here a work-item owns a
 // point and the atoms stream through LDS in tiles, FP64 throughout.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "pxh_internal.h"
 
@@ -24,6 +26,7 @@ namespace {
 
 constexpr int kFieldTile = 256;
 constexpr double kInvSqrtPi = 0.56418958354775628;  // 1 / sqrt(pi)
+constexpr double kToBohr = 1.8897261254578281;       // bohr per Angstrom
 
 struct TypeRadii {
     double rq[kMaxTypes], rmu[kMaxTypes];
@@ -188,6 +191,74 @@ __global__ __launch_bounds__(kFieldTile) void k_plasmon_density(int n_points, co
     }
 }
 
+// Field of BEM surface charges (point charges at the tessera centroids) at a
+// point p, as calculate_induced_field_at_point_bem
+// (src/bem/bem_module.f90:1151-1213): the point is inside, and its field 0,
+// unless |p - m| > |c_l - m| + 0.5 Angstrom, with m the mean of the
+// centroids and l the first centroid nearest to p; outside, the incident e0
+// minus sum_i q_i d / |d|^3 over |d| > 0.1 bohr, d = c_i - p
+// (src/matrix/matrix_module.f90:1444-1464).  One pass: the nearest centroid
+// and the sum are found together and the sum is dropped for inside points.
+// ci[4] holds x, y, z and |c_i - m| + 0.5 Angstrom.
+__global__ __launch_bounds__(kFieldTile) void k_bem_field(int n_points, const double *__restrict__ pts, int n,
+                                                          const double4 *__restrict__ ci,
+                                                          const double2 *__restrict__ q, double mx, double my,
+                                                          double mz, int i_pol, double e0, double2 *__restrict__ E) {
+    __shared__ double4 sc[kFieldTile];
+    __shared__ double2 sq[kFieldTile];
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * kFieldTile + tid;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (p < n_points) {
+        px = pts[3 * (size_t)p];
+        py = pts[3 * (size_t)p + 1];
+        pz = pts[3 * (size_t)p + 2];
+    }
+    double2 ex = make_double2(0.0, 0.0), ey = ex, ez = ex;
+    double best = 1.0e6 * kToBohr, rl = 0.0;
+    for (int t0 = 0; t0 < n; t0 += kFieldTile) {
+        const int j = t0 + tid;
+        if (j < n) {
+            sc[tid] = ci[j];
+            sq[tid] = q[j];
+        }
+        __syncthreads();
+        const int m = min(kFieldTile, n - t0);
+        for (int jj = 0; jj < m; ++jj) {
+            const double4 a = sc[jj];
+            const double dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (best > r) {
+                best = r;
+                rl = a.w;
+            }
+            if (!(r > 1.0e-1)) continue;
+            const double c = 1.0 / (r * r * r);
+            const double2 v = sq[jj];
+            ex.x -= dx * v.x * c;
+            ex.y -= dx * v.y * c;
+            ey.x -= dy * v.x * c;
+            ey.y -= dy * v.y * c;
+            ez.x -= dz * v.x * c;
+            ez.y -= dz * v.y * c;
+        }
+        __syncthreads();
+    }
+    if (p < n_points) {
+        const double rp = sqrt((px - mx) * (px - mx) + (py - my) * (py - my) + (pz - mz) * (pz - mz));
+        const bool out = rp > rl;
+        if (out) {
+            if (i_pol == 0) ex.x += e0;
+            if (i_pol == 1) ey.x += e0;
+            if (i_pol == 2) ez.x += e0;
+        }
+        const double2 z = make_double2(0.0, 0.0);
+        E[3 * (size_t)p] = out ? ex : z;
+        E[3 * (size_t)p + 1] = out ? ey : z;
+        E[3 * (size_t)p + 2] = out ? ez : z;
+    }
+}
+
 // Sum of the per-slice partial results, in slice order (deterministic):
 // parts[s * count + i] for s < n_slices.
 __global__ void k_sum_slices(size_t count, int n_slices, const double2 *__restrict__ parts,
@@ -265,6 +336,28 @@ void plasmon_density(pxh_ctx *c, int n_points, const double *d_pts, const double
                                                         d_rho_mu);
     PXH_HIP(hipGetLastError());
     PXH_HIP(hipStreamSynchronize(c->stream));  // before the partials are freed
+}
+
+void bem_induced_field(pxh_ctx *c, int n_tess, const double *h_centroids, const double2 *d_q, int n_points,
+                       const double *d_pts, int i_pol, double e0, double2 *d_E) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n_tess; ++i)
+        for (int k = 0; k < 3; ++k) m[k] += h_centroids[3 * i + k] / n_tess;
+    std::vector<double4> ci(n_tess);
+    for (int i = 0; i < n_tess; ++i) {
+        const double *x = h_centroids + 3 * i;
+        const double r = std::sqrt((x[0] - m[0]) * (x[0] - m[0]) + (x[1] - m[1]) * (x[1] - m[1]) +
+                                   (x[2] - m[2]) * (x[2] - m[2]));
+        ci[i] = make_double4(x[0], x[1], x[2], r + 0.5 * kToBohr);
+    }
+    DevBuf<double4> dci;
+    dci.upload(ci.data(), (size_t)n_tess, c->stream);
+    const int blocks = (n_points + kFieldTile - 1) / kFieldTile;
+    if (blocks > 0)
+        k_bem_field<<<blocks, kFieldTile, 0, c->stream>>>(n_points, d_pts, n_tess, dci.p, d_q, m[0], m[1], m[2], i_pol,
+                                                          e0, d_E);
+    PXH_HIP(hipGetLastError());
+    PXH_HIP(hipStreamSynchronize(c->stream));  // before the centroids are freed
 }
 
 void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,

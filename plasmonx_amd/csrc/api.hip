@@ -274,6 +274,26 @@ int pxh_induced_field(pxh_ctx *c, int n_points, const double *points_bohr, const
     });
 }
 
+int pxh_bem_induced_field(pxh_ctx *c, int n_tess, const double *centroids_bohr, const double *charges, int n_points,
+                          const double *points_bohr, int i_pol, double e0, double *field) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(n_tess > 0 && n_points >= 0 && n_points <= INT_MAX - 256 && i_pol >= 0 && i_pol < 3, PXH_EINVAL,
+                  "bad tessera or point count or field direction");
+        PXH_CHECK(centroids_bohr && charges && (n_points == 0 || (points_bohr && field)), PXH_EINVAL, "null array");
+        if (n_points == 0) return;
+        const size_t np = (size_t)n_points;
+        pxh::DevBuf<double> dp;
+        pxh::DevBuf<double2> dq, de;
+        dp.upload(points_bohr, 3 * np, c->stream);
+        dq.upload(reinterpret_cast<const double2 *>(charges), (size_t)n_tess, c->stream);
+        de.alloc(3 * np);
+        pxh::bem_induced_field(c, n_tess, centroids_bohr, dq.p, n_points, dp.p, i_pol, e0, de.p);
+        PXH_HIP(hipMemcpyAsync(field, de.p, sizeof(double2) * 3 * np, hipMemcpyDeviceToHost, c->stream));
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 int pxh_plasmon_density(pxh_ctx *c, int n_points, const double *points_bohr, const double *variables, double *rho,
                         double *rho_q, double *rho_mu) {
     if (!c) return PXH_EINVAL;

@@ -209,6 +209,8 @@ struct SysParam {
 // complex (double2); sys_list holds the systems to process (device).
 void induced_field(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, int i_pol, double e0,
                    double2 *d_E);
+void bem_induced_field(pxh_ctx *c, int n_tess, const double *h_centroids, const double2 *d_q, int n_points,
+                       const double *d_pts, int i_pol, double e0, double2 *d_E);
 void plasmon_density(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, double2 *d_rho,
                      double2 *d_rho_q, double2 *d_rho_mu);
 void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,

@@ -117,6 +117,17 @@ module pxh
          complex(c_double_complex), intent(out) :: field(*)
       end function
 
+      integer(c_int) function pxh_bem_induced_field(ctx, n_tess, centroids_bohr, charges, n_points, points_bohr, &
+            i_pol, e0, field) bind(C, name="pxh_bem_induced_field")
+         import :: c_int, c_double, c_double_complex, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n_tess, n_points, i_pol
+         real(c_double), intent(in) :: centroids_bohr(*), points_bohr(*)
+         complex(c_double_complex), intent(in) :: charges(*)
+         real(c_double), value :: e0
+         complex(c_double_complex), intent(out) :: field(*)
+      end function
+
       integer(c_int) function pxh_plasmon_density(ctx, n_points, points_bohr, variables, rho, rho_q, rho_mu) &
             bind(C, name="pxh_plasmon_density")
          import :: c_int, c_double, c_double_complex, c_ptr

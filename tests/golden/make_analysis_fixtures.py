@@ -50,6 +50,7 @@ CASES = {
     "analysis_field_3d_x_y_z_wfqfmu_ag": ("3.0000000000", ["x", "y", "z"], "field"),
     "analysis_density_3d_x_wfq_sodium": ("1.5800000000", ["x"], "density"),
     "analysis_density_3d_y_wfqfmu_au": ("3.0000000000", ["y"], "density"),
+    "analysis_field_3d_y_bem": ("1.9100000000", ["y"], "field"),
 }
 SAMPLE = 600  # cube values kept per direction
 
@@ -61,6 +62,12 @@ def read_info(text):
         if ":" in ln:
             k, v = ln.split(":", 1)
             head[k.strip()] = v.strip()
+    if head["Model"] == "bem":  # tessera centroids only
+        n = int(head["NVar"])
+        i0 = next(i for i, ln in enumerate(lines) if ln.startswith("Input Geometry"))
+        coord = [[float(v) for v in ln.split()[1:4]] for ln in lines[i0 + 1:i0 + 1 + n]]
+        return {"model": "bem", "kernel": head["Kernel"], "e0": float(head["Input |E|"]), "symbols": [],
+                "coord_ang": coord, "types": [], "dynamic": [], "n_var": n}
     n = int(head["NAtoms"])
     i0 = next(i for i, ln in enumerate(lines) if ln.startswith("Input Geometry"))
     symbols, coord, tidx = [], [], []
@@ -125,6 +132,7 @@ PLANE_CASES = {
     "analysis_density_2d_x_wfq_sodium": ("1.5800000000", "x", "density", "xy", ["6.35"], None),
     "analysis_density_2d_x_wfq_sodium_n_planes": ("1.5800000000", "x", "density", "xy", ["6.35", "6.85", "7.35"],
                                                   None),
+    "analysis_field_2d_y_xy_bem": ("1.9100000000", "y", "field", "xy", ["0.00"], None),
 }
 VOLUME_CASES = {
     # case: (frequency tag, field direction, points per axis, user min, user max)
@@ -139,7 +147,7 @@ def read_target(ref, case, tag):
     with tarfile.open(os.path.join(tdir, case + ".tar.gz")) as tf:
         info = read_info(tf.extractfile(case + ".info").read().decode())
         n = len(info["symbols"])
-        n_var = 4 * n if info["model"] == "wfqfmu" else n
+        n_var = info.get("n_var") or (4 * n if info["model"] == "wfqfmu" else n)
         var = read_freq(tf.extractfile(f"{case}-{tag}.freq").read().decode(), n_var)
     refdir = os.path.join(tdir, "reference")
     sub = [d for d in os.listdir(refdir) if os.path.isdir(os.path.join(refdir, d))][0]
@@ -150,6 +158,7 @@ def target_fields(case, info):
     return {"case": case, "model": info["model"], "kernel": info["kernel"], "e0": info["e0"],
             "symbols": info["symbols"], "coord_ang": info["coord_ang"], "types": info["types"],
             "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
+            # BEM: coord_ang are the tessera centroids and the variables their charges
             "grid_offset_ang": 10.0}
 
 

@@ -150,3 +150,28 @@ def test_effective_volume_gpu(case):
         ctx.set_atoms(s)
         f = field_enhancement(ctx, pts, var, "xyz".index(d), fx["e0"])
     check_volume(fx, lo, hi, step, pts, f)
+
+
+def test_bem_field_cube_and_plane():
+    """pxh_bem_induced_field on the whole BEM cube and plane: equal to the
+    oracle (zeros inside included) and to the reference's printed values."""
+    from test_analysis_bem import bem_case, bem_cube_points, bem_plane_points
+    with Context(0) as ctx:
+        fx, c, q, i_pol = bem_case("analysis_field_3d_y_bem")
+        _, _, pts = bem_cube_points(fx)
+        got = ctx.bem_induced_field(c, q, pts * TOBOHR, i_pol, fx["e0"])
+        ref = pxo.bem_induced_field(c, q, pts * TOBOHR, i_pol, fx["e0"])
+        assert np.array_equal(got == 0, ref == 0)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+        cube = next(iter(fx["cube"].values()))
+        idx = np.array(cube["idx"])
+        f = np.sqrt((np.abs(got[idx]) ** 2).sum(axis=1)) / fx["e0"]
+        val = np.array(cube["values"])
+        assert np.all(np.abs(f - val) <= 1.01 * half_ulp(val))
+
+        fx, c, q, i_pol = bem_case("analysis_field_2d_y_xy_bem")
+        pl = fx["planes"][0]
+        pts = bem_plane_points(fx, pl)
+        got = ctx.bem_induced_field(c, q, pts * TOBOHR, i_pol, fx["e0"])
+        f = np.sqrt((np.abs(got[np.array(pl["idx"])]) ** 2).sum(axis=1)) / fx["e0"]
+        assert np.allclose(f, pl["values"], rtol=1e-10, atol=0)
