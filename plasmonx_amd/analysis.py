@@ -119,6 +119,20 @@ def effective_volume(enhancement, lo_ang, hi_ang, step_ang):
     return vol / 1e3, vol / side[k] / 1e2, side[k], "XYZ"[k]
 
 
+def write_plt(path, lo_ang, hi_ang, n, values):
+    """Binary grid in the layout of write_plt_file (plot_module.f90:452-495):
+    int32 3, 200, nz, ny, nx; float32 z min, z max, y min, y max, x min,
+    x max (Angstrom); then the values as float32 with x fastest.  values are
+    in cube order (x outermost, as cubic_grid returns the points)."""
+    v = np.asarray(values, dtype=np.float64).reshape(n[0], n[1], n[2])
+    head = np.array([3, 200, n[2], n[1], n[0]], dtype="<i4")
+    box = np.array([lo_ang[2], hi_ang[2], lo_ang[1], hi_ang[1], lo_ang[0], hi_ang[0]], dtype="<f4")
+    with open(path, "wb") as f:
+        f.write(head.tobytes())
+        f.write(box.tobytes())
+        f.write(np.ascontiguousarray(v.transpose(2, 1, 0)).astype("<f4").tobytes())
+
+
 def write_cube(path, atomic_numbers, coord_ang, lo_ang, step_ang, n, values):
     """Gaussian cube in the layout of write_cube_file (plot_module.f90:496-562):
     header in bohr, atoms, then the values x outermost / z innermost, six

@@ -139,6 +139,13 @@ VOLUME_CASES = {
     "analysis_field_volume_x_wfq_sodium": ("1.5800000000", "x", 30, [53.0, 0.0, 0.0], [55.0, 10.0, 10.0]),
     "analysis_field_volume_z_wfqfmu_au": ("3.0000000000", "z", 30, [10.0, -4.0, -4.0], [12.0, 4.0, 4.0]),
 }
+PLT_CASES = {
+    # case: (frequency tag, field directions, quantity)
+    "analysis_field_3d_x_wfq_sodium_plt": ("1.5800000000", ["x"], "field"),
+    "analysis_field_3d_x_y_z_wfqfmu_ag_plt": ("3.0000000000", ["x", "y", "z"], "field"),
+    "analysis_density_3d_x_wfq_sodium_plt": ("1.5800000000", ["x"], "density"),
+    "analysis_density_3d_y_wfqfmu_au_plt": ("3.0000000000", ["y"], "density"),
+}
 PLANE_KEEP = 1000  # planes with more points keep an even sample of SAMPLE
 
 
@@ -160,6 +167,19 @@ def target_fields(case, info):
             "dynamic": info["dynamic"], "r0_bohr": {s: R0_BOHR[s] for s in set(info["symbols"])},
             # BEM: coord_ang are the tessera centroids and the variables their charges
             "grid_offset_ang": 10.0}
+
+
+def read_plt(data):
+    """write_plt_file (plot_module.f90:452-495): int32 3, 200, nz, ny, nx;
+    float32 z min/max, y min/max, x min/max (Angstrom); float32 values with x
+    fastest."""
+    import struct
+    ints = struct.unpack("<5i", data[:20])
+    box = struct.unpack("<6f", data[20:44])
+    n = ints[2] * ints[3] * ints[4]
+    vals = struct.unpack("<%df" % n, data[44:44 + 4 * n])
+    assert len(data) == 44 + 4 * n
+    return list(ints), list(box), list(vals)
 
 
 def read_csv(text):
@@ -233,6 +253,26 @@ def main():
         with open(os.path.join(here, case + ".json"), "w") as f:
             json.dump(fx, f)
         print(case, len(info["symbols"]), "atoms", [len(p["idx"]) for p in out], "plane values")
+
+    for case, (tag, dirs, what) in PLT_CASES.items():
+        info, var, rdir = read_target(a.ref, case, tag)
+        out, head = {}, None
+        for d in dirs:
+            parts = {"values": ""} if what == "field" else {"re": "-densityRe", "im": "-densityIm"}
+            read = {}
+            for key, suffix in parts.items():
+                with open(os.path.join(rdir, f"{case}-E{d}-{tag}{suffix}.plt"), "rb") as f:
+                    ints, box, read[key] = read_plt(f.read())
+                head = head or {"ints": ints, "box_ang": box}
+            total = len(next(iter(read.values())))
+            extra = [max(range(total), key=v.__getitem__) for v in read.values()]
+            idx = sorted(set(list(range(0, total, max(1, total // SAMPLE))) + extra))
+            out[d] = {"idx": idx, **{k: [v[i] for i in idx] for k, v in read.items()}}
+        fx = {**target_fields(case, info), "quantity": what, "plt": head, "variables": {d: var[d] for d in dirs},
+              "values": out}
+        with open(os.path.join(here, case + ".json"), "w") as f:
+            json.dump(fx, f)
+        print(case, len(info["symbols"]), "atoms", head, {d: len(out[d]["idx"]) for d in dirs}, "plt values")
 
     for case, (tag, d, n, lo, hi) in VOLUME_CASES.items():
         info, var, rdir = read_target(a.ref, case, tag)

@@ -121,3 +121,51 @@ def test_effective_volume_of_a_constant_field():
     box = np.prod(step) * 30 ** 3
     assert abs(vol * 1e3 - box) < 1e-9 and abs(area * 1e2 - box / 2.0) < 1e-9 and axis == "X"
     _ = cubic_grid  # the cubic grid of the default box is covered by the cube tests
+
+
+PLT_CASES = ["analysis_field_3d_x_wfq_sodium_plt", "analysis_field_3d_x_y_z_wfqfmu_ag_plt",
+             "analysis_density_3d_x_wfq_sodium_plt", "analysis_density_3d_y_wfqfmu_au_plt"]
+
+
+def plt_check(fx, path, d, vals_cube_order):
+    """The written .plt against the reference's: the header exactly, the
+    sampled float32 values within one float32 rounding of the reference."""
+    from plasmonx_amd.analysis import write_plt
+    lo, step, _ = cubic_grid(fx["coord_ang"], (30, 30, 30), fx["grid_offset_ang"])
+    hi = lo + 29 * step
+    write_plt(path, lo, hi, (30, 30, 30), vals_cube_order)
+    data = np.fromfile(path, dtype="<i4", count=5)
+    assert list(data) == fx["plt"]["ints"]
+    box = np.fromfile(path, dtype="<f4", count=11)[5:]
+    assert np.array_equal(box, np.array(fx["plt"]["box_ang"], dtype=np.float32))
+    got = np.fromfile(path, dtype="<f4")[11:]
+    return got
+
+
+def plt_values(fx, d, field_fn, density_fn):
+    """{part: values in cube order} on the 30^3 grid."""
+    _, _, pts = cubic_grid(fx["coord_ang"], (30, 30, 30), fx["grid_offset_ang"])
+    var = np.array([complex(a, b) for a, b in fx["variables"][d]])
+    if fx["quantity"] == "field":
+        e = field_fn(pts * TOBOHR, var, "xyz".index(d), fx["e0"])
+        return {"values": np.sqrt((np.abs(e) ** 2).sum(axis=1)) / fx["e0"]}
+    rho = density_fn(pts * TOBOHR, var)
+    return {"re": rho.real, "im": rho.imag}
+
+
+def check_plt_case(fx, tmp_path, field_fn, density_fn):
+    for d, ref in fx["values"].items():
+        for part, v in plt_values(fx, d, field_fn, density_fn).items():
+            got = plt_check(fx, tmp_path / f"{d}{part}.plt", d, v)
+            idx = np.array(ref["idx"])
+            want = np.array(ref[part], dtype=np.float32)
+            ulp = np.spacing(np.abs(want))
+            assert np.all(np.abs(got[idx].astype(np.float64) - want) <= ulp), (d, part)
+
+
+@pytest.mark.parametrize("case", PLT_CASES)
+def test_plt_oracle(case, tmp_path):
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    check_plt_case(fx, tmp_path, lambda p, v, i, e0: pxo.induced_field(s, p, v, i, e0),
+                   lambda p, v: pxo.plasmon_density(s, p, v)[0])

@@ -175,3 +175,13 @@ def test_bem_field_cube_and_plane():
         got = ctx.bem_induced_field(c, q, pts * TOBOHR, i_pol, fx["e0"])
         f = np.sqrt((np.abs(got[np.array(pl["idx"])]) ** 2).sum(axis=1)) / fx["e0"]
         assert np.allclose(f, pl["values"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("case", ["analysis_field_3d_x_y_z_wfqfmu_ag_plt", "analysis_density_3d_y_wfqfmu_au_plt"])
+def test_plt_gpu(case, tmp_path):
+    from test_analysis_planes import check_plt_case
+    fx = load(case)
+    s = system_from_analysis_fixture(fx)
+    with Context(0) as ctx:
+        ctx.set_atoms(s)
+        check_plt_case(fx, tmp_path, ctx.induced_field, ctx.plasmon_density)
