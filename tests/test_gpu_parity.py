@@ -303,3 +303,19 @@ def test_bem_ief_iterative_agrees_with_inversion(gpu_ctx, case):
     ref = pxo.bem_solve_spectrum(m, freq, eps, fx["e0"], method=0)
     assert np.all(got["true_residual"] < tol)
     assert relerr(got["polar"], ref["polar"]) < 1e-6
+
+
+@pytest.mark.parametrize("case", ["wfqfmu_ag_inversion", "wfqfmu_hetero_agau_inversion"])
+def test_yaml_input_to_log_results(gpu_ctx, case):
+    """A reference input file read by plasmonx_amd.inputs, solved by libpxh:
+    the log's result columns (E14.6) to 1e-5."""
+    import os
+    from plasmonx_amd.inputs import read_input, system_from_case
+    c = read_input(os.path.join(os.path.dirname(__file__), "golden", "inputs", case, case + ".yaml"))
+    s = system_from_case(c)
+    freq = np.array(c["freq_au"])
+    gpu_ctx.set_atoms(s)
+    gpu_ctx.set_alpha(s.alpha_table(freq))
+    got = gpu_ctx.solve_spectrum(s.n_var, freq, c["e0"], method=_lib.INVERSION)
+    exp = np.array(load_fixture(case)["expected"]["results"])
+    assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5, 1e-6).all()
