@@ -12,7 +12,11 @@
  *   - invariants: Fermi f(s*r0) = 1/2, K rows sum to zero, T symmetric,
  *     sum of charges = 0 for every solution, T -> point-charge / point-dipole
  *     limits for r >> R, on-the-fly matvec == assembled matrix times x;
- *   - cross sections recomputed from the polarizability (closed form).
+ *   - cross sections recomputed from the polarizability (closed form);
+ *   - induced field and plasmon density at points: the cubes (E14.6), plane
+ *     CSVs (E25.16), .plt grids and effective volumes the reference's
+ *     analysis tests printed; the point-dipole limit of the field far from a
+ *     +-q pair; the density's zeroth and first moments (q, q r_i + mu_i).
  */
 #include "pxo.h"
 

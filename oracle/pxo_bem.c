@@ -14,7 +14,10 @@
  *     near-field rule) enters the printed D and is pinned through it;
  *   - the accurate-mode sum rule sum_k A_k D_ki = 2 pi A_i (exact by
  *     construction of the diagonal) and the quasi-static sphere limit
- *     alpha -> R^3 (eps - 1)/(eps + 2) on an icosphere (within 2 %).
+ *     alpha -> R^3 (eps - 1)/(eps + 2) on an icosphere (within 2 %);
+ *   - the field of the surface charges at points (tests/test_analysis_bem.py):
+ *     the analysis_field_3d_y_bem cube and analysis_field_2d_y_xy_bem CSV,
+ *     zeros inside included, and the Coulomb field of one charge.
  */
 #include "pxo.h"
 
