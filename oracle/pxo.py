@@ -149,6 +149,25 @@ def rhs(s, e0, freq=0.0, rhs_form=1):
     return r
 
 
+def t_qq(s):
+    """The charge-charge block T_qq (n x n) of the target's kernel."""
+    lib = load()
+    oa = OracleAtoms(s)
+    T = np.zeros((oa.n, oa.n), order="F")
+    lib.pxo_t_qq(oa.ref, P(T))
+    return T
+
+
+def static_rhs_q(s, e0, rhs_form=1):
+    """The static potential V (n, 3) of the external field, 'field' (1) or
+    'potential' (2) form."""
+    lib = load()
+    oa = OracleAtoms(s)
+    V = np.zeros((oa.n, 3), order="F")
+    lib.pxo_static_rhs_q(oa.ref, ctypes.c_double(e0), ctypes.c_int(rhs_form), P(V))
+    return V
+
+
 def kbar(s):
     lib = load()
     oa = OracleAtoms(s)

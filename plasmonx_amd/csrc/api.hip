@@ -138,52 +138,67 @@ int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *
                   "null input array");
         PXH_CHECK(kernel != PXH_GAUSSIAN || r_q, PXH_EINVAL, "the Gaussian kernel needs r_q");
         PXH_CHECK(model != PXH_WFQFMU || (r_q && r_mu), PXH_EINVAL, "omegaFQFmu needs r_q and r_mu");
-        pxh::Atoms &A = c->atoms;
-        A.model = model;
-        A.kernel = kernel;
-        A.n = n_atoms;
-        A.nt = n_types;
-        A.h_coord.assign(coord_bohr, coord_bohr + 3 * (size_t)n_atoms);
-        A.h_type.resize(n_atoms);
+        // every check runs on local copies: a rejected call leaves the
+        // context's current target as it was
+        std::vector<int> type(n_atoms);
         std::vector<int> seen(n_types, 0);
         for (int i = 0; i < n_atoms; ++i) {
             PXH_CHECK(type_of_atom[i] >= 1 && type_of_atom[i] <= n_types, PXH_EINVAL, "type_of_atom out of range");
-            A.h_type[i] = type_of_atom[i] - 1;
-            seen[A.h_type[i]] = 1;
+            type[i] = type_of_atom[i] - 1;
+            seen[type[i]] = 1;
         }
         int used = 0;
         for (int t = 0; t < n_types; ++t) used += seen[t];
-        A.hetero = used > 1;
+        const bool hetero = used > 1;
         // plain omegaFQ has no alloy form (wfq_module.f90:203-204)
-        PXH_CHECK(!(A.hetero && model == PXH_WFQ), PXH_EINVAL, "heterogeneous omegaFQ is not supported");
+        PXH_CHECK(!(hetero && model == PXH_WFQ), PXH_EINVAL, "heterogeneous omegaFQ is not supported");
         auto cp = [&](const double *p, size_t k) { return p ? std::vector<double>(p, p + k) : std::vector<double>(k, 0.0); };
-        A.eta = cp(eta, n_types);
-        A.rq = cp(r_q, n_types);
-        A.rmu = cp(r_mu, n_types);
-        A.aij = cp(a_ij, n_types);
-        A.tau = cp(tau, n_types);
-        A.dens = cp(density, n_types);
-        A.fd = cp(fermi_d, (size_t)n_types * n_types);
-        A.fs = cp(fermi_s, (size_t)n_types * n_types);
-        A.r0 = cp(r0, (size_t)n_types * n_types);
+        std::vector<double> v_eta = cp(eta, n_types), v_rq = cp(r_q, n_types), v_rmu = cp(r_mu, n_types),
+                            v_aij = cp(a_ij, n_types), v_tau = cp(tau, n_types), v_dens = cp(density, n_types);
         for (int t = 0; t < n_types; ++t) {
-            PXH_CHECK(A.tau[t] > 0.0 && A.dens[t] > 0.0, PXH_EINVAL, "tau and density must be positive");
-            if (kernel == PXH_GAUSSIAN) PXH_CHECK(A.rq[t] > 0.0, PXH_EINVAL, "r_q must be positive");
-            if (model == PXH_WFQFMU) PXH_CHECK(A.rq[t] > 0.0 && A.rmu[t] > 0.0, PXH_EINVAL, "r_q, r_mu must be positive");
+            PXH_CHECK(v_tau[t] > 0.0 && v_dens[t] > 0.0, PXH_EINVAL, "tau and density must be positive");
+            if (kernel == PXH_GAUSSIAN) PXH_CHECK(v_rq[t] > 0.0, PXH_EINVAL, "r_q must be positive");
+            if (model == PXH_WFQFMU) PXH_CHECK(v_rq[t] > 0.0 && v_rmu[t] > 0.0, PXH_EINVAL, "r_q, r_mu must be positive");
         }
-        A.h_nbfrac.assign((size_t)n_types * n_atoms, 0.0);
+        std::vector<double> nbfrac((size_t)n_types * n_atoms, 0.0);
         for (int i = 0; i < n_atoms; ++i) {
             if (nb_counts && nb_total) {
                 PXH_CHECK(nb_total[i] > 0, PXH_EINVAL, "nb_total must be positive");
                 for (int t = 0; t < n_types; ++t)
-                    A.h_nbfrac[t + (size_t)n_types * i] =
-                        (double)nb_counts[t + (size_t)n_types * i] / (double)nb_total[i];
+                    nbfrac[t + (size_t)n_types * i] = (double)nb_counts[t + (size_t)n_types * i] / (double)nb_total[i];
             } else {
-                A.h_nbfrac[A.h_type[i] + (size_t)n_types * i] = 1.0;
+                nbfrac[type[i] + (size_t)n_types * i] = 1.0;
             }
         }
+        // from here on the context's atoms change: a failure below (device
+        // memory, HIP) leaves the context without a target
+        c->target = 0;
+        pxh::Atoms &A = c->atoms;
+        A.n = 0;
+        A.model = model;
+        A.kernel = kernel;
+        A.nt = n_types;
+        A.hetero = hetero;
+        A.h_coord.assign(coord_bohr, coord_bohr + 3 * (size_t)n_atoms);
+        A.h_type = std::move(type);
+        A.eta = std::move(v_eta);
+        A.rq = std::move(v_rq);
+        A.rmu = std::move(v_rmu);
+        A.aij = std::move(v_aij);
+        A.tau = std::move(v_tau);
+        A.dens = std::move(v_dens);
+        A.fd = cp(fermi_d, (size_t)n_types * n_types);
+        A.fs = cp(fermi_s, (size_t)n_types * n_types);
+        A.r0 = cp(r0, (size_t)n_types * n_types);
+        A.h_nbfrac = std::move(nbfrac);
+        A.n = n_atoms;
+        try {
+            pxh::build_atoms_device(c);
+        } catch (...) {
+            A.n = 0;
+            throw;
+        }
         c->target = model;
-        pxh::build_atoms_device(c);
     });
 }
 
@@ -206,6 +221,8 @@ int pxh_set_mesh(pxh_ctx *c, int variant, int green, int n_tess, const double *v
         PXH_CHECK(n_tess > 0 && verts9, PXH_EINVAL, "empty mesh");
         PXH_CHECK(green == PXH_ACCURATE || sphere_r_bohr > 0.0, PXH_EINVAL,
                   "the approximate Green function needs a sphere radius");
+        // a failure while the mesh is built leaves the context without a target
+        c->target = 0;
         pxh::Mesh &M = c->mesh;
         M.n = n_tess;
         M.variant = variant;
@@ -215,8 +232,8 @@ int pxh_set_mesh(pxh_ctx *c, int variant, int green, int n_tess, const double *v
         M.eps_solvent = eps_solvent;
         M.charge_constraint = charge_constraint != 0;
         M.h_verts.assign(verts9, verts9 + 9 * (size_t)n_tess);
-        c->target = PXH_BEM;
         pxh::build_mesh_device(c);
+        c->target = PXH_BEM;
     });
 }
 
@@ -235,6 +252,7 @@ int pxh_solve_spectrum(pxh_ctx *c, int n_freq, const double *freq_au, double e0,
     if (!c) return PXH_EINVAL;
     int status = PXH_OK;
     const int g = guarded(c, [&] {
+        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
         PXH_CHECK(n_freq > 0 && freq_au, PXH_EINVAL, "no frequencies");
         PXH_CHECK(pol_mask > 0 && pol_mask < 8, PXH_EINVAL, "pol_mask must select x, y and/or z");
         PXH_CHECK(method >= PXH_INVERSION && method <= PXH_ITERATIVE_OTF, PXH_EINVAL, "unknown method");
@@ -324,6 +342,7 @@ int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(n_vec > 0 && x && y, PXH_EINVAL, "bad vectors");
+        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
         const bool bem = c->target == PXH_BEM;
         const int nv = bem ? c->mesh.n : c->atoms.n_var();
         PXH_CHECK(nv > 0, PXH_EINVAL, "no target set");
@@ -358,6 +377,7 @@ int pxh_constant_matrix(pxh_ctx *c, double *m) {
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(m, PXH_EINVAL, "null output");
+        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
         if (c->target == PXH_BEM) {
             const size_t nn = (size_t)c->mesh.n * c->mesh.n;
             PXH_HIP(hipMemcpyAsync(m, c->mesh.D.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
@@ -378,6 +398,8 @@ int pxh_dynamic_rhs(pxh_ctx *c, double freq_au, double e0, int rhs_form, double 
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(rhs, PXH_EINVAL, "null output");
+        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
+        PXH_CHECK(rhs_form == PXH_RHS_FIELD || rhs_form == PXH_RHS_POTENTIAL, PXH_EINVAL, "unknown rhs_form");
         const bool bem = c->target == PXH_BEM;
         const int nv = bem ? c->mesh.n : c->atoms.n_var();
         PXH_CHECK(nv > 0, PXH_EINVAL, "no target set");

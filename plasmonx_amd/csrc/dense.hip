@@ -855,10 +855,13 @@ __global__ void k_fill_one(int n, double2 *__restrict__ X, size_t ldx_f) {
     if (r < n) X[ldx_f * f + r] = make_double2(1.0, 0.0);
 }
 
-// Charge constraint: x_k -= x1 * sum(x_k) / sum(x1)   (algorithm_module.f90:306-311)
-__global__ __launch_bounds__(256) void k_charge_constraint(int n, double2 *__restrict__ X, size_t ldx_f,
-                                                           const double2 *__restrict__ X1, size_t ldx1_f) {
+// Charge constraint: x_k -= x1 * sum(x_k) / sum(x1)   (algorithm_module.f90:306-311),
+// for the field directions k of pol_mask (the others hold no solution)
+__global__ __launch_bounds__(256) void k_charge_constraint(int n, int pol_mask, double2 *__restrict__ X,
+                                                           size_t ldx_f, const double2 *__restrict__ X1,
+                                                           size_t ldx1_f) {
     const int f = blockIdx.z, k = blockIdx.x;
+    if (!((pol_mask >> k) & 1)) return;
     double2 *x = X + ldx_f * f + (size_t)n * k;
     const double2 *x1 = X1 + ldx1_f * f;
     __shared__ double s[4][256];
@@ -1089,7 +1092,7 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
             k_fill_one<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, Z1.p, (size_t)n);
             k_lu_solve<<<dim3(1, 1, nf), kSolveThreads, 0, st>>>(n, A.p, (size_t)n * n, Z1.p, (size_t)n, n, 1);
             k_scatter_rows<<<dim3((n + 255) / 256, 1, nf), 256, 0, st>>>(n, perm, Z1.p, (size_t)n, X1.p, (size_t)n);
-            k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, X, (size_t)rhs_stride, X1.p, (size_t)n);
+            k_charge_constraint<<<dim3(3, 1, nf), 256, 0, st>>>(n, pol_mask, X, (size_t)rhs_stride, X1.p, (size_t)n);
         }
         PXH_HIP(hipGetLastError());
         std::vector<int> hi(nf);

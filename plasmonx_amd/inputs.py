@@ -39,56 +39,115 @@ def value_unit(v):
     return float(m.group(1).replace("d", "e").replace("D", "e")), (m.group(2).strip() or None)
 
 
+FS_TO_AU = 41.3413733351702      # atomic units of time per femtosecond
+BOHR_ANG = 0.529177210544        # Angstrom per bohr
+BOHR_NM = 0.0529177210544        # nm per bohr
+SIGMA0_SI_TO_AU = 0.00000021739  # S/m -> a.u.
+
+
 def freq_to_au(v):
-    """Frequency in a.u. from eV (default), nm or a.u."""
+    """Frequency in a.u. from eV (the default), nm, micron, cm-1, THz or a.u.
+    (validation.py:8-33)."""
     x, u = value_unit(v)
     u = (u or "ev").lower()
     if u == "ev":
         return x / AU_TO_EV
     if u == "nm":
         return (10000000.0 / AU_TO_CM) / x
+    if u in ("micron", "um"):
+        return (10000.0 / AU_TO_CM) / x
+    if u in ("cm-1", "cm^-1"):
+        return x / AU_TO_CM
+    if u == "thz":
+        return x / 6579.683920502
     if u in ("au", "a.u."):
         return x
     raise ValueError(f"unknown frequency unit {u!r}")
 
 
+# accepted units per parameter: unit -> conversion to a.u. (None stands for
+# no unit written); the reference divides by the size of the atomic unit
+# (eV, Angstrom, nm, s^-1) and multiplies by the atomic units per fs and per S/m
+def _div(s):
+    return lambda x: x / s
+
+
+def _mul(s):
+    return lambda x: x * s
+
+
+_ONE = lambda x: x  # noqa: E731
+_AU = {None: _ONE, "au": _ONE, "a.u.": _ONE}
+_UNITS = {
+    # atom_types (validation.py:35-77)
+    "eta": {**_AU, "ev": _div(AU_TO_EV)},
+    "chi": {**_AU, "ev": _div(AU_TO_EV)},
+    "alpha": {**_AU, **{u: _div(BOHR_NM ** 3) for u in ("nm^3", "nm3")},
+              **{u: _div(BOHR_ANG ** 3) for u in ("angstrom^3", "ang^3", "angstrom3", "ang3")}},
+    "rq": {**_AU, "angstrom": _div(BOHR_ANG), "ang": _div(BOHR_ANG), "nm": _div(BOHR_NM)},
+    # parameters (validation.py:79-140)
+    "tau": {**_AU, "fs": _mul(FS_TO_AU), "s": _mul(FS_TO_AU * 1e15)},
+    "gamma": {**_AU, "fs^-1": _div(FS_TO_AU), "fs-1": _div(FS_TO_AU), "s^-1": _div(FS_TO_AU * 1e15),
+              "s-1": _div(FS_TO_AU * 1e15)},
+    "sigma0": {None: _mul(SIGMA0_SI_TO_AU), "s/m": _mul(SIGMA0_SI_TO_AU), "au": _ONE, "a.u.": _ONE},
+    "a_ij": {**_AU, **{u: _div(BOHR_NM ** 2) for u in ("nm^2", "nm2")},
+             **{u: _div(BOHR_ANG ** 2) for u in ("angstrom^2", "ang2", "ang^2", "angstrom2")}},
+}
+_UNITS["rmu"] = _UNITS["rq"]
+
+
 def param_to_au(v, kind):
-    """A per-type parameter in atomic units (tau, sigma0, A_ij, w_p, R_q,
-    R_mu; eta, alpha and chi are taken as given)."""
-    x, u = value_unit(v)
-    u = None if u is None else u.lower()
-    if kind == "tau":
-        return x if u in (None, "au", "a.u.") else x * 41.3413733351702 * (1e15 if u == "s" else 1.0)
-    if kind == "sigma0":
-        return x * 0.00000021739 if u in (None, "s/m") else x
-    if kind == "a_ij":
-        if u in (None, "au", "a.u."):
-            return x
-        if u in ("ang2", "ang^2", "angstrom2", "angstrom^2"):
-            return x / 0.529177210544 ** 2
-        return x / 0.0529177210544 ** 2
+    """A per-type parameter in atomic units, with the units the reference
+    accepts for it (validation.py:35-140): eta and chi in eV; alpha in nm3 or
+    Angstrom3; R_q, R_mu in Angstrom or nm; tau in fs or s; gamma in fs-1 or
+    s-1; sigma0 in S/m (the default); A_ij in nm2 or Angstrom2; w_p in any
+    frequency unit.  Other units raise ValueError."""
     if kind == "w_p":
         return freq_to_au(v)
-    if kind in ("rq", "rmu"):
-        if u in (None, "au", "a.u."):
-            return x
-        return x / 0.529177210544 if u in ("ang", "angstrom") else x / 0.0529177210544
-    if kind in ("eta", "alpha", "chi"):
-        return x
-    raise ValueError(f"unknown parameter {kind!r}")
+    if kind not in _UNITS:
+        raise ValueError(f"unknown parameter {kind!r}")
+    x, u = value_unit(v)
+    key = None if u is None else u.lower()
+    if key not in _UNITS[kind]:
+        raise ValueError(f"unsupported unit {u!r} for {kind!r}")
+    return _UNITS[kind][key](x)
 
 
 def frequencies(field):
-    """The spectrum in a.u.: the external list, or min + step (i - 1)."""
+    """The spectrum in a.u. (field_module.f90:157-176): the external list, or
+    n = ceil((max - min) / step) + 1 points from min by step, with any three
+    of min, max, step and nfreq given (validation.py:823-850)."""
     if "external freq" in field:
         ef = field["external freq"]
+        if isinstance(ef, str):  # a comma-separated list, each value with its unit
+            ef = [e for e in ef.split(",") if e.strip()]
         ef = ef if isinstance(ef, list) else [ef]
         return [freq_to_au(f) for f in ef]
-    n = int(field.get("nfreq", 0))
-    lo_au, st_au = freq_to_au(field["min freq"]), freq_to_au(field["step freq"])
-    if n == 0:
-        n = int(round((freq_to_au(field["max freq"]) - lo_au) / st_au)) + 1
-    return [lo_au + st_au * i for i in range(n)]
+    keys = ("min freq", "max freq", "step freq")
+    have = [k for k in keys if k in field]
+    if len(have) + ("nfreq" in field) < 3:
+        raise ValueError("at least three of 'min freq', 'max freq', 'step freq', 'nfreq' are needed")
+    # the missing one of the three follows from the raw numbers as written,
+    # then min and step are converted (the count uses the written values, as
+    # the reference's validation does before field_module converts them)
+    raw = {k: value_unit(field[k]) for k in have}
+    unit = next((u for _, u in raw.values() if u), None)
+    x = {k: v[0] for k, v in raw.items()}
+    n = int(field["nfreq"]) if "nfreq" in field else None
+    if len(have) == 3:
+        n_calc = int(math.ceil((x["max freq"] - x["min freq"]) / x["step freq"])) + 1
+        if n is not None and n != n_calc:
+            raise ValueError("'nfreq' is not coherent with min, max and step")
+        n = n_calc
+    elif "max freq" not in x:
+        x["max freq"] = x["min freq"] + (n - 1) * x["step freq"]
+    elif "min freq" not in x:
+        x["min freq"] = x["max freq"] - (n - 1) * x["step freq"]
+    else:
+        x["step freq"] = (x["max freq"] - x["min freq"]) / (n - 1) if n > 1 else 0.0
+    u = "" if unit is None else " " + unit
+    lo, st = freq_to_au(f"{x['min freq']!r}{u}"), freq_to_au(f"{x['step freq']!r}{u}")
+    return [lo + st * i for i in range(n)]
 
 
 def read_alpha_csv(path, freqs_au):
@@ -124,6 +183,45 @@ def parse_geometry(text):
     return sym, xyz
 
 
+# atom types the reference fills in when the input's atom_types section does
+# not name them (python/input_/validation.py:952-979)
+PREDEFINED_ATOM_TYPES = {
+    "Au": {"chi": 0.0, "eta": 0.52685, "alpha": 39.5297, "rq": 3.4996572071713126, "rmu": 3.0710227258450864},
+    "Ag": {"chi": 0.0, "eta": 0.4829838, "alpha": 49.9843, "rq": 3.3775793621738925, "rmu": 2.368922250085755},
+    "C": {"chi": 0.0, "eta": 0.372124, "rq": 0.0},
+    "Na": {"chi": 0.0, "eta": 0.292000, "rq": 0.0},
+}
+# Drude and connectivity parameters of a type the input's parameters section
+# does not name (validation.py:1134-1163); tau in a.u., sigma0 in S/m
+PREDEFINED_PARAMETERS = {
+    "Au": {"tau": 318.018, "sigma0": 11834849.81, "a_ij": 9.61, "fermi_function": {"d": 12.0, "s": 1.1},
+           "permittivity": "gold etchegoin"},
+    "Ag": {"tau": 1633.608, "sigma0": 6.5313e7, "a_ij": 9.61, "fermi_function": {"d": 12.0, "s": 1.1},
+           "permittivity": "silver etchegoin"},
+    "C": {"tau": 170.0, "a_ij": 1.7424, "fermi_function": {"d": 100.0, "s": 1.2}, "fermi energy": 0.4},
+    "Na": {"tau": 1323.0, "sigma0": 2.39494681e7, "scaling sigma0-tau": 10.0, "a_ij": 12.07910025,
+           "fermi_function": {"d": 12.0, "s": 1.1}},
+}
+
+
+def drude_parameters(p, name):
+    """(tau, sigma0) in a.u. from the two of tau, sigma0, w_p, gamma a type
+    gives (carbon: tau or gamma alone), as validation.py:1268-1292:
+    tau = 1/gamma, sigma0 = w_p^2 tau / (4 pi)."""
+    tau = param_to_au(p["tau"], "tau") if "tau" in p else None
+    if tau is None and "gamma" in p:
+        tau = 1.0 / param_to_au(p["gamma"], "gamma")
+    if tau is None:
+        raise ValueError(f"type {name!r}: tau or gamma is needed")
+    if name == "C":
+        return tau, 0.0
+    if "sigma0" in p:
+        return tau, param_to_au(p["sigma0"], "sigma0")
+    if "w_p" in p:
+        return tau, param_to_au(p["w_p"], "w_p") ** 2 * tau / (4.0 * math.pi)
+    raise ValueError(f"type {name!r}: give (sigma0, tau), (w_p, tau), (w_p, gamma) or (sigma0, gamma)")
+
+
 def _type_key(k):
     return k.lower().capitalize() if len(k) <= 2 else k.upper()
 
@@ -153,11 +251,17 @@ def read_input(path):
     for s in sym:
         if s not in names:
             names.append(s)
-    params = {(k if k.lower().startswith("interaction") else _type_key(k)): v for k, v in y["parameters"].items()}
-    atom_types = {_type_key(k): v for k, v in y["atom_types"].items()}
+    given_params = y.get("parameters") or {}
+    params = {(k if k.lower().startswith("interaction") else _type_key(k)): v for k, v in given_params.items()}
+    atom_types = {_type_key(k): v for k, v in (y.get("atom_types") or {}).items()}
     types = []
     for nm in names:
-        p, at = params[nm], atom_types[nm]
+        if nm not in atom_types and nm not in PREDEFINED_ATOM_TYPES:
+            raise ValueError(f"{path}: no atom_types entry for {nm!r} and no predefined values")
+        if nm not in params and nm not in PREDEFINED_PARAMETERS:
+            raise ValueError(f"{path}: no parameters entry for {nm!r} and no predefined values")
+        at = atom_types.get(nm, PREDEFINED_ATOM_TYPES.get(nm))
+        p = params.get(nm, PREDEFINED_PARAMETERS.get(nm))
         t = {"name": nm, "eta": param_to_au(at["eta"], "eta"), "alpha": param_to_au(at.get("alpha", 0.0), "alpha")}
         rq = param_to_au(at.get("rq", 0.0), "rq")
         rmu = param_to_au(at.get("rmu", 0.0), "rmu")
@@ -166,16 +270,8 @@ def read_input(path):
         if model == "wfqfmu" and rmu == 0.0:
             rmu = (t["alpha"] / (3.0 * math.sqrt(math.pi / 2.0))) ** (1.0 / 3.0)
         t["rq"], t["rmu"] = rq, rmu
-        t["a_ij"] = param_to_au(p["A_ij"], "a_ij")
-        tau = param_to_au(p["tau"], "tau") if "tau" in p else None
-        if "gamma" in p and tau is None:
-            tau = 1.0 / param_to_au(p["gamma"], "tau")
-        if "sigma0" in p:
-            sigma0 = param_to_au(p["sigma0"], "sigma0")
-        elif "w_p" in p:
-            sigma0 = param_to_au(p["w_p"], "w_p") ** 2 * tau / (4.0 * math.pi)
-        else:
-            sigma0 = 0.0
+        t["a_ij"] = param_to_au(p["a_ij"] if "a_ij" in p else p["A_ij"], "a_ij")
+        tau, sigma0 = drude_parameters(p, nm)
         scaling = float(p.get("scaling sigma0-tau", 1.0)) or 1.0
         t["density"] = float(p["fermi energy"]) / (math.pi * AU_TO_EV) if nm == "C" else sigma0 / tau
         t["tau"] = tau / scaling
@@ -187,7 +283,7 @@ def read_input(path):
             if "wfqfmu file" in p and p["wfqfmu file"] != "none":
                 t["alpha_w"] = read_alpha_csv(os.path.join(d, p["wfqfmu file"]), freqs)
             else:
-                t["permittivity"] = p["permittivity"]
+                t["permittivity"] = p.get("permittivity", PREDEFINED_PARAMETERS.get(nm, {}).get("permittivity"))
         types.append(t)
     nt = len(names)
     fd = [[0.0] * nt for _ in range(nt)]

@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
-"""Copy four of the reference's test inputs into tests/golden/inputs.
+"""Copy some of the reference's test inputs into tests/golden/inputs.
 
 Run in a checkout that has the reference tree (default /root/reference):
 
     python tests/golden/copy_inputs.py [--ref /root/reference]
 
-Each case keeps its YAML input as it is and, of each alpha(omega) table it
-names, the rows at the case's frequencies (the reader matches rows by
-frequency, so the rest is not read).  Data only; nothing is run.
+Each case keeps its YAML input as it is, the xyz file it names and, of each
+alpha(omega) table it names, the rows at the case's frequencies (the reader
+matches rows by frequency, so the rest is not read).  For the minimal decks
+(the reference's defaults fill in the atom types and parameters) it also
+writes log.json: what the case's reference/*.log prints about the input
+(spectrum and per-type parameters, as printed) and the 20 result columns at
+up to 12 frequencies spread over the spectrum.  Data only; nothing is run.
 """
 import argparse
+import json
 import os
 import shutil
 import sys
@@ -21,13 +26,38 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from plasmonx_amd.inputs import AU_TO_EV, frequencies  # noqa: E402
 
 CASES = ["wfq_sodium_inversion", "wfqfmu_ag_inversion", "wfqfmu_hetero_agau_inversion", "wfqfmu_ag_diff_units"]
+MINIMAL = ["wfq_sodium_minimal_input", "wfq_graphene_minimal_input", "wfqfmu_ag_minimal_input",
+           "wfqfmu_au_minimal_input", "wfqfmu_hetero_agau_minimal_input", "wfqfmu_au_read_xyz"]
+HEADER_KEYS = ["Num. Freq.", "Min. Freq.", "Max. Freq.", "Step Freq.", "AtomType", "Eta", "Alpha", "R_q", "R_mu",
+               "tau", "sigma_0", "scaling sigma0-tau", "A_ij", "fermi_d", "fermi_s", "density"]
+
+
+def log_summary(path, keep=12):
+    """Header lines 'key : value' of the input echo, and the result rows."""
+    lines = open(path).read().splitlines()
+    header = []
+    for ln in lines:
+        if ln.strip().startswith("Results for w ="):
+            break
+        if ":" in ln:
+            k, v = ln.split(":", 1)
+            if k.strip() in HEADER_KEYS:
+                header.append([k.strip(), v.strip()])
+    rows = {}
+    for k, ln in enumerate(lines):
+        if ln.strip().startswith("Results for w =") and k + 2 < len(lines) and "Isotr. Real Polar." in lines[k + 2]:
+            ev = float(ln.split()[-2].replace("D", "E"))
+            rows[ev] = [float(lines[j].split("=")[1].split()[0].replace("D", "E")) for j in range(k + 2, k + 22)]
+    evs = sorted(rows)
+    pick = sorted(set(evs[int(round(i * (len(evs) - 1) / max(1, keep - 1)))] for i in range(min(keep, len(evs)))))
+    return {"header": header, "results_ev": pick, "results": [rows[e] for e in pick]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
-    for case in CASES:
+    for case in CASES + MINIMAL:
         src = os.path.join(a.ref, "tests", case, case + ".yaml")
         dst = os.path.join(HERE, "inputs", case)
         os.makedirs(dst, exist_ok=True)
@@ -35,7 +65,15 @@ def main():
         with open(src) as f:
             y = yaml.safe_load(f)
         ev = [w * AU_TO_EV for w in frequencies(y.get("field", {}))]
-        for p in y["parameters"].values():
+        geo = y.get("input_geometry")
+        if isinstance(geo, dict) and "external xyz file" in geo:
+            shutil.copy(os.path.join(a.ref, "tests", case, geo["external xyz file"]),
+                        os.path.join(dst, geo["external xyz file"]))
+        if case in MINIMAL:
+            log = os.path.join(a.ref, "tests", case, "reference", case + ".log")
+            with open(os.path.join(dst, "log.json"), "w") as f:
+                json.dump(log_summary(log), f, indent=0)
+        for p in (y.get("parameters") or {}).values():
             if isinstance(p, dict) and p.get("wfqfmu file", "none") != "none":
                 with open(os.path.join(a.ref, "tests", case, p["wfqfmu file"])) as f:
                     rows = [ln for ln in f if ln.strip() and not ln.startswith("#") and

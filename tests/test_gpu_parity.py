@@ -319,3 +319,33 @@ def test_yaml_input_to_log_results(gpu_ctx, case):
     got = gpu_ctx.solve_spectrum(s.n_var, freq, c["e0"], method=_lib.INVERSION)
     exp = np.array(load_fixture(case)["expected"]["results"])
     assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5, 1e-6).all()
+
+
+@pytest.mark.parametrize("case", ["wfq_sodium_minimal_input", "wfq_graphene_minimal_input", "wfqfmu_ag_minimal_input",
+                                  "wfqfmu_au_minimal_input", "wfqfmu_hetero_agau_minimal_input",
+                                  "wfqfmu_au_read_xyz"])
+def test_minimal_deck_to_log_results(gpu_ctx, case):
+    """The reference's minimal decks, whose atom types and parameters come
+    from the reference's defaults, read and solved by inversion at up to 12
+    of the logged frequencies: the result columns (E14.6) to 1e-5."""
+    import json
+    import os
+    from plasmonx_amd.inputs import read_input, system_from_case
+    d = os.path.join(os.path.dirname(__file__), "golden", "inputs", case)
+    c = read_input(os.path.join(d, case + ".yaml"))
+    with open(os.path.join(d, "log.json")) as f:
+        log = json.load(f)
+    ev = np.array(c["freq_au"]) * 27.211386245981
+    idx = [int(np.argmin(np.abs(ev - e))) for e in log["results_ev"]]
+    freq = np.array(c["freq_au"])[idx]
+    s = system_from_case(c)
+    for t in s.types:  # the etchegoin fits at the picked frequencies only
+        if t.alpha_w is not None:
+            t.alpha_w = np.asarray(t.alpha_w)[idx]
+    gpu_ctx.set_option("kbar_cutoff", 0)
+    gpu_ctx.set_atoms(s)
+    if s.model == "wfqfmu":
+        gpu_ctx.set_alpha(s.alpha_table(freq))
+    got = gpu_ctx.solve_spectrum(s.n_var, freq, c["e0"], method=_lib.INVERSION)
+    exp = np.array(log["results"])
+    assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5, 1e-6).all()
