@@ -137,6 +137,18 @@ int pxh_solve_spectrum(pxh_ctx *ctx, int n_freq, const double *freq_au, double e
  * pxh_set_permittivity (0-based). */
 int pxh_matvec(pxh_ctx *ctx, double freq_au, int i_freq, int n_vec, const double *x, double *y);
 
+/* y = A(omega_i) x as pxh_matvec, in FP64, for atomistic targets, with x and
+ * y in device memory the caller owns (n_var x n_vec complex each), and
+ * without any device allocation: the product's working memory is a 64 MiB
+ * array the library holds on each GPU it runs on until the process ends.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Returns PXH_ENOMEM when n_vec vectors need more than that
+ * array (about n_var x n_vec x (splits + 1) x 16 bytes); every call on one
+ * GPU shares the array, so calls that overlap on one GPU (other threads,
+ * other contexts' streams) race.  The results are complete on return. */
+int pxh_matvec_noalloc(pxh_ctx *ctx, double freq_au, int i_freq, int n_vec, const double *x_dev, double *y_dev);
+
 /* The frequency-independent real matrix, n_var x n_var column-major.
  * Replaces target_%construct_constant_matrix (wfq_module.f90:192-206,
  * wfqfmu_module.f90:275-323, bem_module.f90:678-782). */

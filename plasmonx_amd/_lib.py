@@ -30,6 +30,7 @@ EXPORTS = [
     "pxh_induced_field",
     "pxh_plasmon_density",
     "pxh_bem_induced_field",
+    "pxh_matvec_noalloc",
 ]
 
 
@@ -61,6 +62,7 @@ def load():
     lib.pxh_set_permittivity.argtypes = [P, I, P]
     lib.pxh_solve_spectrum.argtypes = [P, I, P, D, I, I, I, D, I, I, P, P, P, P, P]
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
+    lib.pxh_matvec_noalloc.argtypes = [P, D, I, I, P, P]
     lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
     lib.pxh_plasmon_density.argtypes = [P, I, P, P, P, P, P]
     lib.pxh_bem_induced_field.argtypes = [P, I, P, P, I, P, I, D, P]

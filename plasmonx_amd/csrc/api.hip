@@ -98,6 +98,12 @@ int pxh_ctx_create(int device, size_t hbm_budget_bytes, pxh_ctx **out) {
     c->budget = hbm_budget_bytes;
     const int st = guarded(c, [&] {
         PXH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        // the scratch array of pxh_matvec_noalloc on this device; a failure
+        // here is reported by that entry point, which looks it up again
+        try {
+            pxh::scratch_setup(device);
+        } catch (const Error &) {
+        }
         for (int k = 0; k < pxh::TimerRing::kN; ++k) {
             PXH_HIP(hipEventCreate(&c->timers.beg[k]));
             PXH_HIP(hipEventCreate(&c->timers.end[k]));

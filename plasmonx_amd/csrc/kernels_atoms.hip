@@ -683,11 +683,11 @@ int choose_splits(const pxh_ctx *c, int n, int groups) {
 
 // Second stage of every product: t = sum of the split partials (+ eta x_q),
 // then y_q = K-bar t_q - z_q x_q and y_mu = t_mu + Lambda x_mu.
+// part: the split partials [split][row][system]; tt: room for t [row][system].
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits, bool fp32_class) {
+                    int ld, int splits, bool fp32_class, double2 *part, double2 *tt) {
     Atoms &A = c->atoms;
     const int n = A.n, nv = A.n_var();
-    double2 *part = c->work_part.p, *tt = c->work_t.p;
     const size_t tot = (size_t)nv * nsys;
     k_reduce_splits<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(n, nv, nsys, splits, part, d_sys_list, x,
                                                                         ld, A.pos.p, A.d_eta.p, tt);
@@ -723,19 +723,44 @@ double far_field_r2(const Atoms &A, bool fp64) {
 
 namespace {
 
+// Launch shape of the pair sweep: source atoms per split and the number of
+// splits that hold atoms.
+struct SweepShape {
+    int j_chunk, splits;
+};
+
+SweepShape sweep_shape(const pxh_ctx *c, int n, int groups) {
+    const int splits = choose_splits(c, n, groups);
+    const int j_chunk = ((n + splits - 1) / splits + TB - 1) / TB * TB;
+    return {j_chunk, (n + j_chunk - 1) / j_chunk};
+}
+
+// Systems per pair-kernel pass of the FP64 product (see apply_atoms_fp64).
+int fp64_kv(const Atoms &A, int nsys) {
+    if (nsys == 1) return 1;
+    if (A.model == PXH_WFQ && nsys >= 8) return 8;
+    if (A.model == PXH_WFQFMU && nsys >= 4) return 4;
+    return 2;
+}
+
+// ext_part / ext_t: caller-provided room for the split partials and t (both
+// null: the context's product buffers, grown as needed).
 template <typename R, int KV>
 void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
-                      double2 *y, int ld) {
+                      double2 *y, int ld, double2 *ext_part = nullptr, double2 *ext_t = nullptr) {
     Atoms &A = c->atoms;
     const int n = A.n, nv = A.n_var();
     const bool fmu = A.model == PXH_WFQFMU;
     const int groups = (nsys + KV - 1) / KV;
-    const int splits = choose_splits(c, n, groups);
-    const int j_chunk = ((n + splits - 1) / splits + TB - 1) / TB * TB;
-    const int splits_eff = (n + j_chunk - 1) / j_chunk;
-    c->work_part.alloc((size_t)splits_eff * nsys * nv);
-    c->work_t.alloc((size_t)nsys * nv);
-    double2 *part = c->work_part.p;
+    const SweepShape sh = sweep_shape(c, n, groups);
+    const int j_chunk = sh.j_chunk, splits_eff = sh.splits;
+    double2 *part = ext_part, *tt = ext_t;
+    if (!part || !tt) {
+        c->work_part.alloc((size_t)splits_eff * nsys * nv);
+        c->work_t.alloc((size_t)nsys * nv);
+        part = c->work_part.p;
+        tt = c->work_t.p;
+    }
     dim3 grid((n + TB - 1) / TB, splits_eff, groups);
     const double rfar2 = far_field_r2(A, sizeof(R) == 8);
     const bool f32 = sizeof(R) == 4;
@@ -751,7 +776,7 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     // flops: per ordered pair (i != j) kPairEvalFlops for the interaction
     // elements plus kPairVecFlops per complex vector for the contraction
     timer_end(c, timer, f32 ? 3 : 4);
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, f32);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, f32, part, tt);
     const double pairs = (double)n * (double)(n - 1);
     const double fl = pairs * ((double)(fmu ? kPairEvalFlopsFmu : kPairEvalFlopsQ) * groups +
                                (double)(fmu ? kPairVecFlopsFmu : kPairVecFlopsQ) * nsys);
@@ -778,17 +803,31 @@ __global__ void k_stored_t(int nv, const double *__restrict__ T, int nsys, const
 
 }  // namespace
 
+// Systems per pass: omegaFQ (charges only) has room for 8; omegaFQFmu shares
+// each pair block over 4 (two waves per SIMD).
 void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
-                      double2 *y, int ld) {
+                      double2 *y, int ld, double2 *ext_part, double2 *ext_t) {
     if (nsys <= 0) return;
-    if (nsys == 1)
-        apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld);
-    else if (c->atoms.model == PXH_WFQ && nsys >= 8)  // charges only: room for 8 systems per pass
-        apply_atoms_impl<double, 8>(c, nsys, d_sys_list, d_par, x, y, ld);
-    else if (c->atoms.model == PXH_WFQFMU && nsys >= 4)  // 4 systems share each pair block (2 waves / SIMD)
-        apply_atoms_impl<double, 4>(c, nsys, d_sys_list, d_par, x, y, ld);
-    else
-        apply_atoms_impl<double, 2>(c, nsys, d_sys_list, d_par, x, y, ld);
+    switch (fp64_kv(c->atoms, nsys)) {
+        case 1:
+            apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
+            break;
+        case 8:
+            apply_atoms_impl<double, 8>(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
+            break;
+        case 4:
+            apply_atoms_impl<double, 4>(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
+            break;
+        default:
+            apply_atoms_impl<double, 2>(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
+    }
+}
+
+size_t fp64_product_partials(const pxh_ctx *c, int nsys) {
+    const Atoms &A = c->atoms;
+    const int kv = fp64_kv(A, nsys);
+    const SweepShape sh = sweep_shape(c, A.n, (nsys + kv - 1) / kv);
+    return (size_t)sh.splits * nsys * A.n_var();
 }
 
 void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
@@ -992,7 +1031,7 @@ void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPa
     k_stored_t<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(nv, A.Tstore.p, nsys, d_sys_list, x, ld,
                                                                     c->work_part.p);
     PXH_HIP(hipGetLastError());
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1, false);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1, false, c->work_part.p, c->work_t.p);
 }
 
 void constant_matrix_atoms(pxh_ctx *c, double *d_m) {

@@ -340,8 +340,6 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
 
 }  // namespace
 
-void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits, bool fp32_class);
 
 // Power-of-two scale of the T elements: the largest Gaussian-damped element
 // of any type pair (T_qq <= 2 kappa_qq / sqrt(pi), |g d| <= 0.45 kappa_qmu^2,
@@ -382,7 +380,7 @@ void launch_sweep(pxh_ctx *c, int iblocks, int splits, int per, float rfar2, flo
 }  // namespace
 
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits, bool fp32_class);
+                    int ld, int splits, bool fp32_class, double2 *part, double2 *t);
 
 namespace {
 
@@ -404,7 +402,7 @@ int tune_splits(pxh_ctx *c, const std::vector<int> &cand, int iblocks, int jstep
             const int per = (jsteps + sp - 1) / sp;
             PXH_HIP(hipEventRecord(e0, c->stream));
             launch_sweep<K>(c, iblocks, sp, per, rfar2, tsc, ns, nsys, 0);
-            finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sp, true);
+            finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sp, true, c->work_part.p, c->work_t.p);
             PXH_HIP(hipEventRecord(e1, c->stream));
             PXH_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -507,7 +505,7 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     }
     PXH_HIP(hipGetLastError());
     timer_end(c, t_product, 3);
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true, c->work_part.p, c->work_t.p);
     // algorithmic flops: the contraction of the 4N x 4N real T with 2 nsys real
     // columns (2 flops per element and column) plus the pair evaluation
     const double nn = (double)n * (double)(n - 1);

@@ -196,6 +196,9 @@ namespace pxh {
 int timer_begin(pxh_ctx *c);
 void timer_end(pxh_ctx *c, int t, int stat_index);
 void timer_drain(pxh_ctx *c);
+// noalloc.hip: looks up this device's scratch array of pxh_matvec_noalloc
+// (once per device, with the device current)
+void scratch_setup(int device);
 // kernels_atoms.hip
 void build_atoms_device(pxh_ctx *c);
 void build_kbar(pxh_ctx *c);
@@ -213,8 +216,12 @@ void bem_induced_field(pxh_ctx *c, int n_tess, const double *h_centroids, const 
                        const double *d_pts, int i_pol, double e0, double2 *d_E);
 void plasmon_density(pxh_ctx *c, int n_points, const double *d_pts, const double2 *d_var, double2 *d_rho,
                      double2 *d_rho_q, double2 *d_rho_mu);
+// ext_part / ext_t (both or neither): caller-provided room for the split
+// partials (fp64_product_partials(c, nsys) entries) and for t (nsys x n_var
+// entries); null: the context's product buffers.
 void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
-                      double2 *y, int ld);
+                      double2 *y, int ld, double2 *ext_part = nullptr, double2 *ext_t = nullptr);
+size_t fp64_product_partials(const pxh_ctx *c, int nsys);
 void apply_atoms_fp32(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld);
 void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,

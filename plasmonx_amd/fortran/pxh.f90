@@ -92,6 +92,16 @@ module pxh
          complex(c_double_complex), intent(out) :: y(*)
       end function
 
+      ! x, y: device addresses (n_var x n_vec complex each) the caller owns
+      integer(c_int) function pxh_matvec_noalloc(ctx, freq_au, i_freq, n_vec, x, y) &
+            bind(C, name="pxh_matvec_noalloc")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), value :: freq_au
+         integer(c_int), value :: i_freq, n_vec
+         type(c_ptr), value :: x, y
+      end function
+
       integer(c_int) function pxh_constant_matrix(ctx, m) bind(C, name="pxh_constant_matrix")
          import :: c_int, c_double, c_ptr
          type(c_ptr), value :: ctx
