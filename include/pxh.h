@@ -226,14 +226,19 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * source splits and keeps the fastest for the process). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
-/* Counters of the last pxh_solve_spectrum call (n <= 16): [0] vector
+/* Counters of the last pxh_solve_spectrum call (n <= 24): [0] vector
  * products, [1] FP32 block products, [2] FP64 block products, [3] / [4]
  * pair-kernel ms (FP32 / FP64, HIP events on the context's stream),
  * [5] wall ms, [6] pair evaluations, [7] outer refinements, [8] / [9]
  * pair-kernel launches (FP32 / FP64), [10] / [11] their algorithmic flops,
  * [12] / [13] ms and launches of the MFMA pair kernel alone, [14] / [15]
  * p50 / p99 service time of a frequency point in ms (admission of its first
- * field direction to convergence of its last; iterative methods). */
+ * field direction to convergence of its last; iterative methods),
+ * [16] / [17] / [18] ms, launches and algorithmic flops of the BEM matrix
+ * products (4 flop per matrix element and system), [19] / [20] / [21] ms,
+ * launches and flops of the dense LU's trailing updates (8 flop per complex
+ * multiply-add), [22] algorithmic bytes of the BEM matrix products (the
+ * matrix once plus each vector read and written). */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
 /* ------------------------------------------------------------------ serving

@@ -482,7 +482,7 @@ int pxh_get_stats(pxh_ctx *c, double *stats, int n) {
     if (!c || !stats) return PXH_EINVAL;
     const int st = guarded(c, [&] { pxh::timer_drain(c); });
     if (st != PXH_OK) return st;
-    for (int k = 0; k < n && k < 16; ++k) stats[k] = c->stats.v[k];
+    for (int k = 0; k < n && k < pxh::Stats::kN; ++k) stats[k] = c->stats.v[k];
     return PXH_OK;
 }
 

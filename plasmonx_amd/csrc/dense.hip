@@ -939,11 +939,21 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         else
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
     };
+    // every trailing-update launch timed on the context's stream, with its
+    // algorithmic flops: 8 real flop per complex multiply-add
+    auto timed_update = [&](double rows, double cols, double depth, auto launch) {
+        const int t = timer_begin(c);
+        launch();
+        timer_end(c, t, 19);
+        c->stats.v[20] += 1;
+        c->stats.v[21] += 8.0 * rows * cols * depth * nf;
+    };
     // trailing update of the rows and columns from k + depth
     auto trailing = [&](int k, int depth) {
         const int rest = n - k - depth;
         const int t = (rest + 63) / 64;
-        k_gemm_update<2, 2><<<dim3(t, t, nf), 256, 0, st>>>(n, k, depth, n, A, ldf);
+        timed_update(rest, rest, depth,
+                     [&] { k_gemm_update<2, 2><<<dim3(t, t, nf), 256, 0, st>>>(n, k, depth, n, A, ldf); });
     };
     int k = 0;
     while (k < n) {
@@ -955,7 +965,9 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         if (rest > NB) {
             // second panel of the pair, with a trailing part after it
             const int k2 = k + NB, rest2 = rest - NB;
-            k_gemm_update<4, 1><<<dim3((rest + 127) / 128, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
+            timed_update(rest, NB, NB, [&] {
+                k_gemm_update<4, 1><<<dim3((rest + 127) / 128, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
+            });
             panel(k2, NB);
             k_swap_trsm<<<dim3((rest2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapRight);
             k_swap_trsm<<<dim3((k2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapLeft);

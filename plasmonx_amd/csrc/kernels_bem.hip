@@ -539,6 +539,11 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
         k_gemm_pack<<<pack_blocks, 256, 0, st>>>(n, cpad, nsys, d_sys_list, xin, x_by_k ? n : ld, x_by_k, coltiles,
                                                  c->work_pack.p);
         double *part = ksplit > 1 ? c->work_gemm.p : nullptr;
+        // the matrix product alone (with its split-K reduction), timed on the
+        // context's stream; algorithmic work: 4 flop per matrix element and
+        // system (a real matrix on a complex vector), the matrix read once
+        // plus each system's vector read and written
+        const int t_gemm = timer_begin(c);
         if (fp32)
             k_bem_gemm<float><<<gemm_blocks, 256, 0, st>>>(n, cpad, spw, rowtiles, coltiles, ksplit, Mf,
                                                            c->work_pack.p, d_sys_list, nsys, o, part);
@@ -548,6 +553,10 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
         if (ksplit > 1)
             k_gemm_reduce<<<red_blocks, 256, 0, st>>>(n, rowtiles * kGemmRows, coltiles, ksplit, part, d_sys_list,
                                                       nsys, o);
+        timer_end(c, t_gemm, 16);
+        c->stats.v[17] += 1;
+        c->stats.v[18] += 4.0 * n * (double)n * nsys;
+        c->stats.v[22] += (double)n * n * (fp32 ? 4.0 : 8.0) + 32.0 * n * (double)nsys;
     };
     if (M.variant == PXH_DPCM) {
         // y = D x + lambda x

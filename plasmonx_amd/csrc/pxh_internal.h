@@ -143,7 +143,8 @@ struct Options {
 };
 
 struct Stats {
-    double v[16] = {};
+    static constexpr int kN = 24;
+    double v[kN] = {};
 };
 
 // Kernel timers that do not stall the host: a ring of event pairs recorded
