@@ -52,8 +52,10 @@ enum {
     PXH_ENODEV = 6
 };
 
-/* target models (reference target_%name_: wfq, wfqfmu, bem) */
-enum { PXH_WFQ = 1, PXH_WFQFMU = 2, PXH_BEM = 3 };
+/* target models (reference target_%name_: wfq, wfqfmu, bem; static fq, fqfmu) */
+enum { PXH_WFQ = 1, PXH_WFQFMU = 2, PXH_BEM = 3, PXH_FQ = 4, PXH_FQFMU = 5 };
+/* keyword 'what' of the static models: energy | static response */
+enum { PXH_ENERGY = 0, PXH_STATIC_FIELD = 1 };
 /* charge-charge kernels (keyword 'forcefield kernel') */
 enum { PXH_GAUSSIAN = 1, PXH_COULOMB = 2, PXH_OHNO = 3 };
 /* keyword 'algorithm method': inversion | iterative | iterative on the fly */
@@ -87,6 +89,38 @@ int pxh_set_atoms(pxh_ctx *ctx, int model, int kernel, int n_atoms, const double
                   const double *r_mu, const double *a_ij, const double *fermi_d, const double *fermi_s,
                   const double *r0, const double *tau, const double *density, const int32_t *nb_counts,
                   const int32_t *nb_total);
+
+/* Static FQ / FQFMu targets (model PXH_FQ / PXH_FQFMU in pxh_set_atoms, where
+ * a_ij, the Fermi tables, tau, density and the neighbour counts are unused
+ * and may be NULL): the electronegativities chi and, for FQFMu, the atomic
+ * polarizabilities alpha per type (a.u.), and the molecules as runs of
+ * consecutive atoms, atoms_per_molecule[n_mol] (each molecule's charges sum
+ * to zero: one Lagrange multiplier each).  Replaces the chi / alpha of
+ * target_ and create_n_atoms_per_molecule (input_module.f90:1007-1031); the
+ * reference's add_lagrangian_blocks (matrix_module.f90:351-382) places
+ * molecules correctly only when they all have the same size, this one for
+ * any sizes. */
+int pxh_set_static(pxh_ctx *ctx, int n_types, const double *chi, const double *alpha, int n_mol,
+                   const int32_t *atoms_per_molecule);
+
+/* Static solve of an FQ / FQFMu target, unknowns [q_1..q_N, lambda_1..
+ * lambda_M, mu_1x, mu_1y, mu_1z, ...] (n_var = N + n_mol, + 3N for FQFMu).
+ * Replaces algorithm%solve_ground_state and solve_static_field
+ * (algorithm_module.f90:95-148) with real_inversion (:469-504) and the
+ * models' right-hand sides, energies and polarizability
+ * (fq_module.f90:179-597, fqfmu_module.f90:81-349).
+ *   what:      PXH_ENERGY (right-hand side -chi; energy = -1/2 x^T b) or
+ *              PXH_STATIC_FIELD (three right-hand sides: rhs_form potential
+ *              of the field e0 along x, y, z on the charge rows, and for
+ *              FQFMu 1 on dipole component k, 0 when e0 is 0, as
+ *              rhs_module.f90:96-127 writes it);
+ *   variables: n_var x 1 (energy) or n_var x 3 (static field) real, may be NULL;
+ *   polar:     3 x 3 static polarizability of FQ (static field), may be NULL;
+ *              FQFMu returns PXH_EINVAL for it, as the reference stops there
+ *              (fqfmu_module.f90:340-347);
+ *   energy:    may be NULL. */
+int pxh_solve_static(pxh_ctx *ctx, int what, double e0, int rhs_form, double *variables, double *polar,
+                     double *energy);
 
 /* Interband polarizability alpha_t(omega) per type and frequency (complex,
  * n_types x n_freq).  Replaces read_alpha (wfqfmu_module.f90:97-166). */

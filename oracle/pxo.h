@@ -15,7 +15,7 @@
  *     [ti + nt*tj] (column-major like the reference's fermi_d(ti,tj)).
  *
  * What pins each function is listed in the header block of pxo_atoms.c,
- * pxo_linalg.c and pxo_bem.c, and in the tests under tests/.
+ * pxo_linalg.c, pxo_bem.c and pxo_static.c, and in the tests under tests/.
  */
 #ifndef PXO_H
 #define PXO_H
@@ -101,6 +101,16 @@ int pxo_solve_spectrum(const pxo_atoms *a, int n_freq, const double *freq,
                        pxo_cplx *polar /* 9*n_freq */,
                        double *results /* n_freq*20, column-major */,
                        int *iters /* 3*n_freq */);
+
+/* ---- static FQ / FQFmu (pxo_static.c); model PXO_WFQ = charges only (FQ),
+ * PXO_WFQFMU = charges and dipoles (FQFmu); what 0 = ground state (energy),
+ * 1 = static field; mol_count[n_mol]: atoms per molecule, contiguous runs ---- */
+int pxo_static_n_var(const pxo_atoms *a, int n_mol);
+void pxo_static_matrix(const pxo_atoms *a, const double *alpha, int n_mol, const int *mol_count, double *A);
+void pxo_static_rhs(const pxo_atoms *a, const double *chi, int n_mol, int what, double e0, int rhs_form,
+                    double *rhs);
+int pxo_static_solve(const pxo_atoms *a, const double *chi, const double *alpha, int n_mol, const int *mol_count,
+                     int what, double e0, int rhs_form, double *variables, double *polar, double *energy);
 
 /* ---- permittivity models (pxo_atoms.c) ---- */
 void pxo_silver_etchegoin(int polar, int n, const double *freq, pxo_cplx *out);

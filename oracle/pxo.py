@@ -58,7 +58,7 @@ class OracleAtoms:
             r0=f(np.asarray(s.r0).T.reshape(-1)), nb_counts=np.ascontiguousarray(s.nb_counts, dtype=np.int32),
             nb_total=np.ascontiguousarray(s.nb_total, dtype=np.int32))
         kern = {"gaussian": 1, "coulomb": 2, "ohno": 3}[s.kernel]
-        self.st = Atoms(2 if s.model == "wfqfmu" else 1, kern, s.n_atoms, s.n_types, int(s.heterogeneous),
+        self.st = Atoms(2 if s.model in ("wfqfmu", "fqfmu") else 1, kern, s.n_atoms, s.n_types, int(s.heterogeneous),
                         *[P(self.arrs[k]) for k in ("coord", "type", "eta", "r_q", "r_mu", "a_ij", "tau", "density",
                                                     "fermi_d", "fermi_s", "r0", "nb_counts", "nb_total")])
         self.n_var = s.n_var
@@ -290,3 +290,35 @@ def bem_induced_field(centroids_bohr, charges, points_bohr, i_pol, e0):
                               ctypes.c_int(i_pol), ctypes.c_double(e0), P(E))
     return E
 
+
+
+def static_matrix(s, mol_count):
+    """The static FQ / FQFmu matrix (n_var x n_var, unknowns [q, lambda, mu])."""
+    lib = load()
+    oa = OracleAtoms(s)
+    mc = np.ascontiguousarray(mol_count, dtype=np.int32)
+    alpha = np.ascontiguousarray(s.arr("alpha"), dtype=np.float64)
+    nv = lib.pxo_static_n_var(oa.ref, ctypes.c_int(len(mc)))
+    A = np.zeros((nv, nv), order="F")
+    lib.pxo_static_matrix(oa.ref, P(alpha), ctypes.c_int(len(mc)), P(mc), P(A))
+    return A
+
+
+def static_solve(s, mol_count, what, e0=0.0, rhs_form=1):
+    """Static FQ / FQFmu solve: what 0 = ground state, 1 = static field.
+    Returns (variables (n_var, ncol), polar (3, 3) or None, energy)."""
+    lib = load()
+    oa = OracleAtoms(s)
+    mc = np.ascontiguousarray(mol_count, dtype=np.int32)
+    chi = np.ascontiguousarray(s.arr("chi"), dtype=np.float64)
+    alpha = np.ascontiguousarray(s.arr("alpha"), dtype=np.float64)
+    nv = lib.pxo_static_n_var(oa.ref, ctypes.c_int(len(mc)))
+    ncol = 1 if what == 0 else 3
+    var = np.zeros((nv, ncol), order="F")
+    pol = np.zeros((3, 3), order="F")
+    en = ctypes.c_double(0.0)
+    st = lib.pxo_static_solve(oa.ref, P(chi), P(alpha), ctypes.c_int(len(mc)), P(mc), ctypes.c_int(what),
+                              ctypes.c_double(e0), ctypes.c_int(rhs_form), P(var), P(pol), ctypes.byref(en))
+    if st != 0:
+        raise RuntimeError(f"pxo_static_solve status {st}")
+    return var, (pol if what == 1 and s.model != "fqfmu" else None), en.value

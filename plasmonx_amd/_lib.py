@@ -14,7 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpxh.so")
 
 PXH_OK, PXH_EINVAL, PXH_ENOCONV, PXH_ESINGULAR, PXH_ENOMEM, PXH_EHIP, PXH_ENODEV = range(7)
-WFQ, WFQFMU, BEM = 1, 2, 3
+WFQ, WFQFMU, BEM, FQ, FQFMU = 1, 2, 3, 4, 5
+ENERGY, STATIC_FIELD = 0, 1
 GAUSSIAN, COULOMB, OHNO = 1, 2, 3
 INVERSION, ITERATIVE, ITERATIVE_OTF = 0, 1, 2
 FP64, MIXED = 0, 1
@@ -31,6 +32,7 @@ EXPORTS = [
     "pxh_plasmon_density",
     "pxh_bem_induced_field",
     "pxh_matvec_noalloc",
+    "pxh_set_static", "pxh_solve_static",
 ]
 
 
@@ -63,6 +65,8 @@ def load():
     lib.pxh_solve_spectrum.argtypes = [P, I, P, D, I, I, I, D, I, I, P, P, P, P, P]
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
     lib.pxh_matvec_noalloc.argtypes = [P, D, I, I, P, P]
+    lib.pxh_set_static.argtypes = [P, I, P, P, I, P]
+    lib.pxh_solve_static.argtypes = [P, I, D, I, P, P, P]
     lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
     lib.pxh_plasmon_density.argtypes = [P, I, P, P, P, P, P]
     lib.pxh_bem_induced_field.argtypes = [P, I, P, P, I, P, I, D, P]
@@ -136,6 +140,30 @@ class Context:
                                     _ptr(nbt))
         self._check(st)
         self.n_var = sysm.n_var
+
+    def set_static(self, sysm, mol_count):
+        """Static FQ / FQFMu data of the current target (set_atoms of a model
+        'fq' / 'fqfmu'): chi and alpha of its types, atoms per molecule."""
+        chi = np.ascontiguousarray(sysm.arr("chi"), dtype=np.float64)
+        alpha = np.ascontiguousarray(sysm.arr("alpha"), dtype=np.float64)
+        mc = np.ascontiguousarray(mol_count, dtype=np.int32)
+        self._check(self.lib.pxh_set_static(self._h, sysm.n_types, _ptr(chi), _ptr(alpha), mc.shape[0], _ptr(mc)))
+        self.n_static = sysm.n_atoms + mc.shape[0] + (3 * sysm.n_atoms if sysm.model == "fqfmu" else 0)
+
+    def solve_static(self, what, e0=0.0, rhs_form=RHS_FIELD, want_polar=True):
+        """Static solve: what ENERGY or STATIC_FIELD.  Returns a dict with
+        'variables' (n_var, ncol), 'energy' and, for FQ static field,
+        'polar' (3, 3)."""
+        ncol = 1 if what == ENERGY else 3
+        var = np.zeros((ncol, self.n_static), dtype=np.float64)  # column-major n_var x ncol
+        pol = np.zeros((3, 3), dtype=np.float64) if (want_polar and what == STATIC_FIELD) else None
+        en = np.zeros(1)
+        self._check(self.lib.pxh_solve_static(self._h, int(what), float(e0), int(rhs_form), _ptr(var), _ptr(pol),
+                                              _ptr(en)))
+        out = {"variables": var.T.copy(), "energy": float(en[0])}
+        if pol is not None:
+            out["polar"] = pol.T.copy()  # [a][b] = alpha_ab (column-major on the C side)
+        return out
 
     def set_alpha(self, alpha_w):
         """alpha_w: complex (n_types, n_freq)."""

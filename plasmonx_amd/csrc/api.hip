@@ -136,14 +136,18 @@ int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *
                   const double *tau, const double *density, const int32_t *nb_counts, const int32_t *nb_total) {
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
-        PXH_CHECK(model == PXH_WFQ || model == PXH_WFQFMU, PXH_EINVAL, "model must be PXH_WFQ or PXH_WFQFMU");
+        PXH_CHECK(model == PXH_WFQ || model == PXH_WFQFMU || model == PXH_FQ || model == PXH_FQFMU, PXH_EINVAL,
+                  "model must be PXH_WFQ, PXH_WFQFMU, PXH_FQ or PXH_FQFMU");
+        const bool stat = model == PXH_FQ || model == PXH_FQFMU;
+        const bool dipoles = model == PXH_WFQFMU || model == PXH_FQFMU;
         PXH_CHECK(kernel >= PXH_GAUSSIAN && kernel <= PXH_OHNO, PXH_EINVAL, "unknown kernel");
         PXH_CHECK(n_atoms > 0 && n_types > 0 && n_types <= pxh::kMaxTypes, PXH_EINVAL,
                   "n_atoms must be > 0 and 1 <= n_types <= 8");
-        PXH_CHECK(coord_bohr && type_of_atom && eta && a_ij && fermi_d && fermi_s && r0 && tau && density, PXH_EINVAL,
-                  "null input array");
+        PXH_CHECK(coord_bohr && type_of_atom && eta, PXH_EINVAL, "null input array");
+        PXH_CHECK(stat || (a_ij && fermi_d && fermi_s && r0 && tau && density), PXH_EINVAL,
+                  "the dynamic models need a_ij, the Fermi tables, tau and density");
         PXH_CHECK(kernel != PXH_GAUSSIAN || r_q, PXH_EINVAL, "the Gaussian kernel needs r_q");
-        PXH_CHECK(model != PXH_WFQFMU || (r_q && r_mu), PXH_EINVAL, "omegaFQFmu needs r_q and r_mu");
+        PXH_CHECK(!dipoles || (r_q && r_mu), PXH_EINVAL, "the dipole models need r_q and r_mu");
         // every check runs on local copies: a rejected call leaves the
         // context's current target as it was
         std::vector<int> type(n_atoms);
@@ -158,13 +162,14 @@ int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *
         const bool hetero = used > 1;
         // plain omegaFQ has no alloy form (wfq_module.f90:203-204)
         PXH_CHECK(!(hetero && model == PXH_WFQ), PXH_EINVAL, "heterogeneous omegaFQ is not supported");
+        (void)dipoles;
         auto cp = [&](const double *p, size_t k) { return p ? std::vector<double>(p, p + k) : std::vector<double>(k, 0.0); };
         std::vector<double> v_eta = cp(eta, n_types), v_rq = cp(r_q, n_types), v_rmu = cp(r_mu, n_types),
                             v_aij = cp(a_ij, n_types), v_tau = cp(tau, n_types), v_dens = cp(density, n_types);
         for (int t = 0; t < n_types; ++t) {
-            PXH_CHECK(v_tau[t] > 0.0 && v_dens[t] > 0.0, PXH_EINVAL, "tau and density must be positive");
+            if (!stat) PXH_CHECK(v_tau[t] > 0.0 && v_dens[t] > 0.0, PXH_EINVAL, "tau and density must be positive");
             if (kernel == PXH_GAUSSIAN) PXH_CHECK(v_rq[t] > 0.0, PXH_EINVAL, "r_q must be positive");
-            if (model == PXH_WFQFMU) PXH_CHECK(v_rq[t] > 0.0 && v_rmu[t] > 0.0, PXH_EINVAL, "r_q, r_mu must be positive");
+            if (dipoles) PXH_CHECK(v_rq[t] > 0.0 && v_rmu[t] > 0.0, PXH_EINVAL, "r_q, r_mu must be positive");
         }
         std::vector<double> nbfrac((size_t)n_types * n_atoms, 0.0);
         for (int i = 0; i < n_atoms; ++i) {
@@ -181,7 +186,11 @@ int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *
         c->target = 0;
         pxh::Atoms &A = c->atoms;
         A.n = 0;
-        A.model = model;
+        A.model = dipoles ? PXH_WFQFMU : PXH_WFQ;
+        A.is_static = stat;
+        A.chi.clear();
+        A.alpha_s.clear();
+        A.mol_count.clear();
         A.kernel = kernel;
         A.nt = n_types;
         A.hetero = hetero;
@@ -206,6 +215,52 @@ int pxh_set_atoms(pxh_ctx *c, int model, int kernel, int n_atoms, const double *
         }
         c->target = model;
     });
+}
+
+int pxh_set_static(pxh_ctx *c, int n_types, const double *chi, const double *alpha, int n_mol,
+                   const int32_t *atoms_per_molecule) {
+    if (!c) return PXH_EINVAL;
+    return guarded(c, [&] {
+        PXH_CHECK(c->target == PXH_FQ || c->target == PXH_FQFMU, PXH_EINVAL, "no static FQ / FQFMu target set");
+        pxh::Atoms &A = c->atoms;
+        PXH_CHECK(n_types == A.nt && chi, PXH_EINVAL, "chi must be given for the n_types of the current atoms");
+        PXH_CHECK(c->target == PXH_FQ || alpha, PXH_EINVAL, "FQFMu needs alpha");
+        PXH_CHECK(n_mol >= 0 && (n_mol == 0 || atoms_per_molecule), PXH_EINVAL, "bad molecule table");
+        std::vector<int> mc(atoms_per_molecule, atoms_per_molecule + n_mol);
+        long total = 0;
+        for (int k : mc) {
+            PXH_CHECK(k > 0, PXH_EINVAL, "every molecule needs at least one atom");
+            total += k;
+        }
+        PXH_CHECK(n_mol == 0 || total == A.n, PXH_EINVAL, "atoms_per_molecule must add up to n_atoms");
+        std::vector<double> al(n_types, 0.0);
+        if (alpha)
+            for (int t = 0; t < n_types; ++t) {
+                al[t] = alpha[t];
+                PXH_CHECK(c->target != PXH_FQFMU || al[t] > 0.0, PXH_EINVAL, "alpha must be positive");
+            }
+        A.chi.assign(chi, chi + n_types);
+        A.alpha_s = al;
+        A.mol_count = mc;
+    });
+}
+
+int pxh_solve_static(pxh_ctx *c, int what, double e0, int rhs_form, double *variables, double *polar,
+                     double *energy) {
+    if (!c) return PXH_EINVAL;
+    int status = PXH_OK;
+    const int g = guarded(c, [&] {
+        PXH_CHECK(c->target == PXH_FQ || c->target == PXH_FQFMU, PXH_EINVAL, "no static FQ / FQFMu target set");
+        PXH_CHECK((int)c->atoms.chi.size() == c->atoms.nt, PXH_EINVAL, "pxh_set_static has not been called");
+        PXH_CHECK(what == PXH_ENERGY || what == PXH_STATIC_FIELD, PXH_EINVAL, "what must be ENERGY or STATIC_FIELD");
+        PXH_CHECK(rhs_form == PXH_RHS_FIELD || rhs_form == PXH_RHS_POTENTIAL, PXH_EINVAL, "unknown rhs_form");
+        // the reference stops here: fqfmu_module.f90:340-347
+        PXH_CHECK(!(polar && what == PXH_STATIC_FIELD && c->target == PXH_FQFMU), PXH_EINVAL,
+                  "the static polarizability of FQFMu is not implemented (as in the reference)");
+        status = pxh::solve_static(c, what, e0, rhs_form, variables, polar, energy);
+        if (status == PXH_ESINGULAR) c->last_error = "singular pivot in the static solve";
+    });
+    return g != PXH_OK ? g : status;
 }
 
 int pxh_set_alpha(pxh_ctx *c, int n_types, int n_freq, const double *alpha_w) {
@@ -258,7 +313,8 @@ int pxh_solve_spectrum(pxh_ctx *c, int n_freq, const double *freq_au, double e0,
     if (!c) return PXH_EINVAL;
     int status = PXH_OK;
     const int g = guarded(c, [&] {
-        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU || c->target == PXH_BEM, PXH_EINVAL,
+                  "no dynamic target set (static FQ / FQFMu targets take pxh_solve_static)");
         PXH_CHECK(n_freq > 0 && freq_au, PXH_EINVAL, "no frequencies");
         PXH_CHECK(pol_mask > 0 && pol_mask < 8, PXH_EINVAL, "pol_mask must select x, y and/or z");
         PXH_CHECK(method >= PXH_INVERSION && method <= PXH_ITERATIVE_OTF, PXH_EINVAL, "unknown method");
@@ -348,7 +404,8 @@ int pxh_matvec(pxh_ctx *c, double freq_au, int i_freq, int n_vec, const double *
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(n_vec > 0 && x && y, PXH_EINVAL, "bad vectors");
-        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU || c->target == PXH_BEM, PXH_EINVAL,
+                  "no dynamic target set");
         const bool bem = c->target == PXH_BEM;
         const int nv = bem ? c->mesh.n : c->atoms.n_var();
         PXH_CHECK(nv > 0, PXH_EINVAL, "no target set");
@@ -383,7 +440,8 @@ int pxh_constant_matrix(pxh_ctx *c, double *m) {
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(m, PXH_EINVAL, "null output");
-        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU || c->target == PXH_BEM, PXH_EINVAL,
+                  "no dynamic target set");
         if (c->target == PXH_BEM) {
             const size_t nn = (size_t)c->mesh.n * c->mesh.n;
             PXH_HIP(hipMemcpyAsync(m, c->mesh.D.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
@@ -404,7 +462,8 @@ int pxh_dynamic_rhs(pxh_ctx *c, double freq_au, double e0, int rhs_form, double 
     if (!c) return PXH_EINVAL;
     return guarded(c, [&] {
         PXH_CHECK(rhs, PXH_EINVAL, "null output");
-        PXH_CHECK(c->target != 0, PXH_EINVAL, "no target set");
+        PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU || c->target == PXH_BEM, PXH_EINVAL,
+                  "no dynamic target set");
         PXH_CHECK(rhs_form == PXH_RHS_FIELD || rhs_form == PXH_RHS_POTENTIAL, PXH_EINVAL, "unknown rhs_form");
         const bool bem = c->target == PXH_BEM;
         const int nv = bem ? c->mesh.n : c->atoms.n_var();

@@ -1120,4 +1120,142 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     return status;
 }
 
+namespace {
+
+// Static FQ / FQFmu matrix (fq_module.f90:179-207, fqfmu_module.f90:81-140),
+// unknowns [q_1..q_N, lambda_1..lambda_M, mu_1x, mu_1y, mu_1z, ...]: T (the
+// dense atomistic T, unknowns [q, mu], n_t = N or 4N) in the q and mu rows
+// and columns, with 1/alpha added on the dipole diagonal (matrix_module.f90:
+// 280-288); the Lagrangian columns and rows of ones of each molecule
+// (add_lagrangian_blocks, matrix_module.f90:351-382); zero elsewhere.  One
+// work-item per element, columns over blockIdx.y.
+struct InvAlpha {
+    double v[kMaxTypes];
+};
+
+__global__ void k_assemble_static(int ns, int n, int m, int n_t, const double *__restrict__ T,
+                                  const int *__restrict__ mol, const double4 *__restrict__ pos, InvAlpha ia,
+                                  double2 *__restrict__ A) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (r >= ns) return;
+    // index into T (q or mu unknown), or -1 - k for the multiplier of molecule k
+    auto map = [&](int u) { return u < n ? u : (u < n + m ? -1 - (u - n) : u - m); };
+    const int tr = map(r), tc = map(col);
+    double v = 0.0;
+    if (tr >= 0 && tc >= 0) {
+        v = T[tr + (size_t)n_t * tc];
+        if (tr == tc && tr >= n) v += ia.v[(int)pos[(tr - n) / 3].w];
+    } else if (tr >= 0 && tr < n && tc < 0) {
+        v = mol[tr] == -1 - tc ? 1.0 : 0.0;
+    } else if (tc >= 0 && tc < n && tr < 0) {
+        v = mol[tc] == -1 - tr ? 1.0 : 0.0;
+    }
+    A[r + (size_t)ns * col] = make_double2(v, 0.0);
+}
+
+}  // namespace
+
+// Static FQ / FQFmu solve (algorithm_module.f90:469-504, DSYSV there; here the
+// batched complex LU of the inversion path on the real symmetric system).
+// what 0: ground state, one right-hand side -chi (fq_module.f90:211-240),
+// energy -1/2 x^T b (:364-381); what 1: static field, three right-hand sides
+// (rhs_module.f90:43-127), and for FQ the polarizability from three more
+// solves at unit field: alpha_ab = -sum_i x_a(i) q_i^(b), the closed form of
+// the constrained solve that fq_module.f90:385-597 evaluates through J^-1.
+int solve_static(pxh_ctx *c, int what, double e0, int rhs_form, double *variables, double *polar, double *energy) {
+    hipStream_t st = c->stream;
+    Atoms &A = c->atoms;
+    const int n = A.n, m = (int)A.mol_count.size(), n_t = A.n_var();
+    const bool fmu = A.model == PXH_WFQFMU;
+    const int ns = n + m + (fmu ? 3 * n : 0);
+    const int ncol = what == 0 ? 1 : 3;
+    const bool want_polar = what == 1 && !fmu;
+    const int nz = want_polar ? 6 : ncol;  // right-hand sides solved
+    // host right-hand sides, column-major ns x nz
+    std::vector<double> b((size_t)ns * nz, 0.0);
+    if (what == 0) {
+        for (int i = 0; i < n; ++i) b[i] = -A.chi[A.h_type[i]];
+    } else {
+        double mn[3] = {0, 0, 0};
+        if (rhs_form == PXH_RHS_POTENTIAL)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = A.h_coord[k];
+                for (int i = 0; i < n; ++i) mn[k] = std::min(mn[k], A.h_coord[3 * i + k]);
+            }
+        for (int k = 0; k < 3; ++k)
+            for (int i = 0; i < n; ++i) {
+                const double x = A.h_coord[3 * i + k];
+                // rhs_module.f90:43-90
+                b[i + (size_t)ns * k] = rhs_form == PXH_RHS_POTENTIAL ? -e0 * std::fabs(x - mn[k]) : -e0 * x;
+                if (want_polar) b[i + (size_t)ns * (3 + k)] = -x;
+            }
+        // dipole rows: 1 on component k when e0 is not zero, else 0
+        // (rhs_module.f90:96-127, as written there)
+        if (fmu)
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) b[(n + m + 3 * i + k) + (size_t)ns * k] = e0 == 0.0 ? 0.0 : 1.0;
+    }
+    DevBuf<double> T;
+    T.alloc((size_t)n_t * n_t);
+    dense_t_atoms(c, T.p, false);
+    std::vector<int> mol(n, 0);
+    for (int k = 0, first = 0; k < m; first += A.mol_count[k], ++k)
+        for (int i = first; i < first + A.mol_count[k]; ++i) mol[i] = k;
+    DevBuf<int> dmol;
+    dmol.upload(mol.data(), mol.size(), st);
+    InvAlpha ia{};
+    for (int t = 0; t < A.nt; ++t) ia.v[t] = fmu ? 1.0 / A.alpha_s[t] : 0.0;
+    DevBuf<double2> M, Z, X;
+    M.alloc((size_t)ns * ns);
+    k_assemble_static<<<dim3((ns + 255) / 256, ns), 256, 0, st>>>(ns, n, m, n_t, T.p, dmol.p, A.pos.p, ia, M.p);
+    PXH_HIP(hipGetLastError());
+    DevBuf<int> piv, info;
+    DevBuf<SwapSet> swaps;
+    piv.alloc((size_t)2 * ns);
+    swaps.alloc(1);
+    info.alloc(1);
+    PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int), st));
+    lu_factor(c, ns, 1, M.p, piv.p, swaps.p, info.p);
+    int *perm = piv.p + ns;
+    k_perm_from_piv<<<1, 256, 0, st>>>(ns, piv.p, perm);
+    // right-hand sides in groups of three (the solve kernel's width)
+    const int ngroups = (nz + 2) / 3;
+    std::vector<double2> hz((size_t)ns * 3 * ngroups, make_double2(0.0, 0.0));
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < ns; ++i) hz[i + (size_t)ns * k].x = b[i + (size_t)ns * k];
+    Z.upload(hz.data(), hz.size(), st);
+    X.alloc(hz.size());
+    for (int g = 0; g < ngroups; ++g) {
+        const int mask = (1 << std::min(3, nz - 3 * g)) - 1;
+        double2 *zg = Z.p + (size_t)3 * ns * g, *xg = X.p + (size_t)3 * ns * g;
+        k_lu_solve<<<dim3(3, 1, 1), kSolveThreads, 0, st>>>(ns, M.p, (size_t)ns * ns, zg, (size_t)3 * ns, ns, mask);
+        k_scatter_rows<<<dim3((ns + 255) / 256, 3, 1), 256, 0, st>>>(ns, perm, zg, (size_t)3 * ns, xg,
+                                                                     (size_t)3 * ns);
+    }
+    PXH_HIP(hipGetLastError());
+    std::vector<double2> hx(hz.size());
+    int hinfo = 0;
+    PXH_HIP(hipMemcpyAsync(hx.data(), X.p, sizeof(double2) * hx.size(), hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipStreamSynchronize(st));
+    if (hinfo != 0) return PXH_ESINGULAR;
+    if (variables)
+        for (int k = 0; k < ncol; ++k)
+            for (int i = 0; i < ns; ++i) variables[i + (size_t)ns * k] = hx[i + (size_t)ns * k].x;
+    if (energy) {
+        double e = 0.0;
+        for (int i = 0; i < ns; ++i) e += hx[i].x * b[i];
+        *energy = -0.5 * e;
+    }
+    if (polar && want_polar)
+        for (int a = 0; a < 3; ++a)
+            for (int bb = 0; bb < 3; ++bb) {
+                double sum = 0.0;
+                for (int i = 0; i < n; ++i) sum += A.h_coord[3 * i + a] * hx[i + (size_t)ns * (3 + bb)].x;
+                polar[a + 3 * bb] = -sum;
+            }
+    return PXH_OK;
+}
+
 }  // namespace pxh

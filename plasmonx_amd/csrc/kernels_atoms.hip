@@ -903,7 +903,7 @@ void build_atoms_device(pxh_ctx *c) {
         for (int i = 0; i < A.n; ++i) A.h_nbfrac[A.h_type[i] + (size_t)A.nt * i] = 1.0;
     }
     A.d_nbfrac.upload(A.h_nbfrac.data(), A.h_nbfrac.size(), c->stream);
-    build_kbar(c);
+    if (!A.is_static) build_kbar(c);  // the static models have no K-bar
 }
 
 namespace {

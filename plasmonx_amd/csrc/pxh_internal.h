@@ -75,8 +75,11 @@ struct DevBuf {
 
 // Atomistic system resident in HBM.
 struct Atoms {
-    int model = 0, kernel = 0, n = 0, nt = 0;
+    int model = 0, kernel = 0, n = 0, nt = 0;  // model PXH_WFQ (charges) or PXH_WFQFMU (charges and dipoles)
     bool hetero = false;
+    bool is_static = false;             // static FQ / FQFmu (no K-bar, no Drude parameters)
+    std::vector<double> chi, alpha_s;   // static models: per type (pxh_set_static)
+    std::vector<int> mol_count;         // static models: atoms per molecule, contiguous runs
     std::vector<double> h_coord;        // 3n, bohr (host copy)
     std::vector<int> h_type;            // n, 0-based
     std::vector<double> eta, rq, rmu, aij, tau, dens;
@@ -247,6 +250,8 @@ void build_mesh_device(pxh_ctx *c);
 void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lambda, const double2 *x, double2 *y,
                int ld, bool fp32);
 void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
+// dense.hip: static FQ / FQFmu (what 0 ground state, 1 static field)
+int solve_static(pxh_ctx *c, int what, double e0, int rhs_form, double *variables, double *polar, double *energy);
 // solver.cpp
 int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rhs_form, int pol_mask, int method,
                    double tol, int kdim, int precision, double *variables, double *polar, double *results,

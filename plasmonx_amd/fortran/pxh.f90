@@ -10,7 +10,8 @@ module pxh
 
    integer(c_int), parameter :: PXH_OK = 0, PXH_EINVAL = 1, PXH_ENOCONV = 2, &
                                 PXH_ESINGULAR = 3, PXH_ENOMEM = 4, PXH_EHIP = 5, PXH_ENODEV = 6
-   integer(c_int), parameter :: PXH_WFQ = 1, PXH_WFQFMU = 2, PXH_BEM = 3
+   integer(c_int), parameter :: PXH_WFQ = 1, PXH_WFQFMU = 2, PXH_BEM = 3, PXH_FQ = 4, PXH_FQFMU = 5
+   integer(c_int), parameter :: PXH_ENERGY = 0, PXH_STATIC_FIELD = 1
    integer(c_int), parameter :: PXH_GAUSSIAN = 1, PXH_COULOMB = 2, PXH_OHNO = 3
    integer(c_int), parameter :: PXH_INVERSION = 0, PXH_ITERATIVE = 1, PXH_ITERATIVE_OTF = 2
    integer(c_int), parameter :: PXH_FP64 = 0, PXH_MIXED = 1
@@ -43,6 +44,24 @@ module pxh
          real(c_double), intent(in) :: fermi_d(n_types, *), fermi_s(n_types, *), r0(n_types, *)
          real(c_double), intent(in) :: tau(*), density(*)
          integer(c_int32_t), intent(in) :: nb_counts(n_types, *), nb_total(*)
+      end function
+
+      integer(c_int) function pxh_set_static(ctx, n_types, chi, alpha, n_mol, atoms_per_molecule) &
+            bind(C, name="pxh_set_static")
+         import :: c_int, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n_types, n_mol
+         real(c_double), intent(in) :: chi(*), alpha(*)
+         integer(c_int32_t), intent(in) :: atoms_per_molecule(*)
+      end function
+
+      integer(c_int) function pxh_solve_static(ctx, what, e0, rhs_form, variables, polar, energy) &
+            bind(C, name="pxh_solve_static")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: what, rhs_form
+         real(c_double), value :: e0
+         real(c_double), intent(out) :: variables(*), polar(3, 3), energy
       end function
 
       integer(c_int) function pxh_set_alpha(ctx, n_types, n_freq, alpha_w) bind(C, name="pxh_set_alpha")

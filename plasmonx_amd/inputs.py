@@ -334,3 +334,96 @@ def system_from_case(case):
     return AtomSystem(model=case["model"], kernel=case["kernel"], symbols=list(case["symbols"]),
                       coord_bohr=np.array(case["coord_ang"], dtype=np.float64) * TOBOHR, types=types,
                       fermi_d=np.array(case["fermi_d"]), fermi_s=np.array(case["fermi_s"]), r0=np.array(case["r0"]))
+
+
+_IMOL = re.compile(r"^(.*?)\s*\[\s*IMol\s*=\s*(\d+)\s*\]\s*(.*)$", re.IGNORECASE)
+
+
+def parse_geometry_molecules(text):
+    """Symbols, Angstrom coordinates and molecule index of ``Sym[IMol= k] x y
+    z`` lines (python/input_/io_operations.py:123-190; a line without the
+    tag is molecule 1)."""
+    sym, xyz, mol = [], [], []
+    for line in text.strip().splitlines():
+        m = _IMOL.match(line.strip())
+        if m:
+            name, k, rest = m.group(1).strip(), int(m.group(2)), m.group(3).split()
+        else:
+            p = line.split()
+            if len(p) < 4:
+                continue
+            name, k, rest = p[0], 1, p[1:]
+        sym.append(name)
+        mol.append(k)
+        xyz.append([float(v) for v in rest[:3]])
+    return sym, xyz, mol
+
+
+def molecule_runs(mol):
+    """Atoms per molecule as runs of consecutive equal molecule indices
+    (create_n_atoms_per_molecule, input_module.f90:1007-1031)."""
+    counts = []
+    for k, m in enumerate(mol):
+        if k == 0 or m != mol[k - 1]:
+            counts.append(0)
+        counts[-1] += 1
+    return counts
+
+
+def read_static_input(path):
+    """The case of a reference YAML input for the static models ('what:
+    energy' or 'static response', 'forcefield static: fq | fqfmu'): kernel
+    defaults (validation.py:881-903), per-type chi / eta / alpha / R_q / R_mu
+    with the reference's units and the Gaussian widths from eta and alpha
+    (:1065-1090), the static field (intensity 0, :752-776), molecules from the
+    IMol tags."""
+    d = os.path.dirname(os.path.abspath(path))
+    with open(path) as f:
+        y = yaml.safe_load(f)
+    what = y.get("what")
+    what = (what[0] if isinstance(what, list) else what).lower()
+    if what not in ("energy", "static response"):
+        raise ValueError(f"{path}: not a static (energy / static response) input")
+    ff = y.get("forcefield", {})
+    model = ff["static"].lower()
+    kernel = ff.get("kernel", {"fq": "ohno", "fqfmu": "gaussian"}.get(model, "ohno")).lower()
+    geo = y.get("input_geometry")
+    if isinstance(geo, dict) and "external xyz file" in geo:
+        with open(os.path.join(d, geo["external xyz file"])) as f:
+            geo = "\n".join(f.read().splitlines()[2:])
+    sym, xyz, mol = parse_geometry_molecules(geo)
+    names = []
+    for s in sym:
+        if s not in names:
+            names.append(s)
+    atom_types = {_type_key(k): v for k, v in (y.get("atom_types") or {}).items()}
+    types = []
+    for nm in names:
+        at = atom_types[_type_key(nm)]
+        t = {"name": nm, "chi": param_to_au(at.get("chi", 0.0), "chi"), "eta": param_to_au(at["eta"], "eta"),
+             "alpha": param_to_au(at.get("alpha", 0.0), "alpha"), "rq": param_to_au(at.get("rq", 0.0), "rq"),
+             "rmu": param_to_au(at.get("rmu", 0.0), "rmu")}
+        if kernel == "gaussian":
+            if t["rq"] == 0.0:
+                t["rq"] = math.sqrt(2.0 / math.pi) / t["eta"]
+            if model == "fqfmu" and t["rmu"] == 0.0:
+                t["rmu"] = (t["alpha"] / (3.0 * math.sqrt(math.pi / 2.0))) ** (1.0 / 3.0)
+        types.append(t)
+    field = y.get("field") or {}
+    control = [str(c).lower() for c in (y.get("control") or [])]
+    return {"model": model, "kernel": kernel, "what": what, "rhs_form": field.get("rhs type", "field"),
+            "e0": 0.0 if what == "static response" else float(field.get("field intensity", 0.0)),
+            "types": types, "symbols": sym, "coord_ang": xyz, "mol_count": molecule_runs(mol),
+            "principal_axes": "principal axes" in control}
+
+
+def static_system_from_case(case):
+    """AtomSystem of a static case from read_static_input (the Drude and
+    connectivity fields are unused by the static models)."""
+    types = [AtomType(name=t["name"], eta=t["eta"], a_ij=0.0, tau=0.0, density=0.0, fermi_d=0.0, fermi_s=0.0,
+                      r_q=t["rq"], r_mu=t["rmu"], alpha=t["alpha"], chi=t["chi"]) for t in case["types"]]
+    nt = len(types)
+    z = np.zeros((nt, nt))
+    return AtomSystem(model=case["model"], kernel=case["kernel"], symbols=list(case["symbols"]),
+                      coord_bohr=np.array(case["coord_ang"], dtype=np.float64) * TOBOHR, types=types,
+                      fermi_d=z, fermi_s=z, r0=z)

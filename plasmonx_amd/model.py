@@ -39,11 +39,12 @@ class AtomType:
     alpha: float = 0.0
     alpha_w: object = None   # complex array (n_freq,) of the interband polarizability, or None
     permittivity: str = ""   # 'silver etchegoin' / 'gold etchegoin' when alpha_w comes from a fit
+    chi: float = 0.0         # electronegativity (static FQ / FQFMu)
 
 
 @dataclass
 class AtomSystem:
-    model: str                      # 'wfq' | 'wfqfmu'
+    model: str                      # 'wfq' | 'wfqfmu' (dynamic) | 'fq' | 'fqfmu' (static)
     kernel: str                     # 'gaussian' | 'coulomb' | 'ohno'
     symbols: list
     coord_bohr: np.ndarray          # (n, 3)
@@ -72,11 +73,12 @@ class AtomSystem:
 
     @property
     def n_var(self):
-        return 4 * self.n_atoms if self.model == "wfqfmu" else self.n_atoms
+        """Unknowns of the interaction matrix T: [q] or [q, mu]."""
+        return 4 * self.n_atoms if self.model in ("wfqfmu", "fqfmu") else self.n_atoms
 
     @property
     def model_id(self):
-        return _lib.WFQFMU if self.model == "wfqfmu" else _lib.WFQ
+        return {"wfq": _lib.WFQ, "wfqfmu": _lib.WFQFMU, "fq": _lib.FQ, "fqfmu": _lib.FQFMU}[self.model]
 
     @property
     def kernel_id(self):

@@ -74,3 +74,36 @@ def solve_dynamic_field(target, field, algorithm, ctx=None, device=0, want_varia
     finally:
         if own:
             ctx.close()
+
+
+def solve_ground_state(target, mol_count, ctx=None, device=0):
+    """algorithm%solve_ground_state (algorithm_module.f90:95-111) of a static
+    FQ / FQFMu target: the charges (and dipoles) that minimise the energy
+    with each molecule neutral; returns 'variables' (n_var, 1) in the order
+    [q, lambda, mu] and 'energy' = -1/2 x^T b (a.u.)."""
+    own = ctx is None
+    ctx = ctx or _lib.Context(device)
+    try:
+        ctx.set_atoms(target)
+        ctx.set_static(target, mol_count)
+        return ctx.solve_static(_lib.ENERGY)
+    finally:
+        if own:
+            ctx.close()
+
+
+def solve_static_field(target, mol_count, e0=0.0, rhs_form="field", ctx=None, device=0):
+    """algorithm%solve_static_field (algorithm_module.f90:114-131) of a static
+    FQ / FQFMu target: 'variables' (n_var, 3) for the field e0 along x, y, z
+    and, for FQ, 'polar' (3, 3), the static polarizability
+    (fq_module.f90:385-597; FQFMu has none, as in the reference)."""
+    own = ctx is None
+    ctx = ctx or _lib.Context(device)
+    try:
+        ctx.set_atoms(target)
+        ctx.set_static(target, mol_count)
+        rhs = _lib.RHS_POTENTIAL if rhs_form == "potential" else _lib.RHS_FIELD
+        return ctx.solve_static(_lib.STATIC_FIELD, e0=e0, rhs_form=rhs, want_polar=target.model == "fq")
+    finally:
+        if own:
+            ctx.close()

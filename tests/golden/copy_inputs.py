@@ -7,7 +7,10 @@ Run in a checkout that has the reference tree (default /root/reference):
 
 Each case keeps its YAML input as it is, the xyz file it names and, of each
 alpha(omega) table it names, the rows at the case's frequencies (the reader
-matches rows by frequency, so the rest is not read).  For the minimal decks
+matches rows by frequency, so the rest is not read).  For the static decks
+(fq_*, fqfmu_*) it writes log.json with what the log prints: the model
+matrix (E11.4), right-hand side, charges, dipoles, energy, J^-1 and the
+polarizability tensor.  For the minimal decks
 (the reference's defaults fill in the atom types and parameters) it also
 writes log.json: what the case's reference/*.log prints about the input
 (spectrum and per-type parameters, as printed) and the 20 result columns at
@@ -28,6 +31,7 @@ from plasmonx_amd.inputs import AU_TO_EV, frequencies  # noqa: E402
 CASES = ["wfq_sodium_inversion", "wfqfmu_ag_inversion", "wfqfmu_hetero_agau_inversion", "wfqfmu_ag_diff_units"]
 MINIMAL = ["wfq_sodium_minimal_input", "wfq_graphene_minimal_input", "wfqfmu_ag_minimal_input",
            "wfqfmu_au_minimal_input", "wfqfmu_hetero_agau_minimal_input", "wfqfmu_au_read_xyz"]
+STATIC = ["fq_energy", "fq_polarizability", "fq_principal_axis_energy", "fqfmu_energy", "fqfmu_rq_rmu_energy"]
 HEADER_KEYS = ["Num. Freq.", "Min. Freq.", "Max. Freq.", "Step Freq.", "AtomType", "Eta", "Alpha", "R_q", "R_mu",
                "tau", "sigma_0", "scaling sigma0-tau", "A_ij", "fermi_d", "fermi_s", "density"]
 
@@ -53,10 +57,82 @@ def log_summary(path, keep=12):
     return {"header": header, "results_ev": pick, "results": [rows[e] for e in pick]}
 
 
+def printed_matrix(lines, title):
+    """A matrix printed by out_%print_matrix (blocks of 5 columns, E11.4):
+    {(row, col): value} with 1-based indices, or {} when absent."""
+    start = next((k for k, ln in enumerate(lines) if ln.strip() == title), None)
+    out = {}
+    if start is None:
+        return out
+    k = start + 2
+    cols = []
+    while k < len(lines):
+        ln = lines[k].strip()
+        if ln.startswith("---"):
+            break
+        parts = ln.split()
+        if parts and all(p.isdigit() for p in parts):
+            cols = [int(p) for p in parts]
+        elif parts and parts[0].isdigit() and cols:
+            row = int(parts[0])
+            for c, v in zip(cols, parts[1:]):
+                out[(row, c)] = float(v.replace("D", "E"))
+        k += 1
+    return out
+
+
+def printed_column_block(lines, title, after=0):
+    """Rows 'i v [v v]' following a title line (charges, dipoles, RHS)."""
+    start = next((k for k in range(after, len(lines)) if lines[k].strip() == title), None)
+    rows = []
+    if start is None:
+        return rows
+    for ln in lines[start + 1:]:
+        parts = ln.split()
+        if not parts:
+            if rows:
+                break
+            continue
+        if parts[0].isdigit():
+            rows.append([float(v.replace("D", "E")) for v in parts[1:]])
+        elif rows or parts[0].startswith("---") or parts[0] == "ChErr":
+            if rows:
+                break
+    return rows
+
+
+def static_log_summary(path):
+    """What a static (energy / static response) log prints: the model matrix,
+    the right-hand side, charges, dipoles, energy, J^-1 and the polarizability."""
+    lines = open(path).read().splitlines()
+    out = {}
+    for title in ("FQ Matrix", "FQFMu Matrix", "JMatrix-1", "polarizability tensor (a.u.)"):
+        m = printed_matrix(lines, title)
+        if m:
+            out[title] = [[r, c, v] for (r, c), v in sorted(m.items())]
+    for title in ("RHS", "Charges", "Dipoles"):
+        rows = printed_column_block(lines, title)
+        if rows:
+            out[title] = rows
+    for ln in lines:
+        t = ln.strip()
+        if t.startswith("Energy ="):
+            out["energy"] = float(t.split("=")[1].split()[0])
+        if t.startswith("Polar Iso  ="):
+            out["polar_iso"] = float(t.split("=")[1].split()[0])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     a = ap.parse_args()
+    for case in STATIC:
+        dst = os.path.join(HERE, "inputs", case)
+        os.makedirs(dst, exist_ok=True)
+        shutil.copy(os.path.join(a.ref, "tests", case, case + ".yaml"), os.path.join(dst, case + ".yaml"))
+        with open(os.path.join(dst, "log.json"), "w") as f:
+            json.dump(static_log_summary(os.path.join(a.ref, "tests", case, "reference", case + ".log")), f)
     for case in CASES + MINIMAL:
         src = os.path.join(a.ref, "tests", case, case + ".yaml")
         dst = os.path.join(HERE, "inputs", case)
