@@ -31,6 +31,9 @@ from plasmonx_amd.inputs import AU_TO_EV, frequencies  # noqa: E402
 CASES = ["wfq_sodium_inversion", "wfqfmu_ag_inversion", "wfqfmu_hetero_agau_inversion", "wfqfmu_ag_diff_units"]
 MINIMAL = ["wfq_sodium_minimal_input", "wfq_graphene_minimal_input", "wfqfmu_ag_minimal_input",
            "wfqfmu_au_minimal_input", "wfqfmu_hetero_agau_minimal_input", "wfqfmu_au_read_xyz"]
+# cases whose log tail (backup clean-up, result blocks, maxima analysis) is
+# kept as text: the first three result blocks and the maxima block, verbatim
+TAIL = ["wfq_sodium_inversion", "wfq_sodium_restart"]
 STATIC = ["fq_energy", "fq_polarizability", "fq_principal_axis_energy", "fqfmu_energy", "fqfmu_rq_rmu_energy"]
 HEADER_KEYS = ["Num. Freq.", "Min. Freq.", "Max. Freq.", "Step Freq.", "AtomType", "Eta", "Alpha", "R_q", "R_mu",
                "tau", "sigma_0", "scaling sigma0-tau", "A_ij", "fermi_d", "fermi_s", "density"]
@@ -53,7 +56,10 @@ def log_summary(path, keep=12):
             ev = float(ln.split()[-2].replace("D", "E"))
             rows[ev] = [float(lines[j].split("=")[1].split()[0].replace("D", "E")) for j in range(k + 2, k + 22)]
     evs = sorted(rows)
-    pick = sorted(set(evs[int(round(i * (len(evs) - 1) / max(1, keep - 1)))] for i in range(min(keep, len(evs)))))
+    if len(evs) <= keep:
+        pick = evs
+    else:
+        pick = sorted(set(evs[int(round(i * (len(evs) - 1) / max(1, keep - 1)))] for i in range(keep)))
     return {"header": header, "results_ev": pick, "results": [rows[e] for e in pick]}
 
 
@@ -123,6 +129,19 @@ def static_log_summary(path):
     return out
 
 
+def log_tail_excerpt(path, blocks=3):
+    """Verbatim lines of a dynamic log: any 'Recovering from backup' lines,
+    the clean-up message with its sticks, the first result blocks and the
+    maxima analysis block."""
+    lines = open(path).read().splitlines()
+    rec = [ln for ln in lines if ln.startswith(" Recovering from backup frequency")]
+    k = next(i for i, ln in enumerate(lines) if "I am cleaning up the backup" in ln)
+    out = rec + lines[k - 2:k + 3] + lines[k + 3:k + 3 + 23 * blocks]
+    m = next(i for i, ln in enumerate(lines) if ln.startswith(" Maxima Analysis"))
+    e = next(i for i in range(m + 1, len(lines)) if lines[i].startswith(" ----"))
+    return "\n".join(out + lines[m:e + 1]) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -133,6 +152,20 @@ def main():
         shutil.copy(os.path.join(a.ref, "tests", case, case + ".yaml"), os.path.join(dst, case + ".yaml"))
         with open(os.path.join(dst, "log.json"), "w") as f:
             json.dump(static_log_summary(os.path.join(a.ref, "tests", case, "reference", case + ".log")), f)
+    for case in TAIL:
+        dst = os.path.join(HERE, "inputs", case)
+        os.makedirs(dst, exist_ok=True)
+        shutil.copy(os.path.join(a.ref, "tests", case, case + ".yaml"), os.path.join(dst, case + ".yaml"))
+        with open(os.path.join(dst, "log_tail.txt"), "w") as f:
+            f.write(log_tail_excerpt(os.path.join(a.ref, "tests", case, "reference", case + ".log")))
+        bk = os.path.join(a.ref, "tests", case, "bk")
+        if os.path.isdir(bk):
+            for name in sorted(os.listdir(bk)):
+                shutil.copy(os.path.join(bk, name), os.path.join(dst, name))
+        if case not in CASES:
+            with open(os.path.join(dst, "log.json"), "w") as f:
+                json.dump(log_summary(os.path.join(a.ref, "tests", case, "reference", case + ".log"), keep=100), f,
+                          indent=0)
     for case in CASES + MINIMAL:
         src = os.path.join(a.ref, "tests", case, case + ".yaml")
         dst = os.path.join(HERE, "inputs", case)
