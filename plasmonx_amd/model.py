@@ -38,7 +38,7 @@ class AtomType:
     r_mu: float = 0.0
     alpha: float = 0.0
     alpha_w: object = None   # complex array (n_freq,) of the interband polarizability, or None
-    permittivity: str = ""   # 'silver etchegoin' / 'gold etchegoin' when alpha_w comes from a fit
+    permittivity: str = ""   # 'silver etchegoin' / 'gold etchegoin' / 'silver lorentz-drude': alpha_w from a model
     chi: float = 0.0         # electronegativity (static FQ / FQFMu)
 
 
@@ -109,6 +109,8 @@ class AtomSystem:
                 out[k] = permittivity.silver_etchegoin(freq_au, polar=True)
             elif t.permittivity == "gold etchegoin":
                 out[k] = permittivity.gold_etchegoin(freq_au, polar=True)
+            elif t.permittivity == "silver lorentz-drude":
+                out[k] = permittivity.lorentz_drude_interband(freq_au)
         return out
 
 

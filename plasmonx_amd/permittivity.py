@@ -1,8 +1,9 @@
 """Frequency-dependent material inputs: Etchegoin permittivity / interband
 polarizability of Ag and Au (src/parameters/parameters_module.f90:107-194,
 295-375, with the published Etchegoin-Le Ru-Meyer fit parameters), the
-permittivity CSV reader (:384-462), a Drude-Lorentz interband model and a
-Brendel-Bormann permittivity (not implemented by the reference, :223-225).
+permittivity CSV reader (:384-462) and writer, the Lorentz-Drude silver
+interband polarizability of the omegaFQFmu spectrum and a Brendel-Bormann
+permittivity (not implemented by the reference, :223-225).
 Host-side preparation of the per-frequency tables passed to libpxh.
 """
 import math
@@ -75,18 +76,37 @@ def read_table(path, freq_au):
     return out
 
 
-def drude_lorentz_interband(freq_au, terms=((1.0, 4.0, 0.4),), eps_inf=1.0):
-    """Interband polarizability alpha(omega) = (eps_ib(omega) - 1) / w_p^2-style
-    table from a sum of Lorentz oscillators eps_ib = eps_inf + sum f w0^2 /
-    (w0^2 - w^2 - i g w) (f, w0 [eV], g [eV]), scaled by 1/(4 pi n0) with the
-    silver electron density, in the form the omegaFQFmu mu-diagonal uses."""
-    w = np.asarray(freq_au, dtype=np.float64)
-    eps = np.full(w.shape, eps_inf, dtype=np.complex128)
-    for f, w0, g in terms:
-        w0a, ga = w0 / AU_TO_EV, g / AU_TO_EV
-        eps = eps + f * w0a ** 2 / (w0a ** 2 - w ** 2 - 1j * ga * w)
-    n0 = 10.49 / 107.8682 * 6.022e23 / (1.0e8 * TOBOHR) ** 3
-    return (eps - 1.0) / (4.0 * math.pi * n0)
+# Lorentz-Drude silver (A.D. Rakic et al., Appl. Opt. 37 (1998) 5271,
+# Table 1): plasma energy and the interband oscillators (f_j, w_j, Gamma_j)
+# in eV; the intraband (Drude) term f0 = 0.845, Gamma0 = 0.048 eV is the
+# omegaFQ part of the model and not part of the interband table.
+LD_SILVER = dict(wp_ev=9.01, osc=[(0.065, 0.816, 3.886), (0.124, 4.481, 0.452), (0.011, 8.185, 0.065),
+                                  (0.840, 9.083, 0.916), (5.646, 20.29, 2.419)])
+# conduction-electron (atomic) density of silver: 10.49 g/cm3, 107.8682 g/mol, in bohr^-3
+AG_DENSITY = 10.49 / 107.8682 * 6.022e23 / (1.0e8 * TOBOHR) ** 3
+
+
+def lorentz_drude_interband(freq_au, wp_ev=LD_SILVER["wp_ev"], osc=LD_SILVER["osc"], density=AG_DENSITY):
+    """Interband polarizability per atom of a Lorentz-Drude metal:
+    alpha(w) = eps_ib(w) / (4 pi n0), eps_ib = sum_j f_j wp^2 / (w_j^2 - w^2
+    - i Gamma_j w), the form of the omegaFQFmu dipole term that the
+    Etchegoin 'polar' variant takes (parameters_module.f90:107-194)."""
+    w = np.asarray(freq_au, dtype=np.float64) * AU_TO_EV
+    eib = np.zeros(w.shape, dtype=np.complex128)
+    for f, w0, g in osc:
+        eib = eib + f * wp_ev ** 2 / (w0 ** 2 - w ** 2 - 1j * g * w)
+    return eib / (4.0 * math.pi * density)
+
+
+def write_table(path, freq_au, values):
+    """A 'freq (eV)  Re  Im' table in the layout the reference reads for
+    'wfqfmu file' / 'permittivity file' (parameters_module.f90:384-462): one
+    '#' comment line, then list-directed rows with the frequency written to
+    full precision so that the 1e-8 eV match finds every row."""
+    with open(path, "w") as fh:
+        fh.write("# freq (eV)   Re   Im\n")
+        for f, v in zip(np.asarray(freq_au) * AU_TO_EV, np.asarray(values)):
+            fh.write(f"{float(f)!r:>28}  {float(v.real)!r:>28}  {float(v.imag)!r:>28}\n")
 
 
 def faddeeva(z, N=32):

@@ -4,8 +4,8 @@ types, :1134-1145 parameters; unit rules :8-140).
 
 C1  omegaFQ Na fcc sphere a=4.23 A, r=2 nm (1745 atoms), 61 w in 2.5-4.0 eV, x field
 C2  omegaFQFmu Ag fcc sphere a=4.09 A, r=4 nm (15683 atoms, 62732 unknowns),
-    201 w in 2.8-4.2 eV, 3 fields, interband alpha(w) from the Etchegoin
-    (Drude + critical-point Lorentz) silver model
+    201 w in 2.8-4.2 eV, 3 fields, interband alpha(w) from the Lorentz-Drude
+    silver permittivity (Rakic et al. 1998), passed as a 'wfqfmu file' table
 C3  BEM DPCM / IEF on an icosphere R=10 nm (5120 triangles) and a capped rod
 C4  96 seeded graphene flakes, 128 w in 0.1-1.0 eV
 C5  a seeded Poisson stream of spectrum requests (omegaFQ Na rods, omegaFQFmu
@@ -60,12 +60,24 @@ def c1_sodium_sphere():
 
 
 def c2_silver_sphere(radius=40.0, n_freq=201):
+    """C2: the Ag defaults with the Lorentz-Drude interband polarizability
+    (alpha_table evaluates it; c2_alpha_table writes and reads it as the
+    reference's 'wfqfmu file' table)."""
     xyz = geometry.fcc_sphere(4.09, radius)
     t = silver_type()
+    t.permittivity = "silver lorentz-drude"
     sysm = AtomSystem(model="wfqfmu", kernel="gaussian", symbols=["Ag"] * len(xyz), coord_bohr=xyz * TOBOHR,
                       types=[t], fermi_d=np.array([[12.0]]), fermi_s=np.array([[1.1]]), r0=np.array([[R0_BOHR["Ag"]]]))
     freq = ev(np.linspace(2.8, 4.2, n_freq))
     return sysm, freq
+
+
+def c2_alpha_table(freq, path):
+    """The C2 interband alpha(w) written as a 'freq (eV) Re Im' table at path
+    and read back at the spectrum's frequencies, the route of the
+    reference's 'wfqfmu file' (parameters_module.f90:384-462): (1, n_freq)."""
+    permittivity.write_table(path, freq, permittivity.lorentz_drude_interband(freq))
+    return permittivity.read_table(path, freq)[None, :]
 
 
 def core_shell_sphere(r_core, r_shell, a=4.08):

@@ -41,3 +41,31 @@ def test_icosphere_is_closed_and_outward():
     # divergence theorem: sum of (area vector . centroid) / 3 = enclosed volume
     vol = (n * c).sum() / 6.0
     assert vol == pytest.approx(4 / 3 * np.pi * 10.0 ** 3, rel=0.02)
+
+
+def test_lorentz_drude_interband_limits():
+    """A Lorentz oscillator f wp^2 / (w0^2 - w^2 - i g w) is real and equal to
+    f wp^2 / w0^2 at w -> 0 and purely imaginary, f wp^2 / (g w0), at
+    resonance; the silver table is passive (Im alpha > 0) over the C2
+    spectrum and of the Etchegoin interband term's size there."""
+    from plasmonx_amd import permittivity as pm
+    n0 = pm.AG_DENSITY
+    one = [(0.3, 4.0, 0.5)]
+    wp = 9.0
+    a0 = pm.lorentz_drude_interband(np.array([1e-9 / 27.211386245981]), wp_ev=wp, osc=one)[0]
+    assert abs(a0.real - 0.3 * wp ** 2 / 16.0 / (4 * np.pi * n0)) < 1e-12 * abs(a0) and abs(a0.imag) < 1e-9 * abs(a0)
+    ar = pm.lorentz_drude_interband(np.array([4.0 / 27.211386245981]), wp_ev=wp, osc=one)[0]
+    assert abs(ar.real) < 1e-12 * abs(ar) and abs(ar.imag - 0.3 * wp ** 2 / (0.5 * 4.0) / (4 * np.pi * n0)) < 1e-12 * abs(ar)
+    w = np.linspace(2.8, 4.2, 201) / 27.211386245981
+    ld, et = pm.lorentz_drude_interband(w), pm.silver_etchegoin(w, polar=True)
+    assert np.all(ld.imag > 0)
+    assert np.all(np.abs(ld) > 0.5 * np.abs(et)) and np.all(np.abs(ld) < 2.0 * np.abs(et))
+
+
+def test_alpha_table_round_trip(tmp_path):
+    """write_table then read_table (the 'wfqfmu file' route of C2) returns
+    the model's values exactly at every frequency of the spectrum."""
+    from plasmonx_amd.workloads import c2_alpha_table, c2_silver_sphere
+    s, freq = c2_silver_sphere(radius=8.0, n_freq=201)
+    aw = c2_alpha_table(freq, str(tmp_path / "ag.csv"))
+    assert np.array_equal(aw, s.alpha_table(freq))
