@@ -163,6 +163,20 @@ int pxh_solve_spectrum(pxh_ctx *ctx, int n_freq, const double *freq_au, double e
                        int method, double tol_abs, int krylov_dim, int precision, double *variables, double *polar,
                        double *results, int32_t *iters, double *true_residual);
 
+/* Dense inversion ('inversion' method) of many atomistic targets at once:
+ * ctxs[0..n_ctx) each hold a homogeneous omegaFQ / omegaFQFmu target (and
+ * its alpha table), all on one device; every target is solved at the same
+ * n_freq frequencies.  The targets are grouped by size and the (target,
+ * frequency) matrices of a group are factorised together, padded to the
+ * group's largest n, so one launch of each LU kernel covers many targets
+ * (no counterpart in the reference, which solves one structure per run: the
+ * loop of solve_w_inversion, algorithm_module.f90:189-265, over targets).
+ * Work runs on ctxs[0]'s stream; errors are reported by ctxs[0].
+ *   polar:   n_ctx x n_freq x (3 x 3) complex, may be NULL;
+ *   results: n_ctx blocks of n_freq x 20 (column-major each), may be NULL. */
+int pxh_solve_group(int n_ctx, pxh_ctx *const *ctxs, int n_freq, const double *freq_au, double e0, int rhs_form,
+                    int pol_mask, double *polar, double *results);
+
 /* y = A(omega_i) x for n_vec vectors of n_var complex values: the on-the-fly
  * product plus the diagonal shift.  Replaces target_%product_matrix_vector
  * followed by target_%new_gmres_diagonal_shift (gmres_module.f90:111-121;

@@ -33,6 +33,7 @@ EXPORTS = [
     "pxh_bem_induced_field",
     "pxh_matvec_noalloc",
     "pxh_set_static", "pxh_solve_static",
+    "pxh_solve_group",
 ]
 
 
@@ -66,6 +67,7 @@ def load():
     lib.pxh_matvec.argtypes = [P, D, I, I, P, P]
     lib.pxh_matvec_noalloc.argtypes = [P, D, I, I, P, P]
     lib.pxh_set_static.argtypes = [P, I, P, P, I, P]
+    lib.pxh_solve_group.argtypes = [I, P, I, P, D, I, I, P, P]
     lib.pxh_solve_static.argtypes = [P, I, D, I, P, P, P]
     lib.pxh_induced_field.argtypes = [P, I, P, P, I, D, P]
     lib.pxh_plasmon_density.argtypes = [P, I, P, P, P, P, P]
@@ -274,6 +276,22 @@ class Context:
                 "latency_p50_ms": s[14], "latency_p99_ms": s[15], "bem_gemm_ms": s[16], "bem_gemm_launches": s[17],
                 "bem_gemm_flops": s[18], "lu_update_ms": s[19], "lu_update_launches": s[20],
                 "lu_update_flops": s[21], "bem_gemm_bytes": s[22]}
+
+
+def solve_group(contexts, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7):
+    """Dense inversion of the targets of many contexts at once
+    (pxh_solve_group): per context a dict with 'polar' (n_freq, 3, 3) and
+    'results' (n_freq, 20)."""
+    lib = load()
+    freq = np.ascontiguousarray(freq_au, dtype=np.float64)
+    nf, nc = freq.shape[0], len(contexts)
+    handles = (ctypes.c_void_p * nc)(*[c._h.value for c in contexts])
+    polar = np.zeros((nc, nf, 3, 3), dtype=np.complex128)  # [t][f][b][a]
+    res = np.zeros((nc, 20, nf), dtype=np.float64)
+    st = lib.pxh_solve_group(nc, ctypes.cast(handles, ctypes.c_void_p), nf, _ptr(freq), float(e0), rhs_form,
+                             pol_mask, _ptr(polar), _ptr(res))
+    contexts[0]._check(st)
+    return [{"polar": np.transpose(polar[t], (0, 2, 1)), "results": res[t].T.copy()} for t in range(nc)]
 
 
 class Server:

@@ -331,6 +331,32 @@ int pxh_solve_spectrum(pxh_ctx *c, int n_freq, const double *freq_au, double e0,
     return g != PXH_OK ? g : status;
 }
 
+int pxh_solve_group(int n_ctx, pxh_ctx *const *ctxs, int n_freq, const double *freq_au, double e0, int rhs_form,
+                    int pol_mask, double *polar, double *results) {
+    if (n_ctx <= 0 || !ctxs || !ctxs[0]) return PXH_EINVAL;
+    pxh_ctx *c0 = ctxs[0];
+    int status = PXH_OK;
+    const int g = guarded(c0, [&] {
+        PXH_CHECK(n_freq > 0 && freq_au, PXH_EINVAL, "no frequencies");
+        PXH_CHECK(pol_mask > 0 && pol_mask < 8, PXH_EINVAL, "pol_mask must select x, y and/or z");
+        PXH_CHECK(rhs_form == PXH_RHS_FIELD || rhs_form == PXH_RHS_POTENTIAL, PXH_EINVAL, "unknown rhs_form");
+        for (int t = 0; t < n_ctx; ++t) {
+            const pxh_ctx *c = ctxs[t];
+            PXH_CHECK(c != nullptr, PXH_EINVAL, "null context in the group");
+            PXH_CHECK(c->device == c0->device, PXH_EINVAL, "the contexts of a group must share one device");
+            PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU, PXH_EINVAL,
+                      "every context of a group needs an omegaFQ / omegaFQFmu target");
+            PXH_CHECK(!c->atoms.hetero, PXH_EINVAL, "grouped solves take homogeneous targets (alloys: pxh_solve_spectrum)");
+            PXH_CHECK(c->atoms.model != PXH_WFQFMU || c->alpha_nf >= n_freq, PXH_EINVAL,
+                      "alpha(omega) table shorter than the frequency list");
+            for (int u = 0; u < t; ++u) PXH_CHECK(ctxs[u] != c, PXH_EINVAL, "a context appears twice in the group");
+        }
+        status = pxh::solve_group(ctxs, n_ctx, n_freq, freq_au, e0, rhs_form, pol_mask, polar, results);
+        if (status == PXH_ESINGULAR) c0->last_error = "singular pivot in the LU factorisation";
+    });
+    return g != PXH_OK ? g : status;
+}
+
 int pxh_induced_field(pxh_ctx *c, int n_points, const double *points_bohr, const double *variables, int i_pol,
                       double e0, double *field) {
     if (!c) return PXH_EINVAL;

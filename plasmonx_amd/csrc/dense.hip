@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -1122,6 +1123,98 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
 
 namespace {
 
+// ---------------------------------------------------------------- groups
+// Many targets solved together: the dense matrices of every (target,
+// frequency) slot of a group of targets of similar size are padded to the
+// group's largest n with an identity block, diag(A_g(w), I), and factorised
+// together by the same LU launches (block-diagonal, so partial pivoting never
+// leaves A_g and the padded unknowns come out zero).
+struct GroupSlot {
+    const double *M;       // frequency-independent matrix of the slot's target, nv x nv
+    const double4 *pos;    // its atoms
+    const double *nbfrac;  // its neighbour fractions (nt x n_at)
+    const double2 *rhs;    // its right-hand sides, nv x 3
+    int n_at, nv, nt, fmu;
+    SysParam par;          // the slot's frequency
+};
+
+// A[s] = diag(A_g(w_s), I), N x N column-major; A_g as k_assemble_atoms.
+__global__ void k_assemble_group(int N, const GroupSlot *__restrict__ slots, double2 *__restrict__ A) {
+    const int s = blockIdx.z;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (r >= N) return;
+    const GroupSlot &g = slots[s];
+    double2 v = make_double2(r == col ? 1.0 : 0.0, 0.0);
+    if (r < g.nv && col < g.nv) {
+        v = make_double2(g.M[r + (size_t)g.nv * col], 0.0);
+        if (r == col) {
+            if (r < g.n_at) {
+                const double2 z = g.par.zq[(int)g.pos[r].w];
+                v.x -= z.x;
+                v.y -= z.y;
+            } else if (g.fmu) {
+                const int i = (r - g.n_at) / 3;
+                double2 lam = make_double2(0, 0);
+                for (int t = 0; t < g.nt; ++t) {
+                    lam.x += g.nbfrac[t + (size_t)g.nt * i] * g.par.inv_alpha[t].x;
+                    lam.y += g.nbfrac[t + (size_t)g.nt * i] * g.par.inv_alpha[t].y;
+                }
+                v = lam;
+            }
+        }
+    }
+    A[(size_t)N * N * s + r + (size_t)N * col] = v;
+}
+
+// Z[s][k] = the slot target's right-hand side k (zero-padded to N), for k in mask.
+__global__ void k_group_rhs(int N, int mask, const GroupSlot *__restrict__ slots, double2 *__restrict__ Z) {
+    const int s = blockIdx.z, k = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const GroupSlot &g = slots[s];
+    Z[(size_t)3 * N * s + (size_t)N * k + r] =
+        ((mask >> k) & 1) && r < g.nv ? g.rhs[(size_t)g.nv * k + r] : make_double2(0.0, 0.0);
+}
+
+// out[s][a + 3 b] = sum_i x_b(q_i) r_a(i) (+ mu_a(i) for omegaFQFmu): the
+// polarizability sums of wfq_module.f90:537-559 / wfqfmu_module.f90:894-938
+// for slot s and field direction b = blockIdx.y.
+__global__ __launch_bounds__(256) void k_group_polar(int N, const GroupSlot *__restrict__ slots,
+                                                     const double2 *__restrict__ X, double2 *__restrict__ out) {
+    const int s = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    const GroupSlot &g = slots[s];
+    const double2 *x = X + (size_t)3 * N * s + (size_t)N * b;
+    double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    for (int i = tid; i < g.n_at; i += 256) {
+        const double4 p = g.pos[i];
+        const double2 q = x[i];
+        const double rr[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            acc[a].x += q.x * rr[a];
+            acc[a].y += q.y * rr[a];
+            if (g.fmu) {
+                const double2 mu = x[g.n_at + 3 * i + a];
+                acc[a].x += mu.x;
+                acc[a].y += mu.y;
+            }
+        }
+    }
+    __shared__ double2 sh[3][256];
+    for (int a = 0; a < 3; ++a) sh[a][tid] = acc[a];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off)
+            for (int a = 0; a < 3; ++a) {
+                sh[a][tid].x += sh[a][tid + off].x;
+                sh[a][tid].y += sh[a][tid + off].y;
+            }
+        __syncthreads();
+    }
+    if (tid < 3) out[(size_t)9 * s + tid + 3 * b] = sh[tid][0];
+}
+
 // Static FQ / FQFmu matrix (fq_module.f90:179-207, fqfmu_module.f90:81-140),
 // unknowns [q_1..q_N, lambda_1..lambda_M, mu_1x, mu_1y, mu_1z, ...]: T (the
 // dense atomistic T, unknowns [q, mu], n_t = N or 4N) in the q and mu rows
@@ -1155,6 +1248,121 @@ __global__ void k_assemble_static(int ns, int n, int m, int n_t, const double *_
 }
 
 }  // namespace
+
+// Dense inversions of many atomistic targets together (the frequency loop
+// of solve_w_inversion, algorithm_module.f90:189-265, for every target):
+// targets sorted by size and grouped while the largest n of a group stays
+// within kGroupPad of its smallest; each group's (target, frequency) slots are
+// padded to that n and factorised together in chunks of at most
+// kDenseChunkBytes (and half the free HBM), so one launch of each LU kernel
+// covers many small targets and frequencies.
+constexpr double kGroupPad = 1.12;
+constexpr int kMaxGroupSlots = 8192;
+
+int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq, double e0, int rhs_form,
+                int pol_mask, double *polar, double *results) {
+    pxh_ctx *c0 = ctxs[0];
+    hipStream_t st = c0->stream;
+    for (double &v : c0->stats.v) v = 0.0;  // the counters of this call (pxh_get_stats on ctxs[0])
+    // per target: the constant matrix (kept with the target) and the
+    // right-hand sides, built on its own stream, then joined
+    std::vector<DevBuf<double2>> rhs(n_ctx);
+    for (int t = 0; t < n_ctx; ++t) {
+        pxh_ctx *c = ctxs[t];
+        Atoms &A = c->atoms;
+        const size_t nv = A.n_var();
+        PXH_CHECK(nv * nv * sizeof(double) <= kKeepMconstBytes, PXH_ENOMEM,
+                  "a grouped target is larger than the constant matrix kept with it (2 GB)");
+        if (!A.Mconst_valid) {
+            A.Mconst.alloc(nv * nv);
+            constant_matrix_atoms(c, A.Mconst.p);
+            A.Mconst_valid = true;
+        }
+        rhs[t].alloc(nv * 3);
+        dynamic_rhs_atoms(c, freq[0], e0, rhs_form, rhs[t].p);
+        PXH_HIP(hipStreamSynchronize(c->stream));
+    }
+    std::vector<int> order(n_ctx);
+    for (int t = 0; t < n_ctx; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ctxs[a]->atoms.n_var() < ctxs[b]->atoms.n_var(); });
+    DevBuf<double2> A, Z, X, pol;
+    DevBuf<int> piv, info;
+    DevBuf<SwapSet> swaps;
+    DevBuf<GroupSlot> dslots;
+    int status = PXH_OK;
+    size_t free_b = 0, total_b = 0;
+    PXH_HIP(hipMemGetInfo(&free_b, &total_b));
+    for (size_t g0 = 0; g0 < order.size();) {
+        // the group: targets from g0 while n stays within kGroupPad of the first
+        const int n_first = ctxs[order[g0]]->atoms.n_var();
+        size_t g1 = g0 + 1;
+        while (g1 < order.size() && ctxs[order[g1]]->atoms.n_var() <= kGroupPad * n_first) ++g1;
+        const int N = ctxs[order[g1 - 1]]->atoms.n_var();
+        const size_t per = (size_t)N * N * sizeof(double2);
+        size_t cap = std::max<size_t>(1, std::min(kDenseChunkBytes, free_b / 2) / per);
+        if (c0->budget) cap = std::max<size_t>(1, std::min(cap, c0->budget / per));
+        cap = std::min<size_t>(cap, kMaxGroupSlots);
+        // slots target-major: (order[g0], w0), (order[g0], w1), ...
+        const size_t n_slots = (g1 - g0) * (size_t)n_freq;
+        for (size_t s0 = 0; s0 < n_slots; s0 += cap) {
+            const int cnt = (int)std::min(cap, n_slots - s0);
+            std::vector<GroupSlot> hs(cnt);
+            for (int k = 0; k < cnt; ++k) {
+                const size_t sl = s0 + k;
+                pxh_ctx *c = ctxs[order[g0 + sl / n_freq]];
+                const int w = (int)(sl % n_freq);
+                Atoms &At = c->atoms;
+                hs[k] = GroupSlot{At.Mconst.p, At.pos.p, At.d_nbfrac.p, rhs[order[g0 + sl / n_freq]].p, At.n,
+                                  At.n_var(), At.nt, At.model == PXH_WFQFMU ? 1 : 0, make_sysparam(c, freq[w], w)};
+            }
+            dslots.upload(hs.data(), hs.size(), st);
+            A.alloc((size_t)N * N * cnt);
+            Z.alloc((size_t)3 * N * cnt);
+            X.alloc((size_t)3 * N * cnt);
+            piv.alloc((size_t)2 * N * cnt);
+            swaps.alloc(cnt);
+            info.alloc(cnt);
+            pol.alloc((size_t)9 * cnt);
+            k_assemble_group<<<dim3((N + 255) / 256, N, cnt), 256, 0, st>>>(N, dslots.p, A.p);
+            PXH_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * cnt, st));
+            lu_factor(c0, N, cnt, A.p, piv.p, swaps.p, info.p);
+            int *perm = piv.p + (size_t)N * cnt;
+            k_perm_from_piv<<<cnt, 256, 0, st>>>(N, piv.p, perm);
+            k_group_rhs<<<dim3((N + 255) / 256, 3, cnt), 256, 0, st>>>(N, pol_mask, dslots.p, Z.p);
+            k_lu_solve<<<dim3(3, 1, cnt), kSolveThreads, 0, st>>>(N, A.p, (size_t)N * N, Z.p, (size_t)3 * N, N,
+                                                                   pol_mask);
+            k_scatter_rows<<<dim3((N + 255) / 256, 3, cnt), 256, 0, st>>>(N, perm, Z.p, (size_t)3 * N, X.p,
+                                                                          (size_t)3 * N);
+            k_group_polar<<<dim3(cnt, 3), 256, 0, st>>>(N, dslots.p, X.p, pol.p);
+            PXH_HIP(hipGetLastError());
+            std::vector<double2> hp((size_t)9 * cnt);
+            std::vector<int> hi(cnt);
+            PXH_HIP(hipMemcpyAsync(hp.data(), pol.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipMemcpyAsync(hi.data(), info.p, sizeof(int) * cnt, hipMemcpyDeviceToHost, st));
+            PXH_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < cnt; ++k) {
+                const size_t sl = s0 + k;
+                const int t = order[g0 + sl / n_freq], w = (int)(sl % n_freq);
+                if (hi[k] != 0) status = PXH_ESINGULAR;
+                double pl[18];
+                for (int e = 0; e < 9; ++e) {
+                    pl[2 * e] = hp[(size_t)9 * k + e].x / e0;
+                    pl[2 * e + 1] = hp[(size_t)9 * k + e].y / e0;
+                }
+                if (polar) std::memcpy(polar + 18 * ((size_t)t * n_freq + w), pl, sizeof pl);
+                if (results) {
+                    double r[20];
+                    results_row(freq[w], pl, r);
+                    double *res_t = results + (size_t)20 * n_freq * t;  // n_freq x 20, column-major
+                    for (int e = 0; e < 20; ++e) res_t[w + (size_t)n_freq * e] = r[e];
+                }
+            }
+        }
+        g0 = g1;
+    }
+    return status;
+}
 
 // Static FQ / FQFmu solve (algorithm_module.f90:469-504, DSYSV there; here the
 // batched complex LU of the inversion path on the real symmetric system).

@@ -250,6 +250,13 @@ void build_mesh_device(pxh_ctx *c);
 void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lambda, const double2 *x, double2 *y,
                int ld, bool fp32);
 void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
+// dense.hip: dense inversions of many atomistic targets together (all on
+// one device, the same frequencies); polar n_ctx x n_freq x 9 complex,
+// results n_ctx blocks of n_freq x 20 (column-major)
+int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq, double e0, int rhs_form,
+                int pol_mask, double *polar, double *results);
+// solver.hip: the 20 result columns of one frequency from its 3 x 3 complex polarizability
+void results_row(double freq, const double *pol, double *r);
 // dense.hip: static FQ / FQFmu (what 0 ground state, 1 static field)
 int solve_static(pxh_ctx *c, int what, double e0, int rhs_form, double *variables, double *polar, double *energy);
 // solver.cpp

@@ -121,6 +121,18 @@ module pxh
          type(c_ptr), value :: x, y
       end function
 
+      ! ctxs: array of n_ctx context handles
+      integer(c_int) function pxh_solve_group(n_ctx, ctxs, n_freq, freq_au, e0, rhs_form, pol_mask, polar, &
+                                              results) bind(C, name="pxh_solve_group")
+         import :: c_int, c_double, c_double_complex, c_ptr
+         integer(c_int), value :: n_ctx, n_freq, rhs_form, pol_mask
+         type(c_ptr), intent(in) :: ctxs(*)
+         real(c_double), intent(in) :: freq_au(*)
+         real(c_double), value :: e0
+         complex(c_double_complex), intent(out) :: polar(3, 3, *)
+         real(c_double), intent(out) :: results(*)
+      end function
+
       integer(c_int) function pxh_constant_matrix(ctx, m) bind(C, name="pxh_constant_matrix")
          import :: c_int, c_double, c_ptr
          type(c_ptr), value :: ctx
