@@ -8,24 +8,24 @@
 // factors, the two dipole-dipole coefficients) and the distance vector d:
 //     row q:    [ T_qq,      g1 d_x,                g1 d_y,   g1 d_z ]
 //     row mu_a: [ -g2 d_a,   c1 d_a d_b + c0 delta_ab  (b = x, y, z) ]
-// The scalars are generated on the fly in FP32 (one work-item per pair,
-// stored in LDS), the elements are built in registers directly in the MFMA
-// A-operand layout, split into fp16 hi + lo (value * 2^tscale), and
-// contracted with the block X (4N x 32*NCB real columns: re / im of each
-// system, also fp16 hi + lo with a power-of-two scale per column) by
+// The scalars are generated on the fly in FP32 by the lane that uses them,
+// the elements are built in registers directly in the MFMA A-operand
+// layout, split into fp16 hi + lo (value * 2^tscale), and contracted with
+// the block X (4N x 32*NCB real columns: re / im of each system, also fp16
+// hi + lo with a power-of-two scale per column) by
 // v_mfma_f32_32x32x16_f16: acc += Ahi Bhi + Ahi Blo + Alo Bhi, a product
-// with a 22-bit significand, accumulated in FP32 for a few source steps and
-// in FP64 beyond.
+// with a 22-bit significand, accumulated in FP32 in two levels.
 //
 // Beyond the far-field radius the Gaussian damping (erf -> 1, exp -> 0) is
 // below FP32 resolution and the scalars take their Coulomb limits, so the
 // erf / exp evaluations run only for the near pairs of a wavefront.
 //
-// Workgroup: 256 work-items (4 waves), 32 target atoms (128 rows); wave w
-// owns rows [32w, 32w+32) for all 32*NCB columns.  Each source step stages
-// 16 atoms (K = 64): every work-item evaluates the scalars of 2 pairs, then
-// each wave builds its A fragments and runs 4 k-steps x NCB column blocks x
-// 3 MFMAs.
+// What bounds the kernel on gfx950: a 32x32x16 MFMA issues every 32 cycles
+// per SIMD and hides only a few VALU instructions; the rest of the element
+// generation adds to it, and every X fragment read from LDS costs LDS
+// bandwidth.  Hence: no LDS round trip for the T elements (each lane builds
+// the rows it multiplies), each X fragment read feeds two row blocks, and
+// the X fragments reach LDS by DMA instead of through registers.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -50,7 +50,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kXTarget = 13;       // columns are scaled so that max |x| * s lies in [2^12, 2^13)
 constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 constexpr double kFourOverSqrtPi = 2.2567583341910252;
-constexpr int kAtomsI = 32;        // target atoms per workgroup
+constexpr int kAtomsI = 64;        // target atoms per workgroup
 constexpr int kAtomsJ = 16;        // source atoms per step (K = 64)
 constexpr int kStepsPerFlush = 8;  // source steps between FP32 -> FP64 flushes
 
@@ -135,18 +135,31 @@ __device__ __forceinline__ float qmu_factor(float a, float inv_r3) {
     return (erff(a) - (float)kTwoOverSqrtPi * a * __expf(-a * a)) * inv_r3;
 }
 
+// erf(x) for x >= 0 given e = exp(-x^2), Abramowitz & Stegun 7.1.26:
+// 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e with t = 1 / (1 + p x),
+// absolute error <= 1.5e-7.  Used where every argument of the wave is >= 1,
+// so that the charge-dipole and dipole-dipole differences
+// erf(a) - 2a/sqrt(pi) e keep a relative error of the same order.
+__device__ __forceinline__ float erf_as(float x, float e) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    const float p = fmaf(fmaf(fmaf(fmaf(1.061405429f, t, -1.453152027f), t, 1.421413741f), t, -0.284496736f), t,
+                         0.254829592f);
+    return fmaf(-p * t, e, 1.0f);
+}
+
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 
 // Split of a float pair into fp16 hi + lo: hi = RN16(v), lo = RN16(v - hi).
-// v - hi is exact in FP32; v_fma_mix_f32 reads hi as fp16 directly.
+// v - hi is exact in FP32; v_fma_mix{lo,hi}_f16 read hi as fp16 and write
+// the rounded difference straight into the low / high half of lo.
 __device__ __forceinline__ void split2(float v0, float v1, half2v &hi, half2v &lo) {
     hi = __builtin_convertvector((float2v){v0, v1}, half2v);
-    float r0, r1;
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r0) : "v"(hi), "v"(v0));
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r1) : "v"(hi), "v"(v1));
-    lo = __builtin_convertvector((float2v){r0, r1}, half2v);
+    unsigned r;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(hi), "v"(v0));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(r) : "v"(hi), "v"(v1));
+    lo = __builtin_bit_cast(half2v, r);
 }
 
 __device__ __forceinline__ void split4(float e0, float e1, float e2, float e3, half4 &hi, half4 &lo) {
@@ -157,14 +170,75 @@ __device__ __forceinline__ void split4(float e0, float e1, float e2, float e3, h
     lo = (half4){l0[0], l0[1], l1[0], l1[1]};
 }
 
-// Workgroup: 256 work-items (4 waves), 32 target atoms; wave a owns the 32
-// rows of component a (0: q, 1-3: mu_x, mu_y, mu_z) of those atoms, for all
-// 32*NCB columns.  Per source step (16 atoms, K = 64):
-//   1. every work-item evaluates the 4 x 4 blocks of 2 pairs, splits them
-//      into fp16 hi / lo and stores row a of each block in wave a's
-//      A-fragment layout in LDS: [a][source pair jp = jl / 2][target il][jl % 2][4];
-//   2. each wave runs 4 k-steps x NCB column blocks x 3 MFMAs with A and B
-//      fragments read straight from LDS.
+// The pair scalars of the 4 x 4 block of (i, j), scaled by tsc:
+// o = [T_qq, g1, -g2, c0, c1, dx, dy, dz] (rows q: [T_qq, g1 d];
+// mu_a: [-g2 d_a, c1 d_a d_b + c0 delta_ab]).  An excluded pair (i = j or
+// beyond n) gets r^2 = 3e38, whose scaled elements round to zero in fp16.
+// Beyond the far-field radius the Coulomb limits share one scaled 1/r^3.
+__device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, const double4 &pj, int ti, int nt,
+                                             int hetero, float rfar2, float tsc, const float (*spc)[4], float *o) {
+    const float dx = (float)(pi.x - pj.x);
+    const float dy = (float)(pi.y - pj.y);
+    const float dz = (float)(pi.z - pj.z);
+    const float r2 = valid ? fmaf(dx, dx, fmaf(dy, dy, dz * dz)) : 3.0e38f;
+    const float inv_r = __builtin_amdgcn_rsqf(r2);  // r2 is never denormal here
+    const float inv_r2 = inv_r * inv_r;
+    float tqq = inv_r * tsc;
+    float c0 = tqq * inv_r2;
+    float g1 = c0, g2 = -c0;
+    float c1 = -3.f * c0 * inv_r2;
+    if (r2 < rfar2) {
+        const float inv_r3 = inv_r2 * inv_r;
+        const int tj = (int)pj.w;
+        const float r = r2 * inv_r;
+        const float *pp = spc[ti + nt * tj];
+        const float a1 = r * pp[0], a2 = r * pp[1], a3 = r * pp[2];
+        const float a2h = hetero ? r * spc[tj + nt * ti][1] : a2;
+        const float e3 = __expf(-a3 * a3);
+        float fd1;
+        if (__all(fminf(fminf(a1, a2), fminf(a3, a2h)) >= 1.0f)) {
+            const float e1 = __expf(-a1 * a1), e2 = __expf(-a2 * a2);
+            tqq = erf_as(a1, e1) * inv_r * tsc;
+            g1 = (erf_as(a2, e2) - (float)kTwoOverSqrtPi * a2 * e2) * inv_r3 * tsc;
+            if (hetero) {
+                const float e2h = __expf(-a2h * a2h);
+                g2 = -(erf_as(a2h, e2h) - (float)kTwoOverSqrtPi * a2h * e2h) * inv_r3 * tsc;
+            } else {
+                g2 = -g1;
+            }
+            fd1 = erf_as(a3, e3) - (float)kTwoOverSqrtPi * a3 * e3;
+        } else {
+            tqq = erff(a1) * inv_r * tsc;
+            g1 = qmu_factor(a2, inv_r3) * tsc;
+            g2 = -(hetero ? qmu_factor(a2h, inv_r3) * tsc : g1);
+            fd1 = erff(a3) - (float)kTwoOverSqrtPi * a3 * e3;
+        }
+        const float fd2 = (float)kFourOverSqrtPi * pp[2] * pp[2] * pp[2] * e3;
+        c0 = fd1 * inv_r3 * tsc;
+        c1 = (fd2 * inv_r2 - 3.f * fd1 * inv_r3 * inv_r2) * tsc;
+    }
+    o[0] = tqq;
+    o[1] = g1;
+    o[2] = g2;
+    o[3] = c0;
+    o[4] = c1;
+    o[5] = dx;
+    o[6] = dy;
+    o[7] = dz;
+}
+
+// Workgroup: 256 work-items (4 waves), kAtomsI = 64 target atoms.  Wave w
+// owns targets 32 (w / 2) .. 32 (w / 2) + 31 and components 2 (w % 2) and
+// 2 (w % 2) + 1: two MFMA row blocks (row = target, block = component) for
+// all 32 * NCB columns.  Per source step (16 atoms, K = 64) the X fragments
+// and the source positions reach LDS through the DMA path of the vector
+// memory unit (double-buffered, issued one step ahead, one barrier per
+// step).  In k-step s lane (target r, k half h) evaluates the pairs
+// (r, source 4s + 2h + p), p = 0, 1, builds its two rows of both 4 x 4
+// blocks in registers as the A fragments (fp16 hi / lo) and runs
+// 2 x NCB x 3 MFMAs; each X fragment read from LDS feeds both row blocks.
+// Accumulation: FP32 in the MFMA registers for kStepsPerFlush source
+// steps, then into a second FP32 accumulator (blocked summation).
 // Blocks are placed so that all workgroups of one source split share an XCD
 // (split = linear block id mod splits): with splits a multiple of 8 an XCD's
 // L2 holds the X fragments of its split.
@@ -177,15 +251,15 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
                                                         const _Float16 *__restrict__ xl, double2 *__restrict__ part,
                                                         int nv, int splits, int jsteps_per_split,
                                                         int nsys_total_in_part, int sys_offset) {
-    __shared__ half8 sBh[4 * NCB * 64];  // [k-step][cb][lane]
-    __shared__ half8 sBl[4 * NCB * 64];
-    __shared__ half8 sAh[4 * 8 * kAtomsI];  // [a][jp][il] (half8 = rows a of pairs 2jp, 2jp+1)
-    __shared__ half8 sAl[4 * 8 * kAtomsI];
+    __shared__ half8 sBh[2][4 * NCB * 64];  // [buffer][k-step][cb][lane]
+    __shared__ half8 sBl[2][4 * NCB * 64];
+    __shared__ double4 spos[2][kAtomsJ];
     __shared__ float spc[kMaxTypes * kMaxTypes][4];
     const int tid = threadIdx.x, lane = tid % 64;
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int cp = wave % 2;  // rows 2 cp and 2 cp + 1
     const int split = blockIdx.x % splits;
-    const int i0 = (blockIdx.x / splits) * kAtomsI;
+    const int i0 = (blockIdx.x / splits) * kAtomsI + 32 * (wave / 2);
     const int jb0 = split * jsteps_per_split;
     const int jb_end = min((n + kAtomsJ - 1) / kAtomsJ, jb0 + jsteps_per_split);
     for (int e = tid; e < nt * nt; e += 256) {
@@ -194,131 +268,107 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
         spc[e][2] = (float)pc[e].inv_rmumu;
         spc[e][3] = (float)pc[e].eta_bar;
     }
-    // evaluation role: target il, sources jl0 and jl0 + 8
-    const int il = tid % 32;
-    const int jl0 = tid / 32;
-    const int gi = i0 + il;
+    const int kh = lane / 32;
+    const int gi = i0 + lane % 32;
     const double4 pi = gi < n ? pos[gi] : make_double4(0, 0, 0, 0);
     const int ti = (int)pi.w;
-    f32x16 acc[NCB];
-    double accd[NCB][16];
+    f32x16 acc[2][NCB];
+    float acc2[2][NCB][16];
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            acc[cb][v] = 0.f;
-            accd[cb][v] = 0.0;
-        }
-    // operands of the next source step are fetched into registers one step
-    // ahead, so that their latency hides behind the MFMA block
-    half8 nbh[NCB], nbl[NCB];  // X fragments: elements tid + 256 r of the step
-    double4 npos[2];           // positions of sources jl0 and jl0 + 8
-    auto fetch = [&](int jb) {
-        const size_t base = (size_t)jb * 4 * NCB * 64 + tid;
-#pragma unroll
-        for (int r = 0; r < NCB; ++r) {
-            nbh[r] = *reinterpret_cast<const half8 *>(xh + (base + 256 * r) * 8);
-            nbl[r] = *reinterpret_cast<const half8 *>(xl + (base + 256 * r) * 8);
-        }
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int gj = jb * kAtomsJ + jl0 + 8 * p;
-            npos[p] = gj < n ? pos[gj] : make_double4(0, 0, 0, 0);
-        }
-    };
-    if (jb0 < jb_end) fetch(jb0);
-    // source steps in blocks of kStepsPerFlush: FP32 accumulation inside a
-    // block (the accumulators stay in the MFMA registers), FP64 across blocks
-    for (int jbb = jb0; jbb < jb_end; jbb += kStepsPerFlush) {
-        const int jbe = min(jb_end, jbb + kStepsPerFlush);
-        for (int jb = jbb; jb < jbe; ++jb) {
-            __syncthreads();
-            // ---- X fragments of this source step to LDS (4 * NCB * 64 half8 = 256 NCB)
-#pragma unroll
-            for (int r = 0; r < NCB; ++r) {
-                sBh[tid + 256 * r] = nbh[r];
-                sBl[tid + 256 * r] = nbl[r];
-            }
-            // ---- 4 x 4 blocks of (il, jl0) and (il, jl0 + 8), fp16 hi / lo to LDS
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const int jl = jl0 + 8 * p;
-                const int gj = jb * kAtomsJ + jl;
-                float tqq = 0.f, g1 = 0.f, g2 = 0.f, c0 = 0.f, c1 = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
-                if (gi < n && gj < n && gi != gj) {
-                    const double4 pj = npos[p];
-                    dx = (float)(pi.x - pj.x);
-                    dy = (float)(pi.y - pj.y);
-                    dz = (float)(pi.z - pj.z);
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    const float inv_r = rsqrtf(r2);
-                    const float inv_r2 = inv_r * inv_r, inv_r3 = inv_r2 * inv_r;
-                    tqq = inv_r;
-                    g1 = g2 = c0 = inv_r3;
-                    c1 = -3.f * inv_r3 * inv_r2;
-                    if (r2 < rfar2) {
-                        const int tj = (int)pj.w;
-                        const float r = r2 * inv_r;
-                        const float *pp = spc[ti + nt * tj];
-                        tqq = erff(r * pp[0]) * inv_r;
-                        g1 = qmu_factor(r * pp[1], inv_r3);
-                        g2 = hetero ? qmu_factor(r * spc[tj + nt * ti][1], inv_r3) : g1;
-                        const float a3 = r * pp[2];
-                        const float e3 = __expf(-a3 * a3);
-                        const float fd1 = erff(a3) - (float)kTwoOverSqrtPi * a3 * e3;
-                        const float fd2 = (float)kFourOverSqrtPi * pp[2] * pp[2] * pp[2] * e3;
-                        c0 = fd1 * inv_r3;
-                        c1 = fd2 * inv_r2 - 3.f * fd1 * inv_r3 * inv_r2;
-                    }
-                    tqq *= tsc;
-                    g1 *= tsc;
-                    g2 *= -tsc;
-                    c0 *= tsc;
-                    c1 *= tsc;
-                }
-                const float ux = c1 * dx, uy = c1 * dy, uz = c1 * dz;
-                const float xy = ux * dy, xz = ux * dz, yz = uy * dz;
-                half4 h[4], l[4];
-                split4(tqq, g1 * dx, g1 * dy, g1 * dz, h[0], l[0]);
-                split4(g2 * dx, fmaf(ux, dx, c0), xy, xz, h[1], l[1]);
-                split4(g2 * dy, xy, fmaf(uy, dy, c0), yz, h[2], l[2]);
-                split4(g2 * dz, xz, yz, fmaf(uz, dz, c0), h[3], l[3]);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int idx = (a * 8 + jl / 2) * kAtomsI + il;
-                    reinterpret_cast<half4 *>(&sAh[idx])[jl % 2] = h[a];
-                    reinterpret_cast<half4 *>(&sAl[idx])[jl % 2] = l[a];
-                }
-            }
-            if (jb + 1 < jb_end) fetch(jb + 1);
-            __syncthreads();
-            // ---- 4 k-steps x NCB column blocks x 3 products; A row = target lane % 32,
-            // k = 8 (lane / 32) + h = component h % 4 of source 2 (lane / 32) + h / 4
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int ia = (wave * 8 + 2 * s + lane / 32) * kAtomsI + lane % 32;
-                const half8 ah = sAh[ia];
-                const half8 al = sAl[ia];
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb) {
-                    const half8 bh = sBh[(s * NCB + cb) * 64 + lane];
-                    const half8 bl = sBl[(s * NCB + cb) * 64 + lane];
-                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[cb], 0, 0, 0);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[cb], 0, 0, 0);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[cb], 0, 0, 0);
-                }
-            }
-        }
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                accd[cb][v] += (double)acc[cb][v];
-                acc[cb][v] = 0.f;
+                acc[rb][cb][v] = 0.f;
+                acc2[rb][cb][v] = 0.f;
             }
+    // LDS DMA (global_load_lds): lane L's 16 bytes land at the wave's LDS
+    // base + 16 L.  X fragments: element tid + 256 r of the step; positions:
+    // 32 lanes of wave 0 copy the 16 double4 of the step.
+    auto issue = [&](int jb, int b) {
+        const size_t base = (size_t)jb * 4 * NCB * 64 + tid;
+#pragma unroll
+        for (int r = 0; r < NCB; ++r) {
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(xh + (base + 256 * r) * 8),
+                (__attribute__((address_space(3))) void *)(&sBh[b][64 * wave + 256 * r]), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(xl + (base + 256 * r) * 8),
+                (__attribute__((address_space(3))) void *)(&sBl[b][64 * wave + 256 * r]), 16, 0, 0);
+        }
+        if (wave == 0 && lane < 2 * kAtomsJ) {
+            const int e = min(jb * kAtomsJ + lane / 2, n - 1);  // sources past n are excluded by gj < n
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(reinterpret_cast<const char *>(pos + e) +
+                                                                 16 * (lane % 2)),
+                (__attribute__((address_space(3))) void *)(&spos[b][0]), 16, 0, 0);
+        }
+    };
+    if (jb0 < jb_end) issue(jb0, 0);
+    int buf = 0;
+    for (int jbb = jb0; jbb < jb_end; jbb += kStepsPerFlush) {
+        const int jbe = min(jb_end, jbb + kStepsPerFlush);
+        for (int jb = jbb; jb < jbe; ++jb) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();  // step jb is in LDS; every wave is done with step jb - 1's buffer
+            if (jb + 1 < jb_end) issue(jb + 1, buf ^ 1);
+            const half8 *bh_ = sBh[buf];
+            const half8 *bl_ = sBl[buf];
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s) {
+                half4 h[2][2], l[2][2];  // [row block][pair]
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const int jl = 4 * s + 2 * kh + p;
+                    const int gj = jb * kAtomsJ + jl;
+                    float sc[8];
+                    pair_scalars(gi < n && gj < n && gi != gj, pi, spos[buf][jl], ti, nt, hetero, rfar2, tsc, spc,
+                                 sc);
+                    const float tqq = sc[0], g1 = sc[1], g2 = sc[2], c0 = sc[3], c1 = sc[4];
+                    const float dx = sc[5], dy = sc[6], dz = sc[7];
+                    if (cp == 0) {  // rows q and mu_x
+                        const float ux = c1 * dx;
+                        split4(tqq, g1 * dx, g1 * dy, g1 * dz, h[0][p], l[0][p]);
+                        split4(g2 * dx, fmaf(ux, dx, c0), ux * dy, ux * dz, h[1][p], l[1][p]);
+                    } else {  // rows mu_y and mu_z
+                        const float uy = c1 * dy, uz = c1 * dz;
+                        split4(g2 * dy, uy * dx, fmaf(uy, dy, c0), uy * dz, h[0][p], l[0][p]);
+                        split4(g2 * dz, uz * dx, uz * dy, fmaf(uz, dz, c0), h[1][p], l[1][p]);
+                    }
+                }
+                // A row = target lane % 32, k = 8 (lane / 32) + e = component e % 4 of
+                // source 4s + 2 (lane / 32) + e / 4
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const half8 bh = bh_[(s * NCB + cb) * 64 + lane];
+                    const half8 bl = bl_[(s * NCB + cb) * 64 + lane];
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb) {
+                        const half8 ah = (half8){h[rb][0][0], h[rb][0][1], h[rb][0][2], h[rb][0][3],
+                                                 h[rb][1][0], h[rb][1][1], h[rb][1][2], h[rb][1][3]};
+                        const half8 al = (half8){l[rb][0][0], l[rb][0][1], l[rb][0][2], l[rb][0][3],
+                                                 l[rb][1][0], l[rb][1][1], l[rb][1][2], l[rb][1][3]};
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[rb][cb], 0, 0, 0);
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[rb][cb], 0, 0, 0);
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rb][cb], 0, 0, 0);
+                    }
+                }
+            }
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    acc2[rb][cb][v] += acc[rb][cb][v];
+                    acc[rb][cb][v] = 0.f;
+                }
     }
-    // ---- write partials: acc[cb][v] -> target atom m = (v/4)*8 + (lane/32)*4 + v%4 of the
-    // workgroup, component = wave, column cb*32 + lane%32
+    // ---- write partials: acc2[rb][cb][v] -> target i0 + 8 (v / 4) + 4 (lane / 32) + v % 4,
+    // component 2 cp + rb, column cb * 32 + lane % 32
     const double tinv = 1.0 / (double)tsc;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
@@ -327,14 +377,17 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
         if (sys >= nsys) continue;
         const double cs = scale[col] > 0.f ? tinv / (double)scale[col] : 0.0;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int i = i0 + (v / 4) * 8 + (lane / 32) * 4 + (v % 4);
-            if (i >= n) continue;
-            const int row = wave == 0 ? i : n + 3 * i + wave - 1;
-            double *dst = reinterpret_cast<double *>(
-                part + ((size_t)split * nv + row) * nsys_total_in_part + sys_offset + sys);
-            dst[col & 1] = accd[cb][v] * cs;
-        }
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = i0 + (v / 4) * 8 + (lane / 32) * 4 + (v % 4);
+                if (i >= n) continue;
+                const int c = 2 * cp + rb;
+                const int row = c == 0 ? i : n + 3 * i + c - 1;
+                double *dst = reinterpret_cast<double *>(
+                    part + ((size_t)split * nv + row) * nsys_total_in_part + sys_offset + sys);
+                dst[col & 1] = (double)acc2[rb][cb][v] * cs;
+            }
     }
 }
 
@@ -434,10 +487,10 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     const int iblocks = (n + kAtomsI - 1) / kAtomsI;
     const int groups = (nsys + SYS_PER_GROUP - 1) / SYS_PER_GROUP;
     // source splits: a multiple of 8 (one XCD per split residue) once every
-    // split still spans >= 16 source steps, and >= ~2048 workgroups
+    // split still spans >= 16 source steps, and >= ~1024 workgroups (4 per CU)
     int splits = c->opt.splits;
     if (splits <= 0) {
-        splits = std::max(1, (2048 + iblocks - 1) / iblocks);
+        splits = std::max(1, (1024 + iblocks - 1) / iblocks);
         if (jsteps >= 8 * 16) splits = std::min((splits + 7) / 8 * 8, jsteps / 16 / 8 * 8);
         splits = std::max(1, std::min(splits, jsteps / 16));
     }
