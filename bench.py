@@ -334,7 +334,7 @@ def leg_c1(a):
             "whole_path_tflops": round(path_tf, 2),
             "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                         "kernel": "k_gemm_update (LU trailing updates, v_mfma_f64_16x16x4_f64), 8 flop per "
+                         "kernel": "rocBLAS zgemm_strided_batched (LU trailing updates, FP64 MFMA), 8 flop per "
                                    "complex multiply-add; HIP events per launch",
                          "launches": upd_n, "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4)},
             "cpu_baseline": cpu}
@@ -479,7 +479,7 @@ def run_c4(a):
          dtype="fp64", max_rel_err_vs_alone=max(errs), unpadded_lu_tflops=round(flops / dt / 1e12, 2),
          roofline={"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                   "kernel": "k_gemm_update (LU trailing updates, padded matrices)",
+                   "kernel": "rocBLAS zgemm_strided_batched (LU trailing updates, padded matrices)",
                    "launches": int(st["lu_update_launches"]),
                    "avg_launch_ms": round(st["lu_update_ms"] / max(st["lu_update_launches"], 1), 4)},
          cpu_baseline=None)

@@ -7,6 +7,7 @@
 // Layout: one n x n column-major complex (double2) matrix per frequency,
 // frequencies contiguous; every kernel takes blockIdx.z as the frequency.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstring>
@@ -501,102 +502,6 @@ __global__ __launch_bounds__(128) void k_swap_trsm(int n, int k, int nb, double2
         if (i < nb) a[(size_t)n * (k + i)] = x[i];
 }
 
-// B22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4_f64, FP64
-// throughout).  Output tile of (32 BR) x (32 BC) per workgroup of 256; wave w
-// takes the quarter (w & 1, w >> 1) as BR x BC blocks of 16 x 16.  In the
-// accumulator layout a lane holds 4 elements of one column of a block, and
-// the 16 lanes of a row group hold 16 consecutive columns: with B row-major
-// these are full 256-byte segments in the read-modify-write.  With complex
-// operands split into parts,
-//   Re C += Lr Ur - Li Ui,   Im C += Lr Ui + Li Ur,
-// and one 16-byte load gives a lane both parts of its operand.  Operands come
-// straight from global memory (the L and U blocks are shared by many tiles
-// and stay in L2), one K step of 4 loaded ahead of the products; the products
-// of one K step go block by block, so that 2 BR BC independent products
-// separate two into the same accumulator.
-// Rows from k + nb, columns [k + nb, c_end); nb is a multiple of 4.
-typedef double d4v __attribute__((ext_vector_type(4)));
-template <int BR, int BC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gemm_update(
-    int n, int k, int nb, int c_end, double2 *__restrict__ A, size_t lda_f) {
-    constexpr int TR = 32 * BR, TC = 32 * BC;
-    const int f = blockIdx.z;
-    double2 *Af = A + lda_f * f;
-    const int r0 = k + nb + blockIdx.x * TR, c0 = k + nb + blockIdx.y * TC;
-    if (r0 >= n || c0 >= c_end) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = lane & 15, kk = lane >> 4;
-    const int rw = r0 + 16 * BR * (w & 1), cw = c0 + 16 * BC * (w >> 1);
-    // operand rows / columns of this lane, clamped into the matrix (the
-    // products of clamped lanes land outside the stored range)
-    const double2 *lp[BR], *up[BC];
-#pragma unroll
-    for (int b = 0; b < BR; ++b) lp[b] = Af + (k + kk) + (size_t)n * min(rw + 16 * b + i, n - 1);
-#pragma unroll
-    for (int b = 0; b < BC; ++b) up[b] = Af + min(cw + 16 * b + i, c_end - 1) + (size_t)n * (k + kk);
-    d4v P[BC][BR], Q[BC][BR];
-#pragma unroll
-    for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-        for (int rb = 0; rb < BR; ++rb) {
-            P[cb][rb] = d4v{0, 0, 0, 0};
-            Q[cb][rb] = d4v{0, 0, 0, 0};
-        }
-    double2 l[BR], u[BC];
-#pragma unroll
-    for (int b = 0; b < BR; ++b) l[b] = lp[b][0];
-#pragma unroll
-    for (int b = 0; b < BC; ++b) u[b] = up[b][0];
-    for (int kb = 0; kb < nb; kb += 4) {
-        double2 ln[BR], un[BC];
-        if (kb + 4 < nb) {
-#pragma unroll
-            for (int b = 0; b < BR; ++b) ln[b] = lp[b][kb + 4];
-#pragma unroll
-            for (int b = 0; b < BC; ++b) un[b] = up[b][(size_t)n * (kb + 4)];
-        }
-#pragma unroll
-        for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-            for (int rb = 0; rb < BR; ++rb)
-                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].x, u[cb].x, P[cb][rb], 0, 0, 0);
-#pragma unroll
-        for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-            for (int rb = 0; rb < BR; ++rb)
-                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].y, u[cb].x, Q[cb][rb], 0, 0, 0);
-#pragma unroll
-        for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-            for (int rb = 0; rb < BR; ++rb)
-                P[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-l[rb].y, u[cb].y, P[cb][rb], 0, 0, 0);
-#pragma unroll
-        for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-            for (int rb = 0; rb < BR; ++rb)
-                Q[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(l[rb].x, u[cb].y, Q[cb][rb], 0, 0, 0);
-#pragma unroll
-        for (int b = 0; b < BR; ++b) l[b] = ln[b];
-#pragma unroll
-        for (int b = 0; b < BC; ++b) u[b] = un[b];
-    }
-    // accumulator element v of a lane: block row lane / 16 + 4 v, block
-    // column lane % 16
-#pragma unroll
-    for (int cb = 0; cb < BC; ++cb)
-#pragma unroll
-        for (int rb = 0; rb < BR; ++rb)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = rw + 16 * rb + kk + 4 * v, c = cw + 16 * cb + i;
-                if (r < n && c < c_end) {
-                    double2 &a = Af[c + (size_t)n * r];
-                    a.x -= P[cb][rb][v];
-                    a.y -= Q[cb][rb][v];
-                }
-            }
-}
-
 // Solve U^T (L^T y) = b in place (column-major view: U^T is the lower
 // triangle with the diagonal, L^T the unit upper triangle) for the
 // right-hand sides selected by mask; one workgroup per (rhs, frequency).
@@ -910,6 +815,21 @@ __global__ void k_dense_matvec(int n, const double2 *__restrict__ A, size_t lda_
     y[(size_t)ld * s + r] = acc;
 }
 
+// The context's rocBLAS handle (created on first use, bound to the context's
+// stream; atomics off, so that every result is reproducible bit for bit).
+rocblas_handle blas_handle(pxh_ctx *c) {
+    if (!c->blas) {
+        rocblas_handle h = nullptr;
+        PXH_CHECK(rocblas_create_handle(&h) == rocblas_status_success, PXH_EHIP, "rocblas_create_handle failed");
+        c->blas = h;
+        PXH_CHECK(rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) == rocblas_status_success, PXH_EHIP,
+                  "rocblas_set_atomics_mode failed");
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(c->blas);
+    PXH_CHECK(rocblas_set_stream(h, c->stream) == rocblas_status_success, PXH_EHIP, "rocblas_set_stream failed");
+    return h;
+}
+
 // Right-looking blocked LU of nf matrices with partial pivoting.  Panels go in
 // pairs where a trailing part follows: the second panel's columns take the
 // first panel's update alone, the second panel's window rows of the columns
@@ -940,22 +860,33 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         else
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
     };
-    // every trailing-update launch timed on the context's stream, with its
-    // algorithmic flops: 8 real flop per complex multiply-add
-    auto timed_update = [&](double rows, double cols, double depth, auto launch) {
+    // The trailing updates are plain complex GEMMs, done by rocBLAS
+    // (zgemm_strided_batched over the chunk's matrices).  In B = A^T
+    // row-major, rows [r0, n) and columns [r0, c_end) take
+    // B[r][c] -= sum_d B[r][k + d] B[k + d][c]; the same memory read
+    // column-major is X = B^T, and the update is
+    // X(c0:c_end, r0:n) -= X(c0:c_end, k:k+depth) X(k:k+depth, r0:n).
+    // Each launch is timed on the context's stream, with its algorithmic
+    // flops: 8 real flop per complex multiply-add.
+    rocblas_handle h = blas_handle(c);
+    auto update = [&](int k, int depth, int r0, int c0, int c_end) {
+        const int rows = n - r0, cols = c_end - c0;
+        if (rows <= 0 || cols <= 0) return;
+        const rocblas_double_complex minus_one(-1.0, 0.0), one(1.0, 0.0);
+        const rocblas_stride sf = (rocblas_stride)ldf;
+        const auto *X = reinterpret_cast<rocblas_double_complex *>(A);
+        auto *Xw = reinterpret_cast<rocblas_double_complex *>(A);
         const int t = timer_begin(c);
-        launch();
+        const rocblas_status rs = rocblas_zgemm_strided_batched(
+            h, rocblas_operation_none, rocblas_operation_none, cols, rows, depth, &minus_one,
+            X + c0 + (size_t)n * k, n, sf, X + k + (size_t)n * r0, n, sf, &one, Xw + c0 + (size_t)n * r0, n, sf, nf);
         timer_end(c, t, 19);
+        PXH_CHECK(rs == rocblas_status_success, PXH_EHIP, "rocblas_zgemm_strided_batched failed");
         c->stats.v[20] += 1;
         c->stats.v[21] += 8.0 * rows * cols * depth * nf;
     };
     // trailing update of the rows and columns from k + depth
-    auto trailing = [&](int k, int depth) {
-        const int rest = n - k - depth;
-        const int t = (rest + 63) / 64;
-        timed_update(rest, rest, depth,
-                     [&] { k_gemm_update<2, 2><<<dim3(t, t, nf), 256, 0, st>>>(n, k, depth, n, A, ldf); });
-    };
+    auto trailing = [&](int k, int depth) { update(k, depth, k + depth, k + depth, n); };
     int k = 0;
     while (k < n) {
         const int nb = std::min(NB, n - k);
@@ -966,9 +897,7 @@ void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, 
         if (rest > NB) {
             // second panel of the pair, with a trailing part after it
             const int k2 = k + NB, rest2 = rest - NB;
-            timed_update(rest, NB, NB, [&] {
-                k_gemm_update<4, 1><<<dim3((rest + 127) / 128, 1, nf), 256, 0, st>>>(n, k, NB, k2 + NB, A, ldf);
-            });
+            update(k, NB, k + NB, k + NB, k2 + NB);
             panel(k2, NB);
             k_swap_trsm<<<dim3((rest2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapRight);
             k_swap_trsm<<<dim3((k2 + 127) / 128, 1, nf), 128, 0, st>>>(n, k2, NB, A, ldf, swaps, kSwapLeft);
@@ -1464,6 +1393,11 @@ int solve_static(pxh_ctx *c, int what, double e0, int rhs_form, double *variable
                 polar[a + 3 * bb] = -sum;
             }
     return PXH_OK;
+}
+
+void release_blas(pxh_ctx *c) {
+    if (c->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
+    c->blas = nullptr;
 }
 
 }  // namespace pxh

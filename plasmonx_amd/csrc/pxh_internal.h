@@ -187,6 +187,7 @@ struct pxh_ctx {
     pxh::DevBuf<double> work_gemm;            // split-K partial sums of those products
     pxh::DevBuf<double2> work_lu;             // dense path: the factorised matrices of one frequency chunk
     pxh::DevBuf<int> work_piv;                // their row interchanges, and the permutations built from them
+    void *blas = nullptr;                     // rocBLAS handle of the dense path (created on first use)
     bool use_stored = false;                  // in-memory method: products use the stored T
     // IEF operator form used by the products: 0 the system matrix
     // -(D + lambda I) S; 1 the right-preconditioned -(D + lambda I) (unknown
@@ -200,6 +201,8 @@ namespace pxh {
 int timer_begin(pxh_ctx *c);
 void timer_end(pxh_ctx *c, int t, int stat_index);
 void timer_drain(pxh_ctx *c);
+// dense.hip: releases the context's rocBLAS handle
+void release_blas(pxh_ctx *c);
 // noalloc.hip: looks up this device's scratch array of pxh_matvec_noalloc
 // (once per device, with the device current)
 void scratch_setup(int device);
