@@ -65,6 +65,11 @@ def main():
     ap.add_argument("--requests", type=int, default=24, help="c5: requests in the stream")
     ap.add_argument("--flakes", type=int, default=16, help="c4: flakes (the full batch is 96)")
     ap.add_argument("--flake-max", type=int, default=4000, help="c4: largest flake (the full batch reaches 12000)")
+    ap.add_argument("--c4-freqs", type=int, default=128,
+                    help="c4: solve only this many of the 128 frequencies (evenly spread; for profiling)")
+    ap.add_argument("--freq-blocks", type=int, default=1,
+                    help="c4: the timed step solves the 128 frequencies in this many consecutive grouped solves, "
+                         "each logged when done")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: only --gpus 1 is measured", "n_gpus": a.gpus}))
@@ -442,6 +447,8 @@ def run_c4(a):
     from plasmonx_amd import Context, _lib
     from plasmonx_amd.workloads import c4_flakes
     flakes, freq = c4_flakes(a.flakes, hi=a.flake_max)
+    if a.c4_freqs < len(freq):
+        freq = freq[np.linspace(0, len(freq) - 1, max(1, a.c4_freqs)).astype(int)]
     log(f"[bench] C4: {len(flakes)} flakes, {sum(f.n_atoms for f in flakes)} atoms, {len(freq)} w x 2 fields")
     t = time.perf_counter()
     ctxs = []
@@ -453,35 +460,48 @@ def run_c4(a):
     # warm-up on a few frequencies (kernels loaded, constant matrices kept with the targets)
     _lib.solve_group(ctxs, freq[:2], 1e-4, pol_mask=3)
     times = []
+    upd_ms = upd_flops = upd_n = 0.0
+    blocks = np.array_split(np.arange(len(freq)), max(1, a.freq_blocks))
     for _ in range(a.steps):
         t = time.perf_counter()
-        outs = _lib.solve_group(ctxs, freq, 1e-4, pol_mask=3)
+        parts = []
+        for b, idx in enumerate(blocks):
+            parts.append(_lib.solve_group(ctxs, freq[idx], 1e-4, pol_mask=3))
+            st = ctxs[0].stats()  # the grouped solve's statistics (reset by each call)
+            upd_ms += st["lu_update_ms"]
+            upd_flops += st["lu_update_flops"]
+            upd_n += st["lu_update_launches"]
+            if len(blocks) > 1:
+                log(f"[bench] C4 frequency block {b + 1}/{len(blocks)} done ({time.perf_counter() - t:.1f}s)")
+        outs = [{"polar": np.concatenate([p[k]["polar"] for p in parts]),
+                 "results": np.concatenate([p[k]["results"] for p in parts])} for k in range(len(flakes))]
         times.append(time.perf_counter() - t)
     dt = sum(times) / len(times)
-    st = ctxs[0].stats()
     # check: two flakes alone (pxh_solve_spectrum) at one frequency
     errs = []
     for k in (0, len(flakes) // 2):
-        r = ctxs[k].solve_spectrum(flakes[k].n_var, freq[64:65], 1e-4, pol_mask=3, method=_lib.INVERSION)
-        errs.append(float(np.linalg.norm(outs[k]["polar"][64] - r["polar"][0]) / np.linalg.norm(r["polar"][0])))
+        mid = len(freq) // 2
+        r = ctxs[k].solve_spectrum(flakes[k].n_var, freq[mid:mid + 1], 1e-4, pol_mask=3, method=_lib.INVERSION)
+        errs.append(float(np.linalg.norm(outs[k]["polar"][mid] - r["polar"][0]) / np.linalg.norm(r["polar"][0])))
     for c in ctxs:
         c.close()
     pts = len(flakes) * len(freq)
     flops = sum((8.0 / 3.0) * s.n_var ** 3 for s in flakes) * len(freq)
-    achieved = st["lu_update_flops"] / (st["lu_update_ms"] * 1e-3) / 1e12 if st["lu_update_ms"] > 0 else 0.0
+    achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
     log(f"[bench] C4 {len(flakes)} flakes x {len(freq)} w: {pts / dt:.1f} fp/s ({dt:.2f}s), err vs alone "
         f"{max(errs):.1e}")
     emit("flake-frequency solves/s (C4 graphene batch)", pts / dt, "freq-point solves/s", 1e3 * dt, a,
          {"workload": f"{len(flakes)} seeded graphene flakes (500-{a.flake_max} atoms; the full batch is 96 flakes of "
-                     "500-12k atoms), 128 w 0.1-1.0 eV, x and y fields, grouped dense inversion (one solve, flakes "
-                     "of similar size padded to a common n)",
-          "atoms_total": int(sum(s.n_atoms for s in flakes)), "setup_s": round(t_setup, 2)},
+                     f"500-12k atoms), {len(freq)} w in 0.1-1.0 eV (of 128), x and y fields, grouped dense inversion "
+                     "(flakes of similar size padded to a common n and factorised by the same LU launches)",
+          "atoms_total": int(sum(s.n_atoms for s in flakes)), "setup_s": round(t_setup, 2),
+          "freq_blocks": len(blocks)},
          dtype="fp64", max_rel_err_vs_alone=max(errs), unpadded_lu_tflops=round(flops / dt / 1e12, 2),
          roofline={"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
                    "kernel": "rocBLAS zgemm_strided_batched (LU trailing updates, padded matrices)",
-                   "launches": int(st["lu_update_launches"]),
-                   "avg_launch_ms": round(st["lu_update_ms"] / max(st["lu_update_launches"], 1), 4)},
+                   "launches": int(upd_n),
+                   "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4)},
          cpu_baseline=None)
 
 

@@ -122,6 +122,7 @@ int pxh_ctx_destroy(pxh_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pxh::release_blas(c);
+    pxh::release_solver_ws(c);
     for (int k = 0; k < pxh::TimerRing::kN; ++k) {
         if (c->timers.beg[k]) (void)hipEventDestroy(c->timers.beg[k]);
         if (c->timers.end[k]) (void)hipEventDestroy(c->timers.end[k]);

@@ -188,6 +188,7 @@ struct pxh_ctx {
     pxh::DevBuf<double2> work_lu;             // dense path: the factorised matrices of one frequency chunk
     pxh::DevBuf<int> work_piv;                // their row interchanges, and the permutations built from them
     void *blas = nullptr;                     // rocBLAS handle of the dense path (created on first use)
+    void *solver_ws = nullptr;                // work buffers of the iterative solvers (solver.hip, grow-only)
     bool use_stored = false;                  // in-memory method: products use the stored T
     // IEF operator form used by the products: 0 the system matrix
     // -(D + lambda I) S; 1 the right-preconditioned -(D + lambda I) (unknown
@@ -203,6 +204,8 @@ void timer_end(pxh_ctx *c, int t, int stat_index);
 void timer_drain(pxh_ctx *c);
 // dense.hip: releases the context's rocBLAS handle
 void release_blas(pxh_ctx *c);
+// solver.hip: releases the context's solver work buffers
+void release_solver_ws(pxh_ctx *c);
 // noalloc.hip: looks up this device's scratch array of pxh_matvec_noalloc
 // (once per device, with the device current)
 void scratch_setup(int device);

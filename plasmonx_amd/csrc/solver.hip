@@ -498,8 +498,15 @@ struct HostRing {
     ~HostRing() {
         if (base) (void)hipHostFree(base);
     }
+    // grow-only: a ring that already holds n chunks of this size is reused
+    // (the previous solve has synchronised its stream before returning)
     void init(size_t chunk_bytes, int n) {
-        chunk = (chunk_bytes + 255) / 256 * 256;
+        const size_t want = (chunk_bytes + 255) / 256 * 256;
+        next = 0;
+        if (base && want <= chunk && n <= count) return;
+        if (base) (void)hipHostFree(base);
+        base = nullptr;
+        chunk = want;
         count = n;
         PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&base), chunk * (size_t)n, hipHostMallocDefault));
     }
@@ -531,11 +538,43 @@ struct Batched {
     size_t ldv_slot = 0, ldH_slot = 0;
     int ldH = 0, ldg = 0, ldh = 0;
     HostRing ring;                 // pinned staging of the uploads
-    double *pin_res = nullptr;     // pinned readback of per-slot reals (B)
+    double *pin_res = nullptr;     // pinned readback of per-slot reals (2 B)
+    size_t pin_n = 0;
     ~Batched() {
         if (pin_res) (void)hipHostFree(pin_res);
     }
+    // per-call state back to its defaults; the buffers stay (grow-only)
+    void reset(pxh_ctx *ctx) {
+        c = ctx;
+        nv = m = B = 0;
+        mixed = plain = precond = false;
+        ldv_slot = ldH_slot = 0;
+        ldH = ldg = ldh = 0;
+    }
+    void pin(size_t n) {
+        if (pin_res && n <= pin_n) return;
+        if (pin_res) (void)hipHostFree(pin_res);
+        pin_res = nullptr;
+        PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&pin_res), sizeof(double) * n, hipHostMallocDefault));
+        pin_n = n;
+    }
 };
+
+// Work buffers of the iterative solvers, kept with the context between calls
+// (grow-only), so that a solve allocates only its Krylov basis and what
+// scales with the number of systems: at a few ms per spectrum (BEM
+// multi-shift) a round of hipMalloc / hipHostMalloc per call is a visible
+// share of the time.
+struct SolverWs {
+    Batched b;
+    DevBuf<double2> V, Wb, Hs, gq, snq, colq, sig;  // multi-shift
+    DevBuf<double> csq, msres, mspart, mstres;
+};
+
+SolverWs &solver_ws(pxh_ctx *c) {
+    if (!c->solver_ws) c->solver_ws = new SolverWs();
+    return *static_cast<SolverWs *>(c->solver_ws);
+}
 
 // Small host-to-device uploads go through a ring of pinned chunks, so that
 // the copies are asynchronous and need no synchronisation of their own: the
@@ -1291,11 +1330,11 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
                          int rhs_fstride, int rhs_stride, const double *freq, double tol, int kdim, int precision, bool plain,
                          bool warm, double2 *d_sol /* [n_freq][nv x 3] */, std::vector<int> &iters_out,
                          std::vector<double> &tres_out, std::vector<double> &t_admit, std::vector<double> &t_done) {
-    Batched S;
+    Batched &S = solver_ws(c).b;
+    S.reset(c);
     DevBuf<VT> V;
     const auto t_start = std::chrono::steady_clock::now();
     auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
-    S.c = c;
     S.nv = nv;
     S.mixed = precision == PXH_MIXED;
     S.plain = plain;
@@ -1353,8 +1392,7 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
     S.slots_d.alloc(std::max<size_t>(1, (size_t)B));
     S.ring.init(std::max({sizeof(SysParam), sizeof(int) * (size_t)B, sizeof(long long) * (kProjMax + 1) * (size_t)B}),
                 4 * B + 64);
-    PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.pin_res), sizeof(double) * 2 * (size_t)B,
-                          hipHostMallocDefault));
+    S.pin(2 * (size_t)B);
 
     // projected starting iterates (atomistic targets, cold start): the
     // systems are admitted coarse to fine over the frequency grid, so that
@@ -1827,18 +1865,6 @@ static bool shift_family(const pxh_ctx *c, int n_freq, const double *freq, bool 
 }
 
 namespace {
-struct PinnedDoubles {
-    double *p = nullptr;
-    explicit PinnedDoubles(size_t n) {
-        PXH_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), sizeof(double) * std::max<size_t>(n, 1),
-                              hipHostMallocDefault));
-    }
-    ~PinnedDoubles() {
-        if (p) (void)hipHostFree(p);
-    }
-    PinnedDoubles(const PinnedDoubles &) = delete;
-    PinnedDoubles &operator=(const PinnedDoubles &) = delete;
-};
 }  // namespace
 
 // Multi-shift GMRES over a shift family (shift_family): for each field
@@ -1863,8 +1889,9 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
     const int F = (int)pols.size();
     const int nsh = n_freq;
     const int m = std::max(1, std::min(kdim, nv));
-    Batched S;
-    S.c = c;
+    SolverWs &ws = solver_ws(c);
+    Batched &S = ws.b;
+    S.reset(c);
     S.nv = nv;
     S.m = m;
     S.B = std::max(F, std::max(1, std::min<int>(c->opt.batch, (int)systems.size())));
@@ -1903,8 +1930,9 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         S.par.alloc(B);
         S.h_par.assign(B, SysParam{});
     }
-    DevBuf<double2> V, Wb, Hs, gq, snq, colq, sig;
-    DevBuf<double> csq, msres, mspart, mstres;
+    DevBuf<double2> &V = ws.V, &Wb = ws.Wb, &Hs = ws.Hs, &gq = ws.gq, &snq = ws.snq, &colq = ws.colq,
+                    &sig = ws.sig;
+    DevBuf<double> &csq = ws.csq, &msres = ws.msres, &mspart = ws.mspart, &mstres = ws.mstres;
     V.alloc(S.ldv_slot * F);
     Wb.alloc(S.ldv_slot * F);  // A0 v_j, kept for the true residuals
     const int fin_blk = (nv + 255) / 256;
@@ -1918,7 +1946,10 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
     colq.alloc((size_t)nsh * F * ldg);
     msres.alloc((size_t)nsh * F);
     sig.upload(sigma.data(), nsh, st);
-    PinnedDoubles pin((size_t)nsh * F + B);
+    S.pin((size_t)nsh * F + B);
+    struct {
+        double *p;
+    } pin{S.pin_res};
     auto grid_rows = [&](int nsl) { return dim3((nv + 255) / 256, nsl); };
 
     // system index of (frequency, field slot)
@@ -2406,6 +2437,11 @@ size_t estimate_solve_bytes(const pxh_ctx *c, int n_freq, int pol_mask, int meth
         if (!bem && method == PXH_ITERATIVE) b += nv * nv * sizeof(double);  // stored T
     }
     return b + b / 10;
+}
+
+void release_solver_ws(pxh_ctx *c) {
+    delete static_cast<SolverWs *>(c->solver_ws);
+    c->solver_ws = nullptr;
 }
 
 }  // namespace pxh
