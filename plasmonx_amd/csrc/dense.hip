@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pxh_internal.h"
@@ -225,15 +226,29 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel(int n, int k, int nb, d
 // apply its interchanges to the other columns of the panel as one net
 // permutation.  Same pivots as k_panel; column c below the diagonal is scaled
 // by the reciprocal of the pivot, as LAPACK's ZGETF2 does.
-template <int R>
+//
+// PK: the panel has been copied to a column-major buffer (k_panel_pack:
+// column c - k of matrix f at P + (f NB + c - k) ldp, rows from k), so that a
+// column read is one contiguous run instead of one element per row of the
+// row-major matrix; k_panel_unpack copies it back.
+template <int R, bool PK>
 __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int nb, double2 *__restrict__ A,
                                                              size_t lda_f, int *__restrict__ piv,
-                                                             SwapSet *__restrict__ swaps, int *__restrict__ info) {
+                                                             SwapSet *__restrict__ swaps, int *__restrict__ info,
+                                                             double2 *__restrict__ P, int ldp) {
     constexpr int SUB = NB / R < 8 ? NB / R : 8;
     constexpr int W = kPanelThreads / 64;
     constexpr int kUpdUnroll = R <= 8 ? 4 : 1;  // L loads in flight per row in the rank update
     const int f = blockIdx.z;
     double2 *Af = A + lda_f * f;
+    double2 *Pf = PK ? P + (size_t)f * NB * ldp : nullptr;
+    // element (row r, column col) of the panel, rows >= k, k <= col < k + nb
+    auto el = [&](int r, int col) -> double2 & {
+        if constexpr (PK)
+            return Pf[(size_t)(col - k) * ldp + (r - k)];
+        else
+            return Af[col + (size_t)n * r];
+    };
     const int tid = threadIdx.x, lane = tid % 64, wid = tid / 64;
     __shared__ double wv[2][W];
     __shared__ int wi[2][W];
@@ -252,13 +267,13 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             const int r = k + tid + kPanelThreads * i;
 #pragma unroll
             for (int j = 0; j < SUB; ++j)
-                v[i][j] = (r < n && j < sb) ? Af[(s0 + j) + (size_t)n * r] : make_double2(0, 0);
+                v[i][j] = (r < n && j < sb) ? el(r, s0 + j) : make_double2(0, 0);
         }
         if (done > 0) {
             // U rows [k, s0): forward substitution with the unit lower block
             for (int e = tid; e < done * done; e += kPanelThreads) {
                 const int rr = e % done, cc = e / done;
-                Lw[rr][cc] = Af[(k + cc) + (size_t)n * (k + rr)];
+                Lw[rr][cc] = el(k + rr, k + cc);
             }
             if (tid < done)
 #pragma unroll
@@ -285,7 +300,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
                 if (r >= s0 && r < n) {
 #pragma unroll kUpdUnroll
                     for (int q = 0; q < done; ++q) {
-                        const double2 l = Af[(k + q) + (size_t)n * r];
+                        const double2 l = el(r, k + q);
 #pragma unroll
                         for (int j = 0; j < SUB; ++j) {
                             const double2 u = Uw[q][j];
@@ -405,7 +420,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             if (r < n)
 #pragma unroll
                 for (int j = 0; j < SUB; ++j)
-                    if (j < sb) Af[(s0 + j) + (size_t)n * r] = v[i][j];
+                    if (j < sb) el(r, s0 + j) = v[i][j];
         }
         __syncthreads();
         // its interchanges on the other columns of the panel
@@ -414,14 +429,16 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
             __syncthreads();
             if (tid < nb - sb) {
                 const int col = tid < done ? k + tid : k + tid + sb;
-                double2 *a = Af + col;
+                // a[ld * row] = element (row, col)
+                double2 *a = PK ? Pf + (size_t)(col - k) * ldp - k : Af + col;
+                const size_t ld = PK ? 1 : (size_t)n;
                 double2 *x = v[0];  // the sub-panel is written back; its registers are free
 #pragma unroll
-                for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[(size_t)n * ps_src_w[j]] : make_double2(0, 0);
-                move_rows(a, n, ps_src, ps_dst, ps_m);
+                for (int j = 0; j < SUB; ++j) x[j] = j < sb ? a[ld * ps_src_w[j]] : make_double2(0, 0);
+                move_rows(a, ld, ps_src, ps_dst, ps_m);
 #pragma unroll
                 for (int j = 0; j < SUB; ++j)
-                    if (j < sb) a[(size_t)n * (s0 + j)] = x[j];
+                    if (j < sb) a[ld * (s0 + j)] = x[j];
             }
             __syncthreads();
         }
@@ -430,6 +447,40 @@ __global__ __launch_bounds__(kPanelThreads) void k_panel_reg(int n, int k, int n
     if (tid < 64) {
         SwapSet &S = swaps[f];
         net_permutation(tid, k, nb, spiv, S.src_w, S.dst_o, S.src_o, &S.m_o);
+    }
+}
+
+// Copy of the panel (rows [k, n), columns [k, k + nb)) between the row-major
+// matrix and the column-major panel buffer, 64 rows per workgroup through an
+// LDS tile: each row's nb columns are one contiguous read (or write), each
+// column's 64 rows one contiguous write (or read).
+template <bool TO_P>
+__global__ __launch_bounds__(256) void k_panel_copy(int n, int k, int nb, double2 *__restrict__ A, size_t lda_f,
+                                                    double2 *__restrict__ P, int ldp) {
+    __shared__ double2 tile[NB][65];
+    const int f = blockIdx.z, r0 = k + 64 * blockIdx.x;
+    double2 *Af = A + lda_f * f;
+    double2 *Pf = P + (size_t)f * NB * ldp;
+    if (TO_P) {
+        for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+            const int rr = e / NB, cc = e % NB, r = r0 + rr;
+            if (r < n && cc < nb) tile[cc][rr] = Af[(k + cc) + (size_t)n * r];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+            const int cc = e / 64, rr = e % 64, r = r0 + rr;
+            if (r < n && cc < nb) Pf[(size_t)cc * ldp + (r - k)] = tile[cc][rr];
+        }
+    } else {
+        for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+            const int cc = e / 64, rr = e % 64, r = r0 + rr;
+            if (r < n && cc < nb) tile[cc][rr] = Pf[(size_t)cc * ldp + (r - k)];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+            const int rr = e / NB, cc = e % NB, r = r0 + rr;
+            if (r < n && cc < nb) Af[(k + cc) + (size_t)n * r] = tile[cc][rr];
+        }
     }
 }
 
@@ -839,26 +890,46 @@ rocblas_handle blas_handle(pxh_ctx *c) {
 void lu_factor(pxh_ctx *c, int n, int nf, double2 *A, int *piv, SwapSet *swaps, int *info) {
     hipStream_t st = c->stream;
     const size_t ldf = (size_t)n * n;
+    // panels of more than 4 kPanelThreads rows are factorised in the
+    // column-major panel buffer (contiguous column reads)
+    const bool pack = n > 4 * kPanelThreads;
+    double2 *P = nullptr;
+    if (pack) {
+        c->work_panel.alloc((size_t)nf * NB * n);
+        P = c->work_panel.p;
+    }
     auto panel = [&](int k, int nb) {
         const int rows = n - k;
+        auto reg = [&](auto r_tag, auto pk_tag) {
+            constexpr int R = decltype(r_tag)::value;
+            constexpr bool PK = decltype(pk_tag)::value;
+            k_panel_reg<R, PK><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info, P, n);
+        };
+        using T = std::true_type;
+        using F = std::false_type;
         if (rows <= kPanelThreads)
-            k_panel_reg<1><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+            reg(std::integral_constant<int, 1>{}, F{});
         else if (rows <= 2 * kPanelThreads)
-            k_panel_reg<2><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+            reg(std::integral_constant<int, 2>{}, F{});
         else if (rows <= 4 * kPanelThreads)
-            k_panel_reg<4><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else if (rows <= 8 * kPanelThreads)
-            k_panel_reg<8><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else if (rows <= 12 * kPanelThreads)
-            k_panel_reg<12><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else if (rows <= 16 * kPanelThreads)
-            k_panel_reg<16><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else if (rows <= 20 * kPanelThreads)
-            k_panel_reg<20><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else if (rows <= 24 * kPanelThreads)
-            k_panel_reg<24><<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
-        else
+            reg(std::integral_constant<int, 4>{}, F{});
+        else if (rows <= 24 * kPanelThreads) {
+            const dim3 cg((rows + 63) / 64, 1, nf);
+            k_panel_copy<true><<<cg, 256, 0, st>>>(n, k, nb, A, ldf, P, n);
+            if (rows <= 8 * kPanelThreads)
+                reg(std::integral_constant<int, 8>{}, T{});
+            else if (rows <= 12 * kPanelThreads)
+                reg(std::integral_constant<int, 12>{}, T{});
+            else if (rows <= 16 * kPanelThreads)
+                reg(std::integral_constant<int, 16>{}, T{});
+            else if (rows <= 20 * kPanelThreads)
+                reg(std::integral_constant<int, 20>{}, T{});
+            else
+                reg(std::integral_constant<int, 24>{}, T{});
+            k_panel_copy<false><<<cg, 256, 0, st>>>(n, k, nb, A, ldf, P, n);
+        } else {
             k_panel<<<dim3(1, 1, nf), kPanelThreads, 0, st>>>(n, k, nb, A, ldf, piv, swaps, info);
+        }
     };
     // The trailing updates are plain complex GEMMs, done by rocBLAS
     // (zgemm_strided_batched over the chunk's matrices).  In B = A^T

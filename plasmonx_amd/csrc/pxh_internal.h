@@ -187,6 +187,7 @@ struct pxh_ctx {
     pxh::DevBuf<double> work_gemm;            // split-K partial sums of those products
     pxh::DevBuf<double2> work_lu;             // dense path: the factorised matrices of one frequency chunk
     pxh::DevBuf<int> work_piv;                // their row interchanges, and the permutations built from them
+    pxh::DevBuf<double2> work_panel;          // dense path: column-major copy of the current panel (tall panels)
     void *blas = nullptr;                     // rocBLAS handle of the dense path (created on first use)
     void *solver_ws = nullptr;                // work buffers of the iterative solvers (solver.hip, grow-only)
     bool use_stored = false;                  // in-memory method: products use the stored T
