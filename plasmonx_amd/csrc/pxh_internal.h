@@ -87,6 +87,7 @@ struct Atoms {
     std::vector<double> h_nbfrac;       // nt*n: nb_counts/nb_total
     std::vector<PairConst> h_pair;      // nt*nt
     DevBuf<double4> pos;                // x, y, z, type (as double)
+    DevBuf<int> tperm;                  // target order of the VALU pair kernels (Morton order of the atoms)
     DevBuf<float4> posf;                // centred single-precision copy
     DevBuf<PairConst> pair;
     DevBuf<double> d_eta;               // per type
