@@ -63,6 +63,11 @@ def main():
                     help="c2 (default): the headline omegaFQFmu spectrum; c1 dense LU omegaFQ Na; c3 BEM meshes; "
                          "c4 graphene flake batch; c5 request stream")
     ap.add_argument("--requests", type=int, default=24, help="c5: requests in the stream")
+    ap.add_argument("--c5-rate", type=float, default=200.0, help="c5: Poisson arrival rate (requests/s)")
+    ap.add_argument("--c5-atoms", default="2000,12000", help="c5: core-shell sphere atoms, min,max (full: 2000,60000)")
+    ap.add_argument("--c5-rod-atoms", default="", help="c5: Na rod atoms, min,max (default: as --c5-atoms)")
+    ap.add_argument("--c5-tesserae", default="2000,8000", help="c5: BEM rod tesserae, min,max (full: 2000,20000)")
+    ap.add_argument("--c5-freqs", default="50,100", help="c5: frequencies per request, min,max (full: 50,400)")
     ap.add_argument("--flakes", type=int, default=16, help="c4: flakes (the full batch is 96)")
     ap.add_argument("--flake-max", type=int, default=4000, help="c4: largest flake (the full batch reaches 12000)")
     ap.add_argument("--c4-freqs", type=int, default=128,
@@ -505,14 +510,15 @@ def run_c4(a):
          cpu_baseline=None)
 
 
-def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100)):
+def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100),
+           rod_atoms=None):
     """C5: a seeded Poisson stream of spectrum requests -- omegaFQ Na rods,
     omegaFQFmu Au@Ag core-shell spheres, BEM rods -- served by the request
     loop; latency from arrival (target setup included) to result, and each
     request's error against an FP64 solve at one frequency."""
     from plasmonx_amd import Context, _lib
     from plasmonx_amd.workloads import c3_permittivity, c5_requests
-    reqs = c5_requests(n_requests, rate=rate, atoms=atoms, tesserae=tesserae, n_freq=n_freq)
+    reqs = c5_requests(n_requests, rate=rate, atoms=atoms, tesserae=tesserae, n_freq=n_freq, rod_atoms=rod_atoms)
 
     def c_model(r):
         return getattr(r["target"], "model", "bem")
@@ -579,7 +585,9 @@ def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 80
             "config": {"workload": f"{len(reqs)} Poisson arrivals at {rate:g}/s (seed 7), arrivals over "
                                    f"{arr[-1]:.2f}s: omegaFQ Na rods (dense LU), omegaFQFmu Au@Ag core-shell "
                                    f"spheres (GMRES-IR on the fly), BEM rods (multi-shift GMRES); "
-                                   f"{atoms[0]}-{atoms[1]} atoms, {tesserae[0]}-{tesserae[1]} tesserae, "
+                                   f"{atoms[0]}-{atoms[1]} atoms (core-shell), "
+                                   f"{(rod_atoms or atoms)[0]}-{(rod_atoms or atoms)[1]} atoms (rods), "
+                                   f"{tesserae[0]}-{tesserae[1]} tesserae, "
                                    f"{n_freq[0]}-{n_freq[1]} w each, 3 fields, 4 in flight",
                        "requests": kinds, "freq_points": pts},
             "latency": {"p50_ms": round(float(np.percentile(lat, 50)), 1),
@@ -589,7 +597,11 @@ def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 80
 
 
 def run_c5(a):
-    leg = leg_c5(a, n_requests=a.requests)
+    def pair(txt):
+        lo, hi = (int(v) for v in txt.split(","))
+        return lo, hi
+    leg = leg_c5(a, n_requests=a.requests, rate=a.c5_rate, atoms=pair(a.c5_atoms), tesserae=pair(a.c5_tesserae),
+                 n_freq=pair(a.c5_freqs), rod_atoms=pair(a.c5_rod_atoms) if a.c5_rod_atoms else None)
     emit(leg.pop("metric"), leg.pop("value"), leg.pop("unit"), leg.pop("ms_per_step"), a, leg.pop("config"), **leg)
 
 

@@ -129,12 +129,15 @@ def sodium_rod(length_ang, radius_ang):
                       types=base.types, fermi_d=base.fermi_d, fermi_s=base.fermi_s, r0=base.r0)
 
 
-def c5_requests(n_requests=24, seed=7, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100)):
+def c5_requests(n_requests=24, seed=7, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100),
+                rod_atoms=None):
     """C5: a seeded Poisson stream.  Each request is a dict with its arrival
     time (s), kind, a builder of its target and its frequency list (eV ranges
     as the C1-C3 configurations).  Sizes are drawn log-uniform in the given
-    ranges (the full configuration is 2k-60k atoms, 2k-20k tesserae, 50-400
-    frequencies; the bounds are parameters)."""
+    ranges (the full configuration is 2k-60k atoms for the core-shell
+    spheres, 2k-20k tesserae, 50-400 frequencies; the bounds are parameters).
+    rod_atoms bounds the omegaFQ Na rods separately (default: atoms)."""
+    rod_atoms = rod_atoms or atoms
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -153,7 +156,7 @@ def c5_requests(n_requests=24, seed=7, rate=200.0, atoms=(2000, 12000), tesserae
             build = (lambda v=verts: Mesh(verts_bohr=v, variant="dpcm", green="accurate"))
             freq = ev(np.linspace(3.0, 4.0, nf))
         elif kind == "wfq_rod":
-            na = float(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1]))))
+            na = float(np.exp(rng.uniform(np.log(rod_atoms[0]), np.log(rod_atoms[1]))))
             aspect = float(rng.uniform(2.0, 4.0))
             # fcc Na: 4 atoms per 4.23^3 A^3; volume of a capped rod pi r^2 (L - 2r/3)
             radius = (na * 4.23 ** 3 / 4 / (np.pi * (2 * aspect - 2.0 / 3.0))) ** (1.0 / 3.0)
