@@ -271,7 +271,10 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * second pass of each vector to the next step), "res_batch" / "res_wait"
  * (FP64 residual products wait for 8 systems, at most 5 steps), "autotune"
  * (1: the first full low-precision pass of a new atom count times candidate
- * source splits and keeps the fastest for the process). */
+ * source splits and keeps the fastest for the process), "graphs" (1, the
+ * default: the multi-shift Arnoldi step on a BEM target is captured once as
+ * a HIP graph and replayed, the graph kept with the context while its sizes
+ * and buffers stay the same; 0 = every kernel launched directly). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
 /* Counters of the last pxh_solve_spectrum call (n <= 24): [0] vector
@@ -286,7 +289,9 @@ int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
  * products (4 flop per matrix element and system), [19] / [20] / [21] ms,
  * launches and flops of the dense LU's trailing updates (8 flop per complex
  * multiply-add), [22] algorithmic bytes of the BEM matrix products (the
- * matrix once plus each vector read and written). */
+ * matrix once plus each vector read and written).  [16]-[18] and [22] count
+ * the launches made directly only: the replays of a captured multi-shift
+ * step (option "graphs") are neither timed nor counted there. */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
 /* ------------------------------------------------------------------ serving

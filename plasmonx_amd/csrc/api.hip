@@ -49,6 +49,7 @@ void timer_collect(pxh_ctx *c, int t) {
 }  // namespace
 
 int timer_begin(pxh_ctx *c) {
+    if (c->capturing) return -1;  // an event record would become a graph node
     TimerRing &R = c->timers;
     const int t = R.head;
     if (R.open[t]) return -1;  // as many timers open as the ring holds
@@ -542,6 +543,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.res_wait = (int)value;
         } else if (k == "autotune") {
             c->opt.autotune = value != 0.0;
+        } else if (k == "graphs") {
+            c->opt.graphs = value != 0.0;
         } else if (k == "dcgs2") {
             c->opt.dcgs2 = value != 0.0;
         } else if (k == "initial_guess") {

@@ -518,7 +518,6 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
     Mesh &M = c->mesh;
     const int n = M.n;
     hipStream_t st = c->stream;
-    c->work_t.alloc((size_t)nsys * n);
     const int coltiles = (nsys + kGemmSys - 1) / kGemmSys;
     const int rowtiles = (n + kGemmRows - 1) / kGemmRows;
     // K steps of 4 columns over ksplit workgroups x 4 waves, in multiples of
@@ -554,9 +553,11 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
             k_gemm_reduce<<<red_blocks, 256, 0, st>>>(n, rowtiles * kGemmRows, coltiles, ksplit, part, d_sys_list,
                                                       nsys, o);
         timer_end(c, t_gemm, 16);
-        c->stats.v[17] += 1;
-        c->stats.v[18] += 4.0 * n * (double)n * nsys;
-        c->stats.v[22] += (double)n * n * (fp32 ? 4.0 : 8.0) + 32.0 * n * (double)nsys;
+        if (!c->capturing) {  // replays of a captured step are neither timed nor counted here
+            c->stats.v[17] += 1;
+            c->stats.v[18] += 4.0 * n * (double)n * nsys;
+            c->stats.v[22] += (double)n * n * (fp32 ? 4.0 : 8.0) + 32.0 * n * (double)nsys;
+        }
     };
     if (M.variant == PXH_DPCM) {
         // y = D x + lambda x
@@ -569,11 +570,12 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
         gemm(M.SA.p, M.SAf.p, x, 0, GemmOut{nullptr, nullptr, 0, 0, 1.0, y, ld, 0});
     } else {
         // u = S A^-1 x, then y = -(D u + lambda u)
+        c->work_t.alloc((size_t)nsys * n);
         gemm(M.SA.p, M.SAf.p, x, 0, GemmOut{nullptr, nullptr, 0, 0, 1.0, c->work_t.p, n, 1});
         gemm(M.D.p, M.Df.p, c->work_t.p, 1, GemmOut{d_lambda, c->work_t.p, n, 1, -1.0, y, ld, 0});
     }
     PXH_HIP(hipGetLastError());
-    c->stats.v[6] += (double)n * n * nsys;
+    if (!c->capturing) c->stats.v[6] += (double)n * n * nsys;
 }
 
 void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs) {

@@ -144,6 +144,7 @@ struct Options {
     int res_wait = 5;        // ... for at most this many steps
     int autotune = 1;        // time the candidate launch shapes of a new size once per process
     int x0_projection = 8;   // batched solver: start from the best x in the span of this many converged neighbours
+    int graphs = 1;          // multi-shift GMRES on BEM: Arnoldi steps replayed from a captured HIP graph
 };
 
 struct Stats {
@@ -192,6 +193,7 @@ struct pxh_ctx {
     void *blas = nullptr;                     // rocBLAS handle of the dense path (created on first use)
     void *solver_ws = nullptr;                // work buffers of the iterative solvers (solver.hip, grow-only)
     bool use_stored = false;                  // in-memory method: products use the stored T
+    bool capturing = false;                   // a HIP graph is being captured on the stream: no timers, no launch stats
     // IEF operator form used by the products: 0 the system matrix
     // -(D + lambda I) S; 1 the right-preconditioned -(D + lambda I) (unknown
     // u = S q); 2 the matrix S alone

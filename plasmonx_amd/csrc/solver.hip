@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
 #include <type_traits>
@@ -565,10 +566,31 @@ struct Batched {
 // scales with the number of systems: at a few ms per spectrum (BEM
 // multi-shift) a round of hipMalloc / hipHostMalloc per call is a visible
 // share of the time.
+constexpr size_t kKeepSolveBytes = (size_t)256 << 20;
+
+// A captured multi-shift Arnoldi step and the launch parameters it was
+// captured with (sizes and device / pinned addresses): replayed only while
+// they are all the same.
+struct MsGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct SolverWs {
     Batched b;
     DevBuf<double2> V, Wb, Hs, gq, snq, colq, sig;  // multi-shift
     DevBuf<double> csq, msres, mspart, mstres;
+    std::vector<MsGraph> ms_graphs;  // one per number of active fields
+    // per-call buffers of solve_spectrum kept between calls when together
+    // under kKeepSolveBytes (each hipMalloc / hipFree pair of a 100 MB
+    // solution block costs a C3-sized spectrum ~0.3 ms)
+    DevBuf<double2> rhs_keep, sol_keep, part_keep;
+    DevBuf<double4> bpos_keep;
+    DevBuf<double> barea_keep;
+    ~SolverWs() {
+        for (MsGraph &g : ms_graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 SolverWs &solver_ws(pxh_ctx *c) {
@@ -1085,6 +1107,19 @@ __global__ void k_ms_store_col(const int *__restrict__ slots, const int *__restr
     double2 *Hc = Hs + ldH_f * f + (size_t)ldH * j;
     for (int i = threadIdx.x; i <= j; i += blockDim.x) Hc[i] = hbuf[(size_t)f * ldh + i];
     if (threadIdx.x == 0) Hc[j + 1] = make_double2(nrm[f], 0.0);
+}
+
+// End of an Arnoldi step: jcount[s] += 1 and jp1[s] = jcount[s] + 1 for the
+// listed slots.  The step's counters live on the device, so that one step
+// captured as a HIP graph can be replayed for the next ones.
+__global__ void k_ms_advance(int na, const int *__restrict__ slots, int *__restrict__ jcount,
+                             int *__restrict__ jp1) {
+    for (int k = threadIdx.x; k < na; k += blockDim.x) {
+        const int s = slots[k];
+        const int j = jcount[s] + 1;
+        jcount[s] = j;
+        jp1[s] = j + 1;
+    }
 }
 
 // g[q] = (beta_f, 0, ...) for every shift of the listed right-hand sides.
@@ -1994,20 +2029,18 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         k_ms_init<<<dim3(nsh, (int)act.size()), 64, 0, st>>>(nsh, S.slots_d.p, S.nrm.p, gq.p, ldg);
         for (size_t k = 0; k < act.size(); ++k) c->stats.v[7] += 1;
     }
-    while (!act.empty()) {
-        const int na = (int)act.size();
-        upload_list(S, act);
-        upload(S, S.jcount_d.p, S.jcount.data(), (size_t)B);
+    // One Arnoldi step of the active fields: every launch reads the step
+    // counters (jcount, jp1) from the device and advances them at the end, and
+    // the per-shift residuals and norms go to pinned memory, so the same
+    // sequence can be captured once as a HIP graph (BEM operator: kernels and
+    // read-backs only, the work buffers already sized by an eager step) and
+    // replayed while the active set is unchanged.  jmax bounds the Gram-Schmidt
+    // grids: the current maximum when launched directly, m when captured.
+    auto step = [&](int na, int jmax) {
         k_stage<double2><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot, nullptr,
                                                        S.stage.p);
         apply_op(S, na, true, S.stage.p, S.W.p);
         k_save_col<<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, S.W.p, Wb.p, S.ldv_slot);
-        c->stats.v[2] += 1;
-        c->stats.v[0] += na;
-        for (int f : act) S.h_jp1[f] = S.jcount[f] + 1;
-        upload(S, S.jp1_d.p, S.h_jp1.data(), (size_t)B);
-        int jmax = 0;
-        for (int f : act) jmax = std::max(jmax, S.jcount[f]);
         // classical Gram-Schmidt, two passes
         const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDotGroup - 1) / kDotGroup);
         const dim3 gs((jmax + 1 + 3) / 4, na);
@@ -2030,9 +2063,88 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
         k_ms_store_col<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hbuf.p, ldh, S.nrm.p, Hs.p, ldH, ldH_f);
         k_ms_givens<<<dim3(nsh, na), 64, 0, st>>>(nsh, S.slots_d.p, S.jcount_d.p, Hs.p, ldH, ldH_f, sig.p,
                                                            csq.p, snq.p, gq.p, ldg, msres.p);
+        k_ms_advance<<<1, 64, 0, st>>>(na, S.slots_d.p, S.jcount_d.p, S.jp1_d.p);
         PXH_HIP(hipGetLastError());
         PXH_HIP(hipMemcpyAsync(pin.p, msres.p, sizeof(double) * nsh * F, hipMemcpyDeviceToHost, st));
         PXH_HIP(hipMemcpyAsync(pin.p + (size_t)nsh * F, S.nrm.p, sizeof(double) * F, hipMemcpyDeviceToHost, st));
+    };
+    // Everything the captured step takes as launch parameters (the step above
+    // and apply_bem): a cached graph is replayed only when all of it matches.
+    const Mesh &Msh = c->mesh;
+    auto graph_key = [&](int na) {
+        std::vector<uintptr_t> k{(uintptr_t)na, (uintptr_t)nv, (uintptr_t)nch, (uintptr_t)nb, (uintptr_t)m,
+                                 (uintptr_t)S.ldv_slot, (uintptr_t)ldh, (uintptr_t)ldH, (uintptr_t)ldH_f,
+                                 (uintptr_t)nsh, (uintptr_t)ldg, (uintptr_t)F, (uintptr_t)Msh.n,
+                                 (uintptr_t)Msh.variant, (uintptr_t)c->bem_op};
+        for (const void *p : {(const void *)S.slots_d.p, (const void *)S.jcount_d.p, (const void *)S.jp1_d.p,
+                              (const void *)V.p, (const void *)S.stage.p, (const void *)S.lam.p, (const void *)S.W.p,
+                              (const void *)Msh.D.p, (const void *)Msh.SA.p, (const void *)c->work_t.p,
+                              (const void *)c->work_pack.p, (const void *)c->work_gemm.p, (const void *)Wb.p,
+                              (const void *)S.dpart.p, (const void *)S.wpart.p, (const void *)S.hbuf.p,
+                              (const void *)S.wn2.p, (const void *)S.npart.p, (const void *)S.jc2_d.p,
+                              (const void *)S.nrm.p, (const void *)S.hacc.p, (const void *)Hs.p, (const void *)sig.p,
+                              (const void *)csq.p, (const void *)snq.p, (const void *)gq.p, (const void *)msres.p,
+                              (const void *)pin.p})
+            k.push_back((uintptr_t)p);
+        return k;
+    };
+    hipGraphExec_t gexec = nullptr;  // the captured step for the current number of active fields
+    int eager_steps = 0;             // steps launched directly since the active set last changed
+    bool list_dirty = true;
+    while (!act.empty()) {
+        const int na = (int)act.size();
+        if (list_dirty) {  // the active set changed: its list and counters to the device
+            upload_list(S, act);
+            for (int f = 0; f < B; ++f) S.h_jp1[f] = S.jcount[f] + 1;
+            upload(S, S.jcount_d.p, S.jcount.data(), (size_t)B);
+            upload(S, S.jp1_d.p, S.h_jp1.data(), (size_t)B);
+            list_dirty = false;
+            eager_steps = 0;
+            gexec = nullptr;
+            if (bem && c->opt.graphs) {
+                if ((int)ws.ms_graphs.size() <= na) ws.ms_graphs.resize(na + 1);
+                MsGraph &g = ws.ms_graphs[na];
+                if (g.exec && g.key == graph_key(na)) gexec = g.exec;
+            }
+        }
+        if (!gexec && bem && c->opt.graphs && eager_steps >= 1) {
+            // capture one step (relaxed mode: nothing in it allocates, the
+            // eager step before it has sized every buffer)
+            MsGraph &g = ws.ms_graphs[na];
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            g.key.clear();
+            PXH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            c->capturing = true;
+            hipGraph_t graph = nullptr;
+            try {
+                step(na, m);
+            } catch (...) {
+                c->capturing = false;
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            c->capturing = false;
+            PXH_HIP(hipStreamEndCapture(st, &graph));
+            const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) g.exec = nullptr;
+            PXH_CHECK(ie == hipSuccess, PXH_EHIP, "hipGraphInstantiate of the multi-shift step failed");
+            g.key = graph_key(na);
+            gexec = g.exec;
+        }
+        if (gexec) {
+            PXH_HIP(hipGraphLaunch(gexec, st));
+            if (bem) c->stats.v[6] += (double)c->mesh.n * c->mesh.n * na;
+        } else {
+            int jmax = 0;
+            for (int f : act) jmax = std::max(jmax, S.jcount[f]);
+            step(na, jmax);
+            ++eager_steps;
+        }
+        c->stats.v[2] += 1;
+        c->stats.v[0] += na;
         PXH_HIP(hipStreamSynchronize(st));
         std::vector<int> fin, keep;
         for (int f : act) {
@@ -2072,6 +2184,7 @@ static int run_multishift(pxh_ctx *c, int nv, const std::vector<System> &systems
                 t_done[k] = t;
             }
         }
+        if (!fin.empty()) list_dirty = true;  // the finishes used the slot list for one field each
         act = keep;
     }
     // FP64 true residuals (k_ms_finish), one readback
@@ -2132,6 +2245,21 @@ __global__ void k_polar_partial(int n, int nv, int fmu, int bem_dpcm, const doub
     }
     if (tid == 0)
         for (int k = 0; k < 9; ++k) out[(size_t)blockIdx.x * 9 + k] = sh[k][0];
+}
+
+// sum[f][k] = the k_polar_partial block sums of frequency f added in block
+// order (on the device, so that only 9 values per frequency are read back)
+__global__ void k_polar_sum(int nb, int n_freq, const double2 *__restrict__ part, double2 *__restrict__ sum) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * n_freq) return;
+    const int f = t / 9, k = t % 9;
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double2 v = part[((size_t)nb * f + b) * 9 + k];
+        re += v.x;
+        im += v.y;
+    }
+    sum[t] = make_double2(re, im);
 }
 
 }  // namespace pxh
@@ -2233,13 +2361,18 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     if (!bem && c->atoms.model == PXH_WFQFMU)
         PXH_CHECK(c->alpha_nf >= n_freq, PXH_EINVAL, "alpha(omega) table shorter than the frequency list");
     const size_t stride = (size_t)nv * 3;
-    DevBuf<double2> rhs_all, sol;
     // right-hand sides: one per frequency for alloys (H-bar(omega) enters
     // them), else one shared by all frequencies (frequency stride 0)
     const bool rhs_per_freq = !bem && c->atoms.hetero;
     const size_t rhs_fstride = rhs_per_freq ? stride : 0;
-    rhs_all.alloc(rhs_per_freq ? stride * n_freq : stride);
-    sol.alloc(stride * n_freq);
+    const size_t rhs_n = rhs_per_freq ? stride * n_freq : stride, sol_n = stride * n_freq;
+    SolverWs &ws = solver_ws(c);
+    const bool keep = (rhs_n + sol_n) * sizeof(double2) <= kKeepSolveBytes;
+    DevBuf<double2> rhs_local, sol_local;
+    DevBuf<double2> &rhs_all = keep ? ws.rhs_keep : rhs_local;
+    DevBuf<double2> &sol = keep ? ws.sol_keep : sol_local;
+    rhs_all.alloc(rhs_n);
+    sol.alloc(sol_n);
     // starting iterates: the caller's variables with the "initial_guess"
     // option on the iterative methods (else x0 = 0, as the reference)
     const bool x0_given = c->opt.initial_guess && variables && method != PXH_INVERSION;
@@ -2345,12 +2478,12 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     }
     // polarizabilities and results
     const int nb = 64;
-    DevBuf<double2> part;
-    part.alloc((size_t)nb * 9 * n_freq);
+    DevBuf<double2> &part = ws.part_keep;
+    part.alloc((size_t)nb * 9 * n_freq + (size_t)9 * n_freq);
     const double4 *posp = nullptr;
     const double *area = nullptr;
-    DevBuf<double4> bpos;
-    DevBuf<double> barea;
+    DevBuf<double4> &bpos = ws.bpos_keep;
+    DevBuf<double> &barea = ws.barea_keep;
     if (bem) {
         std::vector<double4> hp(nv);
         for (int i = 0; i < nv; ++i)
@@ -2366,9 +2499,11 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     const int fmu = (!bem && c->atoms.model == PXH_WFQFMU) ? 1 : 0;
     const int dpcm = (bem && c->mesh.variant == PXH_DPCM) ? 1 : 0;
     k_polar_partial<<<dim3(nb, n_freq), RB, 0, st>>>(n_sites, nv, fmu, dpcm, posp, area, sol.p, stride, part.p);
+    double2 *psum = part.p + (size_t)nb * 9 * n_freq;
+    k_polar_sum<<<(9 * n_freq + 255) / 256, 256, 0, st>>>(nb, n_freq, part.p, psum);
     PXH_HIP(hipGetLastError());
-    std::vector<double2> hp((size_t)nb * 9 * n_freq);
-    PXH_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, st));
+    std::vector<double2> hp((size_t)9 * n_freq);
+    PXH_HIP(hipMemcpyAsync(hp.data(), psum, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost, st));
     if (variables)
         PXH_HIP(hipMemcpyAsync(variables, sol.p, sizeof(double2) * stride * n_freq, hipMemcpyDeviceToHost, st));
     PXH_HIP(hipStreamSynchronize(st));
@@ -2376,13 +2511,8 @@ int solve_spectrum(pxh_ctx *c, int n_freq, const double *freq, double e0, int rh
     for (int f = 0; f < n_freq; ++f) {
         double pol[18];
         for (int k = 0; k < 9; ++k) {
-            double re = 0, im = 0;
-            for (int b = 0; b < nb; ++b) {
-                re += hp[((size_t)nb * f + b) * 9 + k].x;
-                im += hp[((size_t)nb * f + b) * 9 + k].y;
-            }
-            pol[2 * k] = re / e0;
-            pol[2 * k + 1] = im / e0;
+            pol[2 * k] = hp[(size_t)9 * f + k].x / e0;
+            pol[2 * k + 1] = hp[(size_t)9 * f + k].y / e0;
         }
         if (polar) std::memcpy(polar + 18 * (size_t)f, pol, sizeof(pol));
         if (results) {
