@@ -175,13 +175,20 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
         // amplify an FP32 sum's rounding (FP32 and FP64 FMAs issue at the same
         // rate here, packed FP32 being off)
         using QAcc = typename std::conditional<FMU, R, double>::type;
+        // FP64 elements: one accumulation level (the tile's sums start from
+        // and return to the running totals, in the same registers)
+        constexpr bool kOneLevel = std::is_same<R, double>::value;
         QAcc tq_re[KV], tq_im[KV];
         R tm_re[KV][3], tm_im[KV][3];
 #pragma unroll
         for (int v = 0; v < KV; ++v) {
-            tq_re[v] = tq_im[v] = (QAcc)0;
+            tq_re[v] = kOneLevel ? (QAcc)aq_re[v] : (QAcc)0;
+            tq_im[v] = kOneLevel ? (QAcc)aq_im[v] : (QAcc)0;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) tm_re[v][a] = tm_im[v][a] = (R)0;
+            for (int a = 0; a < 3; ++a) {
+                tm_re[v][a] = kOneLevel ? (R)am_re[v][a] : (R)0;
+                tm_im[v][a] = kOneLevel ? (R)am_im[v][a] : (R)0;
+            }
         }
         if (active) {
             for (int jj = 0; jj < cnt; ++jj) {
@@ -241,29 +248,30 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
                 for (int v = 0; v < KV; ++v) {
                     const Cx<R> q = tile.q[v][jj];
                     const Cx<R> mx = tile.m[v][0][jj], my = tile.m[v][1][jj], mz = tile.m[v][2][jj];
-                    const R dm_re = dx * mx.re + dy * my.re + dz * mz.re;
-                    const R dm_im = dx * mx.im + dy * my.im + dz * mz.im;
-                    tq_re[v] += tqq * q.re + g1 * dm_re;
-                    tq_im[v] += tqq * q.im + g1 * dm_im;
-                    const R s_re = c1 * dm_re - g2 * q.re;
-                    const R s_im = c1 * dm_im - g2 * q.im;
-                    tm_re[v][0] += s_re * dx + c0 * mx.re;
-                    tm_im[v][0] += s_im * dx + c0 * mx.im;
-                    tm_re[v][1] += s_re * dy + c0 * my.re;
-                    tm_im[v][1] += s_im * dy + c0 * my.im;
-                    tm_re[v][2] += s_re * dz + c0 * mz.re;
-                    tm_im[v][2] += s_im * dz + c0 * mz.im;
+                    const R dm_re = fma(dz, mz.re, fma(dy, my.re, dx * mx.re));
+                    const R dm_im = fma(dz, mz.im, fma(dy, my.im, dx * mx.im));
+                    // every term into the running sums by one FMA each
+                    tq_re[v] = fma(g1, dm_re, fma(tqq, q.re, tq_re[v]));
+                    tq_im[v] = fma(g1, dm_im, fma(tqq, q.im, tq_im[v]));
+                    const R s_re = fma(c1, dm_re, -g2 * q.re);
+                    const R s_im = fma(c1, dm_im, -g2 * q.im);
+                    tm_re[v][0] = fma(c0, mx.re, fma(s_re, dx, tm_re[v][0]));
+                    tm_im[v][0] = fma(c0, mx.im, fma(s_im, dx, tm_im[v][0]));
+                    tm_re[v][1] = fma(c0, my.re, fma(s_re, dy, tm_re[v][1]));
+                    tm_im[v][1] = fma(c0, my.im, fma(s_im, dy, tm_im[v][1]));
+                    tm_re[v][2] = fma(c0, mz.re, fma(s_re, dz, tm_re[v][2]));
+                    tm_im[v][2] = fma(c0, mz.im, fma(s_im, dz, tm_im[v][2]));
                 }
             }
         }
 #pragma unroll
         for (int v = 0; v < KV; ++v) {
-            aq_re[v] += (double)tq_re[v];
-            aq_im[v] += (double)tq_im[v];
+            aq_re[v] = kOneLevel ? (double)tq_re[v] : aq_re[v] + (double)tq_re[v];
+            aq_im[v] = kOneLevel ? (double)tq_im[v] : aq_im[v] + (double)tq_im[v];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                am_re[v][a] += (double)tm_re[v][a];
-                am_im[v][a] += (double)tm_im[v][a];
+                am_re[v][a] = kOneLevel ? (double)tm_re[v][a] : am_re[v][a] + (double)tm_re[v][a];
+                am_im[v][a] = kOneLevel ? (double)tm_im[v][a] : am_im[v][a] + (double)tm_im[v][a];
             }
         }
     }
