@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--batch", type=int, default=48)
     ap.add_argument("--krylov", type=int, default=120)
     ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--splits", type=int, default=0, help="c2: source splits of the pair kernel (0: tuned)")
+    ap.add_argument("--mfma-levels", type=int, default=0, help="c2: FP32 accumulation levels of the pair kernel "
+                                                               "(0: the library's default)")
     ap.add_argument("--check", type=int, default=21, help="frequencies checked against FP64 (plus the peak)")
     ap.add_argument("--plain-check", type=int, default=3, help="frequencies solved by the plain FP64 path")
     ap.add_argument("--no-legs", action="store_true", help="C2 alone, without the C1 / C3 / C5 legs")
@@ -114,6 +117,10 @@ def run_c2(a):
     ctx.set_atoms(s)
     ctx.set_alpha(aw)
     ctx.set_option("batch", a.batch)
+    if a.splits > 0:
+        ctx.set_option("splits", a.splits)
+    if a.mfma_levels > 0:
+        ctx.set_option("mfma_levels", a.mfma_levels)
     if a.traffic:
         traffic_run(ctx, s, freq)
         return None
@@ -185,7 +192,8 @@ def run_c2(a):
             "frac": round(achieved / FP16_MFMA_PEAK_TFLOPS, 4), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "kernel": "k_tsweep_mfma<NCB> (fp16 hi/lo split, 3 MFMA products per element pair; NCB = 16-system "
                       "column blocks occupied, 3 for a full 48-system pass)",
-            "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 3)}
+            "launches": int(launches), "avg_launch_ms": round(ms / max(launches, 1), 3),
+            "source_splits": int(st["mfma_splits"])}
     service = {"p50_ms": round(st["latency_p50_ms"], 1), "p99_ms": round(st["latency_p99_ms"], 1),
                "definition": "per frequency point: admission of its first field to convergence of its last, "
                              "201 points queued at t=0 into 48 solver slots (not a request latency)"}
@@ -385,6 +393,13 @@ def leg_c3(a, which=(("sphere_dpcm", "dpcm", 0),)):
         out = ctx.solve_spectrum(m.n, freq, 1e-4, method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=200,
                                  precision=_lib.MIXED)
         dt = time.perf_counter() - t
+        # the timed run replays the Arnoldi step from a HIP graph, where no
+        # per-launch event can be recorded: the products are timed in one more
+        # spectrum with direct launches
+        ctx.set_option("graphs", 0)
+        ctx.solve_spectrum(m.n, freq, 1e-4, method=_lib.ITERATIVE, tol_abs=tol, krylov_dim=200,
+                           precision=_lib.MIXED)
+        ctx.set_option("graphs", 1)
         st = ctx.stats()
         gm_ms += st["bem_gemm_ms"]
         gm_n += int(st["bem_gemm_launches"])
@@ -428,7 +443,8 @@ def leg_c3(a, which=(("sphere_dpcm", "dpcm", 0),)):
                 "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4)}
     roof.update({"traffic": None, "kernel": "k_bem_gemm<double> (+ split-K reduction): the stored D applied to the "
                                            "3 field directions' Arnoldi vectors (4 flop per element and system, "
-                                           "D read once per launch); HIP events per launch",
+                                           "D read once per launch); HIP events per launch, in one spectrum "
+                                           "run with direct launches after the timed (graph-replayed) run",
                  "launches": gm_n, "avg_launch_ms": round(gm_ms / max(gm_n, 1), 4),
                  "flop_per_byte": round(ai, 3)})
     return {"metric": "freq-point solves/s (C3 BEM, 400 w x 3 fields)", "value": round(total_pts / total_t, 3),

@@ -267,15 +267,16 @@ class Context:
         return r
 
     def stats(self):
-        s = np.zeros(24)
-        self._check(self.lib.pxh_get_stats(self._h, _ptr(s), 24))
+        s = np.zeros(32)
+        self._check(self.lib.pxh_get_stats(self._h, _ptr(s), 32))
         return {"matvecs": s[0], "fp32_products": s[1], "fp64_products": s[2], "sweep_ms_fp32": s[3],
                 "sweep_ms_fp64": s[4], "wall_ms": s[5], "pair_evaluations": s[6], "outer": s[7],
                 "sweep_launches_fp32": s[8], "sweep_launches_fp64": s[9], "sweep_flops_fp32": s[10],
                 "sweep_flops_fp64": s[11], "mfma_kernel_ms": s[12], "mfma_launches": s[13],
                 "latency_p50_ms": s[14], "latency_p99_ms": s[15], "bem_gemm_ms": s[16], "bem_gemm_launches": s[17],
                 "bem_gemm_flops": s[18], "lu_update_ms": s[19], "lu_update_launches": s[20],
-                "lu_update_flops": s[21], "bem_gemm_bytes": s[22]}
+                "lu_update_flops": s[21], "bem_gemm_bytes": s[22], "mfma_splits": s[23],
+                "spectral_targets": s[24], "spectral_ms": s[25], "spectral_flops": s[26]}
 
 
 def solve_group(contexts, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7):

@@ -545,6 +545,9 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.autotune = value != 0.0;
         } else if (k == "graphs") {
             c->opt.graphs = value != 0.0;
+        } else if (k == "spectral") {
+            PXH_CHECK(value == 0 || value == 1 || value == 2, PXH_EINVAL, "spectral must be 0, 1 or 2");
+            c->opt.spectral = (int)value;
         } else if (k == "dcgs2") {
             c->opt.dcgs2 = value != 0.0;
         } else if (k == "initial_guess") {
@@ -555,6 +558,9 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.matvec_mixed = value != 0.0;
         } else if (k == "mfma") {
             c->opt.mfma = value != 0.0;
+        } else if (k == "mfma_levels") {
+            PXH_CHECK(value == 1 || value == 2, PXH_EINVAL, "mfma_levels must be 1 or 2");
+            c->opt.mfma_levels = (int)value;
         } else if (k == "precond") {
             c->opt.precond = value != 0.0;
         } else if (k == "max_outer") {

@@ -866,6 +866,8 @@ __global__ void k_dense_matvec(int n, const double2 *__restrict__ A, size_t lda_
     y[(size_t)ld * s + r] = acc;
 }
 
+}  // namespace
+
 // The context's rocBLAS handle (created on first use, bound to the context's
 // stream; atomics off, so that every result is reproducible bit for bit).
 rocblas_handle blas_handle(pxh_ctx *c) {
@@ -880,6 +882,8 @@ rocblas_handle blas_handle(pxh_ctx *c) {
     PXH_CHECK(rocblas_set_stream(h, c->stream) == rocblas_status_success, PXH_EHIP, "rocblas_set_stream failed");
     return h;
 }
+
+namespace {
 
 // Right-looking blocked LU of nf matrices with partial pivoting.  Panels go in
 // pairs where a trailing part follows: the second panel's columns take the
@@ -1051,6 +1055,17 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     hipStream_t st = c->stream;
     const bool bem = c->target == PXH_BEM;
     const int n = bem ? c->mesh.n : c->atoms.n_var();
+    // homogeneous omegaFQ: one eigendecomposition for every frequency when it
+    // is estimated faster than the LUs (the LU below when T is not positive
+    // definite)
+    if (!bem && rhs_fstride == 0 && spectral_applicable(c) && spectral_pays(c, n, n_freq)) {
+        const bool ok = spectral_decompose(c, d_rhs_all, pol_mask);
+        if (ok) {
+            spectral_charges(c, n_freq, freq, d_sol, (size_t)rhs_stride);
+            c->spectral.release();
+            return PXH_OK;
+        }
+    }
     DevBuf<double> Mtmp, T;
     const double *Mp = nullptr;
     if (!bem) {
@@ -1282,8 +1297,47 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
         dynamic_rhs_atoms(c, freq[0], e0, rhs_form, rhs[t].p);
         PXH_HIP(hipStreamSynchronize(c->stream));
     }
-    std::vector<int> order(n_ctx);
-    for (int t = 0; t < n_ctx; ++t) order[t] = t;
+    // homogeneous omegaFQ targets for which one eigendecomposition is
+    // estimated faster than their LUs: solved alone by the spectral route,
+    // each on its own context (stream and rocBLAS handle)
+    std::vector<int> order;
+    {
+        DevBuf<double2> pol;
+        std::vector<double2> hp;
+        for (int t = 0; t < n_ctx; ++t) {
+            pxh_ctx *c = ctxs[t];
+            if (spectral_applicable(c) && spectral_pays(c, c->atoms.n_var(), n_freq) &&
+                spectral_decompose(c, rhs[t].p, pol_mask)) {
+                pol.alloc((size_t)9 * n_freq);
+                hp.resize((size_t)9 * n_freq);
+                spectral_polar(c, n_freq, freq, pol.p);
+                PXH_HIP(hipMemcpy(hp.data(), pol.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost));
+                c->spectral.release();
+                timer_drain(c);
+                if (c != c0)
+                    for (int k = 24; k < 27; ++k) {
+                        c0->stats.v[k] += c->stats.v[k];
+                        c->stats.v[k] = 0.0;
+                    }
+                for (int w = 0; w < n_freq; ++w) {
+                    double pl[18];
+                    for (int e = 0; e < 9; ++e) {
+                        pl[2 * e] = hp[(size_t)9 * w + e].x / e0;
+                        pl[2 * e + 1] = hp[(size_t)9 * w + e].y / e0;
+                    }
+                    if (polar) std::memcpy(polar + 18 * ((size_t)t * n_freq + w), pl, sizeof pl);
+                    if (results) {
+                        double r[20];
+                        results_row(freq[w], pl, r);
+                        double *res_t = results + (size_t)20 * n_freq * t;  // n_freq x 20, column-major
+                        for (int e = 0; e < 20; ++e) res_t[w + (size_t)n_freq * e] = r[e];
+                    }
+                }
+            } else {
+                order.push_back(t);
+            }
+        }
+    }
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return ctxs[a]->atoms.n_var() < ctxs[b]->atoms.n_var(); });
     DevBuf<double2> A, Z, X, pol;

@@ -293,7 +293,10 @@ __global__ __launch_bounds__(TB) void k_tsweep(int n, int nt, int hetero, const 
 // t[row][k] = sum_split part[split][row][k] + (q rows) eta_i x_q(i): the
 // product intermediates are row-major (all systems of a row adjacent), so
 // that the sparse K-bar stage gathers whole rows.
-__global__ void k_reduce_splits(int n, int nv, int nsys, int splits, const double2 *__restrict__ part,
+// The partials are FP64 (FP64 products) or FP32 (the MFMA products, whose
+// FP32 accumulators they hold); the sum is FP64 either way.
+template <typename P>
+__global__ void k_reduce_splits(int n, int nv, int nsys, int splits, const P *__restrict__ part,
                                 const int *__restrict__ sys_list, const double2 *__restrict__ x, int ld,
                                 const double4 *__restrict__ pos, const double *__restrict__ eta,
                                 double2 *__restrict__ t) {
@@ -302,9 +305,9 @@ __global__ void k_reduce_splits(int n, int nv, int nsys, int splits, const doubl
     const int row = (int)(g / nsys), k = (int)(g % nsys);
     double re = 0.0, im = 0.0;
     for (int s = 0; s < splits; ++s) {
-        const double2 p = part[(size_t)s * nv * nsys + g];
-        re += p.x;
-        im += p.y;
+        const P p = part[(size_t)s * nv * nsys + g];
+        re += (double)p.x;
+        im += (double)p.y;
     }
     if (row < n) {
         const double e = eta[(int)pos[row].w];
@@ -704,14 +707,20 @@ int choose_splits(const pxh_ctx *c, int n, int groups) {
 
 // Second stage of every product: t = sum of the split partials (+ eta x_q),
 // then y_q = K-bar t_q - z_q x_q and y_mu = t_mu + Lambda x_mu.
-// part: the split partials [split][row][system]; tt: room for t [row][system].
+// part: the split partials [split][row][system], FP64 or (part_f32) FP32
+// pairs; tt: room for t [row][system].
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits, bool fp32_class, double2 *part, double2 *tt) {
+                    int ld, int splits, bool fp32_class, double2 *part, double2 *tt, bool part_f32) {
     Atoms &A = c->atoms;
     const int n = A.n, nv = A.n_var();
     const size_t tot = (size_t)nv * nsys;
-    k_reduce_splits<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(n, nv, nsys, splits, part, d_sys_list, x,
-                                                                        ld, A.pos.p, A.d_eta.p, tt);
+    if (part_f32)
+        k_reduce_splits<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(
+            n, nv, nsys, splits, reinterpret_cast<const float2 *>(part), d_sys_list, x, ld, A.pos.p, A.d_eta.p, tt);
+    else
+        k_reduce_splits<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(n, nv, nsys, splits,
+                                                                            (const double2 *)part, d_sys_list, x, ld,
+                                                                            A.pos.p, A.d_eta.p, tt);
     const int nblk = (n + 3) / 4;
     const bool loose = fp32_class && A.kbm_built;
     k_kbar_q<<<8 * ((nblk + 7) / 8), 256, 0, c->stream>>>(
@@ -797,7 +806,7 @@ void apply_atoms_impl(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     // flops: per ordered pair (i != j) kPairEvalFlops for the interaction
     // elements plus kPairVecFlops per complex vector for the contraction
     timer_end(c, timer, f32 ? 3 : 4);
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, f32, part, tt);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits_eff, f32, part, tt, false);
     const double pairs = (double)n * (double)(n - 1);
     const double fl = pairs * ((double)(fmu ? kPairEvalFlopsFmu : kPairEvalFlopsQ) * groups +
                                (double)(fmu ? kPairVecFlopsFmu : kPairVecFlopsQ) * nsys);
@@ -1079,7 +1088,7 @@ void apply_atoms_stored(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPa
     k_stored_t<<<dim3((nv + 255) / 256, nsys), 256, 0, c->stream>>>(nv, A.Tstore.p, nsys, d_sys_list, x, ld,
                                                                     c->work_part.p);
     PXH_HIP(hipGetLastError());
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1, false, c->work_part.p, c->work_t.p);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, 1, false, c->work_part.p, c->work_t.p, false);
 }
 
 void constant_matrix_atoms(pxh_ctx *c, double *d_m) {

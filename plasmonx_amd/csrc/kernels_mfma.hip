@@ -242,13 +242,13 @@ __device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, cons
 // Blocks are placed so that all workgroups of one source split share an XCD
 // (split = linear block id mod splits): with splits a multiple of 8 an XCD's
 // L2 holds the X fragments of its split.
-template <int NCB>
-__global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int hetero, float rfar2, float tsc,
+template <int NCB, int LV>
+__global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int nt, int hetero, float rfar2, float tsc,
                                                         const double4 *__restrict__ pos,
                                                         const PairConst *__restrict__ pc, int nsys,
                                                         const float *__restrict__ scale,
                                                         const _Float16 *__restrict__ xh,
-                                                        const _Float16 *__restrict__ xl, double2 *__restrict__ part,
+                                                        const _Float16 *__restrict__ xl, float2 *__restrict__ part,
                                                         int nv, int splits, int jsteps_per_split,
                                                         int nsys_total_in_part, int sys_offset) {
     __shared__ half8 sBh[2][4 * NCB * 64];  // [buffer][k-step][cb][lane]
@@ -357,15 +357,17 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
             }
             buf ^= 1;
         }
+        if constexpr (LV == 2) {
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb)
+                for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    acc2[rb][cb][v] += acc[rb][cb][v];
-                    acc[rb][cb][v] = 0.f;
-                }
+                    for (int v = 0; v < 16; ++v) {
+                        acc2[rb][cb][v] += acc[rb][cb][v];
+                        acc[rb][cb][v] = 0.f;
+                    }
+        }
     }
     // ---- write partials: acc2[rb][cb][v] -> target i0 + 8 (v / 4) + 4 (lane / 32) + v % 4,
     // component 2 cp + rb, column cb * 32 + lane % 32
@@ -384,9 +386,9 @@ __global__ __launch_bounds__(256, 2) void k_tsweep_mfma(int n, int nt, int heter
                 if (i >= n) continue;
                 const int c = 2 * cp + rb;
                 const int row = c == 0 ? i : n + 3 * i + c - 1;
-                double *dst = reinterpret_cast<double *>(
+                float *dst = reinterpret_cast<float *>(
                     part + ((size_t)split * nv + row) * nsys_total_in_part + sys_offset + sys);
-                dst[col & 1] = (double)acc2[rb][cb][v] * cs;
+                dst[col & 1] = (float)((double)(LV == 2 ? acc2[rb][cb][v] : acc[rb][cb][v]) * cs);
             }
     }
 }
@@ -425,15 +427,22 @@ std::map<std::pair<int, int>, int> g_tuned_splits;
 template <int K>
 void launch_sweep(pxh_ctx *c, int iblocks, int splits, int per, float rfar2, float tsc, int ns, int nsys, int goff) {
     const Atoms &A = c->atoms;
-    k_tsweep_mfma<K><<<iblocks * splits, 256, 0, c->stream>>>(
-        A.n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns, c->work_scale.p, c->work_xh.p, c->work_xl.p,
-        c->work_part.p, A.n_var(), splits, per, nsys, goff);
+    auto go = [&](auto kern) {
+        kern<<<iblocks * splits, 256, 0, c->stream>>>(A.n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns,
+                                                      c->work_scale.p, c->work_xh.p, c->work_xl.p,
+                                                      reinterpret_cast<float2 *>(c->work_part.p), A.n_var(), splits,
+                                                      per, nsys, goff);
+    };
+    if (c->opt.mfma_levels == 1)
+        go(k_tsweep_mfma<K, 1>);
+    else
+        go(k_tsweep_mfma<K, 2>);
 }
 
 }  // namespace
 
 void finish_product(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x, double2 *y,
-                    int ld, int splits, bool fp32_class, double2 *part, double2 *t);
+                    int ld, int splits, bool fp32_class, double2 *part, double2 *t, bool part_f32);
 
 namespace {
 
@@ -455,7 +464,7 @@ int tune_splits(pxh_ctx *c, const std::vector<int> &cand, int iblocks, int jstep
             const int per = (jsteps + sp - 1) / sp;
             PXH_HIP(hipEventRecord(e0, c->stream));
             launch_sweep<K>(c, iblocks, sp, per, rfar2, tsc, ns, nsys, 0);
-            finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sp, true, c->work_part.p, c->work_t.p);
+            finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sp, true, c->work_part.p, c->work_t.p, true);
             PXH_HIP(hipEventRecord(e1, c->stream));
             PXH_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -558,12 +567,13 @@ void apply_atoms_mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
     }
     PXH_HIP(hipGetLastError());
     timer_end(c, t_product, 3);
-    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true, c->work_part.p, c->work_t.p);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, splits, true, c->work_part.p, c->work_t.p, true);
     // algorithmic flops: the contraction of the 4N x 4N real T with 2 nsys real
     // columns (2 flops per element and column) plus the pair evaluation
     const double nn = (double)n * (double)(n - 1);
     c->stats.v[8] += groups;
     c->stats.v[10] += nn * 16.0 * 2.0 * 2.0 * nsys + nn * 42.0 * groups;
+    c->stats.v[23] = splits;
     c->stats.v[6] += (double)n * n * groups;
 }
 

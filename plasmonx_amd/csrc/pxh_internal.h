@@ -135,6 +135,7 @@ struct Options {
     int max_outer = 30;
     int precond = 1;         // diagonal right preconditioning in the batched path
     int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
+    int mfma_levels = 2;     // FP32 accumulation levels of the MFMA pair kernel (1 or 2)
     int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
     double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
     int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies
@@ -145,10 +146,30 @@ struct Options {
     int autotune = 1;        // time the candidate launch shapes of a new size once per process
     int x0_projection = 8;   // batched solver: start from the best x in the span of this many converged neighbours
     int graphs = 1;          // multi-shift GMRES on BEM: Arnoldi steps replayed from a captured HIP graph
+    int spectral = 1;        // homogeneous omegaFQ inversion by one eigendecomposition: 0 never, 1 when it
+                             // is estimated faster than the LU, 2 always
+};
+
+// Spectral route of the homogeneous omegaFQ inversion (spectral.hip): the
+// eigenvalues and T-orthonormal eigenvectors of K T, and the right-hand sides
+// and coordinates in that basis.
+struct SpectralWs {
+    int n = 0, mask = 0;   // n = 0: no decomposition held
+    DevBuf<double> X;      // n x n eigenvectors (X^T T X = I)
+    DevBuf<double> lam;    // n eigenvalues
+    DevBuf<double> c6;     // X^T T b: Re of the three directions, then Im (n x 6)
+    DevBuf<double> w3;     // X^T r: coordinates x, y, z (n x 3)
+    void release() {
+        X.release();
+        lam.release();
+        c6.release();
+        w3.release();
+        n = 0;
+    }
 };
 
 struct Stats {
-    static constexpr int kN = 24;
+    static constexpr int kN = 32;
     double v[kN] = {};
 };
 
@@ -190,6 +211,7 @@ struct pxh_ctx {
     pxh::DevBuf<double2> work_lu;             // dense path: the factorised matrices of one frequency chunk
     pxh::DevBuf<int> work_piv;                // their row interchanges, and the permutations built from them
     pxh::DevBuf<double2> work_panel;          // dense path: column-major copy of the current panel (tall panels)
+    pxh::SpectralWs spectral;                 // dense omegaFQ path: the eigendecomposition of the call
     void *blas = nullptr;                     // rocBLAS handle of the dense path (created on first use)
     void *solver_ws = nullptr;                // work buffers of the iterative solvers (solver.hip, grow-only)
     bool use_stored = false;                  // in-memory method: products use the stored T
@@ -265,6 +287,12 @@ void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
 // results n_ctx blocks of n_freq x 20 (column-major)
 int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq, double e0, int rhs_form,
                 int pol_mask, double *polar, double *results);
+// spectral.hip: the homogeneous omegaFQ inversion by one eigendecomposition
+bool spectral_applicable(const pxh_ctx *c);
+bool spectral_pays(const pxh_ctx *c, int n, int n_freq);
+bool spectral_decompose(pxh_ctx *c, const double2 *d_rhs /* n x 3 */, int pol_mask);
+void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *d_sol, size_t stride);
+void spectral_polar(pxh_ctx *c, int n_freq, const double *freq, double2 *d_out /* 9 per frequency */);
 // solver.hip: the 20 result columns of one frequency from its 3 x 3 complex polarizability
 void results_row(double freq, const double *pol, double *r);
 // dense.hip: static FQ / FQFmu (what 0 ground state, 1 static field)
