@@ -482,6 +482,7 @@ def run_c4(a):
     _lib.solve_group(ctxs, freq[:2], 1e-4, pol_mask=3)
     times = []
     upd_ms = upd_flops = upd_n = 0.0
+    spec_n = spec_ms = 0.0
     blocks = np.array_split(np.arange(len(freq)), max(1, a.freq_blocks))
     for _ in range(a.steps):
         t = time.perf_counter()
@@ -492,6 +493,8 @@ def run_c4(a):
             upd_ms += st["lu_update_ms"]
             upd_flops += st["lu_update_flops"]
             upd_n += st["lu_update_launches"]
+            spec_n += st["spectral_targets"]
+            spec_ms += st["spectral_ms"]
             if len(blocks) > 1:
                 log(f"[bench] C4 frequency block {b + 1}/{len(blocks)} done ({time.perf_counter() - t:.1f}s)")
         outs = [{"polar": np.concatenate([p[k]["polar"] for p in parts]),
@@ -510,7 +513,7 @@ def run_c4(a):
     flops = sum((8.0 / 3.0) * s.n_var ** 3 for s in flakes) * len(freq)
     achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
     log(f"[bench] C4 {len(flakes)} flakes x {len(freq)} w: {pts / dt:.1f} fp/s ({dt:.2f}s), err vs alone "
-        f"{max(errs):.1e}")
+        f"{max(errs):.1e}; spectral route: {int(spec_n)} flakes, decompositions {spec_ms / 1e3:.1f}s; LU updates {upd_ms / 1e3:.1f}s")
     emit("flake-frequency solves/s (C4 graphene batch)", pts / dt, "freq-point solves/s", 1e3 * dt, a,
          {"workload": f"{len(flakes)} seeded graphene flakes (500-{a.flake_max} atoms; the full batch is 96 flakes of "
                      f"500-12k atoms), {len(freq)} w in 0.1-1.0 eV (of 128), x and y fields, grouped dense inversion "

@@ -1282,13 +1282,15 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
     // per target: the constant matrix (kept with the target) and the
     // right-hand sides, built on its own stream, then joined
     std::vector<DevBuf<double2>> rhs(n_ctx);
+    std::vector<char> by_spectral(n_ctx, 0);
     for (int t = 0; t < n_ctx; ++t) {
         pxh_ctx *c = ctxs[t];
         Atoms &A = c->atoms;
         const size_t nv = A.n_var();
-        PXH_CHECK(nv * nv * sizeof(double) <= kKeepMconstBytes, PXH_ENOMEM,
+        by_spectral[t] = spectral_applicable(c) && spectral_pays(c, (int)nv, n_freq);
+        PXH_CHECK(by_spectral[t] || nv * nv * sizeof(double) <= kKeepMconstBytes, PXH_ENOMEM,
                   "a grouped target is larger than the constant matrix kept with it (2 GB)");
-        if (!A.Mconst_valid) {
+        if (!by_spectral[t] && !A.Mconst_valid) {
             A.Mconst.alloc(nv * nv);
             constant_matrix_atoms(c, A.Mconst.p);
             A.Mconst_valid = true;
@@ -1299,42 +1301,60 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
     }
     // homogeneous omegaFQ targets for which one eigendecomposition is
     // estimated faster than their LUs: solved alone by the spectral route,
-    // each on its own context (stream and rocBLAS handle)
-    std::vector<int> order;
-    {
-        DevBuf<double2> pol;
-        std::vector<double2> hp;
-        for (int t = 0; t < n_ctx; ++t) {
-            pxh_ctx *c = ctxs[t];
-            if (spectral_applicable(c) && spectral_pays(c, c->atoms.n_var(), n_freq) &&
-                spectral_decompose(c, rhs[t].p, pol_mask)) {
-                pol.alloc((size_t)9 * n_freq);
-                hp.resize((size_t)9 * n_freq);
-                spectral_polar(c, n_freq, freq, pol.p);
-                PXH_HIP(hipMemcpy(hp.data(), pol.p, sizeof(double2) * hp.size(), hipMemcpyDeviceToHost));
-                c->spectral.release();
-                timer_drain(c);
-                if (c != c0)
-                    for (int k = 24; k < 27; ++k) {
-                        c0->stats.v[k] += c->stats.v[k];
-                        c->stats.v[k] = 0.0;
-                    }
-                for (int w = 0; w < n_freq; ++w) {
-                    double pl[18];
-                    for (int e = 0; e < 9; ++e) {
-                        pl[2 * e] = hp[(size_t)9 * w + e].x / e0;
-                        pl[2 * e + 1] = hp[(size_t)9 * w + e].y / e0;
-                    }
-                    if (polar) std::memcpy(polar + 18 * ((size_t)t * n_freq + w), pl, sizeof pl);
-                    if (results) {
-                        double r[20];
-                        results_row(freq[w], pl, r);
-                        double *res_t = results + (size_t)20 * n_freq * t;  // n_freq x 20, column-major
-                        for (int e = 0; e < 20; ++e) res_t[w + (size_t)n_freq * e] = r[e];
-                    }
-                }
-            } else {
-                order.push_back(t);
+    // each on its own context (stream and rocBLAS handle), one after another
+    // (rocSOLVER decompositions run from several host threads at once gave
+    // wrong eigenpairs on this ROCm release)
+    std::vector<int> order, spec;
+    for (int t = 0; t < n_ctx; ++t) (by_spectral[t] ? spec : order).push_back(t);
+    std::vector<std::vector<double2>> hpol(n_ctx);
+    for (int t : spec) {
+        pxh_ctx *c = ctxs[t];
+        if (spectral_decompose(c, rhs[t].p, pol_mask)) {
+            DevBuf<double2> pol;
+            pol.alloc((size_t)9 * n_freq);
+            spectral_polar(c, n_freq, freq, pol.p);
+            hpol[t].resize((size_t)9 * n_freq);
+            PXH_HIP(hipMemcpyAsync(hpol[t].data(), pol.p, sizeof(double2) * hpol[t].size(), hipMemcpyDeviceToHost,
+                                   c->stream));
+            PXH_HIP(hipStreamSynchronize(c->stream));
+        }
+        c->spectral.release();
+        timer_drain(c);
+    }
+    for (int t : spec) {
+        pxh_ctx *c = ctxs[t];
+        if (c != c0)
+            for (int k = 24; k < 27; ++k) {
+                c0->stats.v[k] += c->stats.v[k];
+                c->stats.v[k] = 0.0;
+            }
+        if (hpol[t].empty()) {  // T not positive definite: the LU groups take it
+            Atoms &A = c->atoms;
+            const size_t nv = A.n_var();
+            PXH_CHECK(nv * nv * sizeof(double) <= kKeepMconstBytes, PXH_ENOMEM,
+                      "a grouped target is larger than the constant matrix kept with it (2 GB)");
+            if (!A.Mconst_valid) {
+                A.Mconst.alloc(nv * nv);
+                constant_matrix_atoms(c, A.Mconst.p);
+                A.Mconst_valid = true;
+                PXH_HIP(hipStreamSynchronize(c->stream));
+            }
+            order.push_back(t);
+            continue;
+        }
+        const std::vector<double2> &hp = hpol[t];
+        for (int w = 0; w < n_freq; ++w) {
+            double pl[18];
+            for (int e = 0; e < 9; ++e) {
+                pl[2 * e] = hp[(size_t)9 * w + e].x / e0;
+                pl[2 * e + 1] = hp[(size_t)9 * w + e].y / e0;
+            }
+            if (polar) std::memcpy(polar + 18 * ((size_t)t * n_freq + w), pl, sizeof pl);
+            if (results) {
+                double r[20];
+                results_row(freq[w], pl, r);
+                double *res_t = results + (size_t)20 * n_freq * t;  // n_freq x 20, column-major
+                for (int e = 0; e < 20; ++e) res_t[w + (size_t)n_freq * e] = r[e];
             }
         }
     }

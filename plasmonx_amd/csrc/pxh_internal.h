@@ -150,17 +150,19 @@ struct Options {
                              // is estimated faster than the LU, 2 always
 };
 
-// Spectral route of the homogeneous omegaFQ inversion (spectral.hip): the
-// eigenvalues and T-orthonormal eigenvectors of K T, and the right-hand sides
-// and coordinates in that basis.
+// Spectral route of the homogeneous omegaFQ inversion (spectral.hip): with
+// T = L L^T, the eigenvalues and eigenvectors Q of L^T K L, and the
+// right-hand sides and coordinates in that basis.
 struct SpectralWs {
     int n = 0, mask = 0;   // n = 0: no decomposition held
-    DevBuf<double> X;      // n x n eigenvectors (X^T T X = I)
+    DevBuf<double> L;      // n x n Cholesky factor of T (lower triangle)
+    DevBuf<double> Q;      // n x n orthonormal eigenvectors of L^T K L
     DevBuf<double> lam;    // n eigenvalues
-    DevBuf<double> c6;     // X^T T b: Re of the three directions, then Im (n x 6)
-    DevBuf<double> w3;     // X^T r: coordinates x, y, z (n x 3)
+    DevBuf<double> c6;     // Q^T L^T b: Re of the three directions, then Im (n x 6)
+    DevBuf<double> w3;     // Q^T L^-1 r: coordinates x, y, z (n x 3)
     void release() {
-        X.release();
+        L.release();
+        Q.release();
         lam.release();
         c6.release();
         w3.release();

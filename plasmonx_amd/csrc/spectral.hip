@@ -5,29 +5,32 @@
 // shift z_q(w) of :239-242 on the diagonal, the same for every atom), and the
 // right-hand sides K (E0 r_k) do not depend on w (rhs_module.f90:140-186).
 // K-bar is symmetric (one type: A_ij = A_t) and T_qq symmetric positive
-// definite (Coulomb energy matrix of Gaussian charges), so K T is similar to
-// the symmetric L^T K L (T = L L^T): its eigenvalues are real and
-//     T K T x = lambda T x        (generalized symmetric-definite problem)
-// gives K T = X diag(lambda) X^-1 with X^T T X = I, i.e. X^-1 = X^T T, and
-//     x_k(w) = X diag(1 / (lambda - z(w))) X^T T b_k
+// definite (Coulomb energy matrix of Gaussian charges), so with the Cholesky
+// factor T = L L^T, K T = L^-T (L^T K L) L^T is similar to the symmetric
+//     S = L^T K L = Q diag(lambda) Q^T,
+// its eigenvalues are real, and with X = L^-T Q (X^T T X = I, X^-1 = Q^T L^T)
+//     x_k(w) = L^-T Q diag(1 / (lambda - z(w))) Q^T L^T b_k
 // for every frequency at O(n^2) each (O(n) for the polarizability), after one
 // eigendecomposition in place of one complex LU (8/3 n^3 flop) per frequency.
 // The reference factorises A(w) per frequency (ZGETRF / ZGETRS,
 // algorithm_module.f90:510-541); the solutions are the same to rounding.
 //
-// Work on the device: T and M = K T by the library's kernels, G = T M and the
-// small products by rocBLAS GEMMs, the eigendecomposition by rocSOLVER DSYGVD
-// (LAPACK semantics: Cholesky of T, reduction, divide-and-conquer
-// tridiagonal eigensolver, back-transformation), the per-frequency
-// coefficients, polarizability sums and the scatter into the solution layout
-// by the kernels below.  A T that is not positive definite (other kernels or
-// parameters) makes DSYGVD report it, and the caller falls back to the LU.
+// Work on the device: T by the library's kernel, its Cholesky factor by
+// rocSOLVER DPOTRF, P = K L by the sparse kernel below (K-bar in CSR: a few
+// neighbours per row), S = L^T P by rocBLAS DTRMM, the eigendecomposition of
+// S by rocSOLVER DSYEVD (LAPACK semantics: tridiagonal reduction,
+// divide-and-conquer, back-transformation), the small products by rocBLAS
+// GEMM / TRMM / TRSM, and the per-frequency coefficients, polarizability
+// sums and the scatter into the solution layout by the kernels below.  A T
+// that is not positive definite (other kernels or parameters) makes DPOTRF
+// report it, and the caller falls back to the LU.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "pxh_internal.h"
@@ -38,6 +41,12 @@ rocblas_handle blas_handle(pxh_ctx *c);
 
 namespace {
 
+// One decomposition at a time in the process: rocSOLVER DSYGVD called from
+// several host threads at once (each with its own rocBLAS handle and stream)
+// gave wrong eigenpairs on this ROCm release, so the request loop's workers
+// and every other caller take their turn here.
+std::mutex g_spectral_mu;
+
 // B6[:, k] = Re b_k, B6[:, 3 + k] = Im b_k for the selected directions (zero otherwise).
 __global__ void k_rhs_split(int n, int mask, const double2 *__restrict__ rhs, double *__restrict__ b6) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
@@ -45,6 +54,28 @@ __global__ void k_rhs_split(int n, int mask, const double2 *__restrict__ rhs, do
     const double2 v = ((mask >> k) & 1) ? rhs[(size_t)n * k + i] : make_double2(0.0, 0.0);
     b6[(size_t)n * k + i] = v.x;
     b6[(size_t)n * (3 + k) + i] = v.y;
+}
+
+// P = K L (n x n, column-major) for the lower-triangular L (entries above the
+// diagonal of its storage are not part of it): with K-bar in CSR (off-diagonal
+// entries, K_ii = -sum_j K_ij), P_ij = sum_k K_ik (L_kj - L_ij), the
+// difference form of the K-bar products (matrix_module.f90:1011-1098).
+// Work-items over rows i, grid rows over columns j.
+__global__ void k_kbar_lower(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                             const double *__restrict__ kij, const double *__restrict__ L, double *__restrict__ P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int z0 = rowptr[i], z1 = rowptr[i + 1];
+    for (int j = blockIdx.y; j < n; j += gridDim.y) {
+        const double *Lj = L + (size_t)n * j;
+        const double lij = i >= j ? Lj[i] : 0.0;
+        double acc = 0.0;
+        for (int z = z0; z < z1; ++z) {
+            const int k = col[z];
+            acc += kij[z] * ((k >= j ? Lj[k] : 0.0) - lij);
+        }
+        P[i + (size_t)n * j] = acc;
+    }
 }
 
 // R3[:, a] = coordinate a of the atoms.
@@ -137,6 +168,20 @@ void check_blas(rocblas_status s, const char *what) {
     PXH_CHECK(s == rocblas_status_success, PXH_EHIP, what);
 }
 
+// B = op(L) B (trmm) or B = op(L)^-1 B (trsm) for the lower-triangular L, from the left.
+void tri_left(rocblas_handle h, bool solve, bool trans, int m, int nr, const double *L, double *B) {
+    const double one = 1.0;
+    const rocblas_operation op = trans ? rocblas_operation_transpose : rocblas_operation_none;
+    if (solve)
+        check_blas(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, op, rocblas_diagonal_non_unit, m, nr, &one,
+                                 L, m, B, m),
+                   "rocblas_dtrsm failed");
+    else
+        check_blas(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, op, rocblas_diagonal_non_unit, m, nr, &one,
+                                 L, m, B, m, B, m),
+                   "rocblas_dtrmm failed");
+}
+
 // C = op(A) B for column-major real matrices (no accumulation).
 void dgemm(rocblas_handle h, bool trans_a, int m, int n, int k, const double *A, int lda, const double *B, int ldb,
            double *C, int ldc) {
@@ -168,78 +213,84 @@ bool spectral_pays(const pxh_ctx *c, int n, int n_freq) {
     return t_eig < 0.8 * t_lu;
 }
 
-size_t spectral_bytes(int n) { return ((size_t)3 * n * n + (size_t)24 * n) * sizeof(double); }
-
 // The decomposition for the right-hand sides rhs (n x 3, the selected
 // directions of mask); on success the context's spectral workspace holds
-// lambda (n), X (n x n), W = X^T r (n x 3) and C = X^T T b (n x 6: Re, Im of
-// each direction).  Returns false when T is not positive definite or the
-// eigensolver does not converge (the caller then uses the LU).
+// lambda (n), Q (n x n), L (n x n, lower), W = Q^T L^-1 r (n x 3) and
+// C = Q^T L^T b (n x 6: Re, Im of each direction).  Returns false when T is
+// not positive definite or the eigensolver does not converge (the caller
+// then uses the LU).
 bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
+    std::lock_guard<std::mutex> lk(g_spectral_mu);
     Atoms &At = c->atoms;
     const int n = At.n;
     hipStream_t st = c->stream;
     SpectralWs &S = c->spectral;
     S.n = 0;
-    const double *M = nullptr;
-    DevBuf<double> Mtmp;
-    if (!At.Mconst_valid) {
-        const bool keep = (size_t)n * n * sizeof(double) <= kKeepMconstBytes;
-        DevBuf<double> &dst = keep ? At.Mconst : Mtmp;
-        dst.alloc((size_t)n * n);
-        constant_matrix_atoms(c, dst.p);
-        At.Mconst_valid = keep;
-        M = dst.p;
-    } else {
-        M = At.Mconst.p;
-    }
-    DevBuf<double> T, B6, U, R3, E;
+    DevBuf<double> P, B6, R3, E;
     DevBuf<int> info;
-    T.alloc((size_t)n * n);
-    dense_t_atoms(c, T.p, false);
-    S.X.alloc((size_t)n * n);
+    S.L.alloc((size_t)n * n);
+    S.Q.alloc((size_t)n * n);
     S.lam.alloc(n);
     S.c6.alloc((size_t)6 * n);
     S.w3.alloc((size_t)3 * n);
+    P.alloc((size_t)n * n);
     B6.alloc((size_t)6 * n);
-    U.alloc((size_t)6 * n);
     R3.alloc((size_t)3 * n);
     E.alloc(n);
-    info.alloc(1);
+    info.alloc(2);
     rocblas_handle h = blas_handle(c);
     const int tm = timer_begin(c);
-    // G = T (K T) = T K T, symmetric (DSYGVD reads its upper triangle)
-    dgemm(h, false, n, n, n, T.p, n, M, n, S.X.p, n);
-    // U = T b, before T is overwritten by its Cholesky factor
+    dense_t_atoms(c, S.L.p, false);
     k_rhs_split<<<dim3((n + 255) / 256, 3), 256, 0, st>>>(n, mask, rhs, B6.p);
     k_pos_cols<<<(n + 255) / 256, 256, 0, st>>>(n, At.pos.p, R3.p);
     PXH_HIP(hipGetLastError());
-    dgemm(h, false, n, 6, n, T.p, n, B6.p, n, U.p, n);
-    // T K T x = lambda T x: eigenvalues ascending into lam, X (X^T T X = I) in place of G
-    check_blas(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n, S.X.p, n, T.p,
-                                n, S.lam.p, E.p, info.p),
-               "rocsolver_dsygvd failed");
-    int hinfo = 0;
-    PXH_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    check_blas(rocsolver_dpotrf(h, rocblas_fill_lower, n, S.L.p, n, info.p), "rocsolver_dpotrf failed");
+    int hinfo[2] = {0, 0};
+    PXH_HIP(hipMemcpyAsync(hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
     PXH_HIP(hipStreamSynchronize(st));
-    if (hinfo != 0) {  // > n: T not positive definite; 1..n: no convergence
+    if (hinfo[0] != 0) {  // T not positive definite
         timer_end(c, tm, 25);
+        PXH_HIP(hipStreamSynchronize(st));
         S.release();
         return false;
     }
-    // C = X^T (T b), W = X^T r
-    dgemm(h, true, n, 6, n, S.X.p, n, U.p, n, S.c6.p, n);
-    dgemm(h, true, n, 3, n, S.X.p, n, R3.p, n, S.w3.p, n);
+    // S = L^T (K L), symmetric; DSYEVD reads its lower triangle
+    k_kbar_lower<<<dim3((n + 255) / 256, std::min(n, 65535)), 256, 0, st>>>(n, At.kb_rowptr.p, At.kb_col.p,
+                                                                          At.kb_kij.p, S.L.p, P.p);
+    PXH_HIP(hipGetLastError());
+    const double one = 1.0;
+    check_blas(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, n, n, &one, S.L.p, n, P.p, n, S.Q.p, n),
+               "rocblas_dtrmm failed");
+    check_blas(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, S.Q.p, n, S.lam.p, E.p, info.p + 1),
+               "rocsolver_dsyevd failed");
+    PXH_HIP(hipMemcpyAsync(hinfo + 1, info.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipStreamSynchronize(st));
+    if (hinfo[1] != 0) {  // no convergence
+        timer_end(c, tm, 25);
+        PXH_HIP(hipStreamSynchronize(st));
+        S.release();
+        return false;
+    }
+    // C = Q^T (L^T b), W = Q^T (L^-1 r)
+    tri_left(h, false, true, n, 6, S.L.p, B6.p);
+    dgemm(h, true, n, 6, n, S.Q.p, n, B6.p, n, S.c6.p, n);
+    tri_left(h, true, false, n, 3, S.L.p, R3.p);
+    dgemm(h, true, n, 3, n, S.Q.p, n, R3.p, n, S.w3.p, n);
     timer_end(c, tm, 25);
+    PXH_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
     c->stats.v[24] += 1;
-    c->stats.v[26] += 13.0 * (double)n * n * n;  // nominal flops: G (2 n^3), DSYGVD (~11 n^3 with vectors)
+    // nominal flops: Cholesky n^3/3, S n^3, tridiagonal reduction 4/3 n^3,
+    // back-transformation of the eigenvectors 2 n^3
+    c->stats.v[26] += (1.0 / 3.0 + 1.0 + 4.0 / 3.0 + 2.0) * (double)n * n * n;
     S.n = n;
     S.mask = mask;
     return true;
 }
 
 // Solutions of every frequency into sol (n x 3 per frequency, frequency
-// stride `stride`), in chunks of frequencies: Y = coefficients, Q = X Y.
+// stride `stride`), in chunks of frequencies: Y = coefficients, then
+// x = L^-T (Q Y).
 void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *sol, size_t stride) {
     SpectralWs &S = c->spectral;
     const int n = S.n;
@@ -257,7 +308,8 @@ void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *sol, 
         const int nf = std::min(chunk, n_freq - f0), ncol = 3 * nf;
         k_spectral_coef<<<dim3((n + 255) / 256, ncol), 256, 0, st>>>(n, ncol, S.mask, S.lam.p, z.p + f0, S.c6.p,
                                                                      Y.p);
-        dgemm(h, false, n, 2 * ncol, n, S.X.p, n, Y.p, n, Q.p, n);
+        dgemm(h, false, n, 2 * ncol, n, S.Q.p, n, Y.p, n, Q.p, n);
+        tri_left(h, true, true, n, 2 * ncol, S.L.p, Q.p);  // x = L^-T (Q y)
         k_spectral_scatter<<<dim3((n + 255) / 256, ncol), 256, 0, st>>>(n, ncol, Q.p, sol + stride * f0, stride);
         PXH_HIP(hipGetLastError());
     }

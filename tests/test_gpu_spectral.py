@@ -125,3 +125,38 @@ def test_large_flake_takes_the_spectral_route_by_itself():
         lu = ctx.solve_spectrum(f.n_var, freq[pick], 1e-4, pol_mask=3, method=_lib.INVERSION)
         assert ctx.stats()["spectral_targets"] == 0
     assert relerr(full["polar"][pick], lu["polar"]) < 1e-9
+
+
+def test_spectral_rods_in_the_request_loop_match_alone():
+    # three Na rods large enough for the spectral route, served concurrently
+    # with an LU request: each result equals the one solved alone
+    from plasmonx_amd.workloads import sodium_rod
+    rods = [sodium_rod(L, 14.0) for L in (110.0, 120.0, 130.0)]
+    freq = np.linspace(2.5, 4.0, 120) / 27.211386245981
+    jobs = []
+    for r in rods:
+        c = Context(0)
+        c.set_atoms(r)
+        jobs.append((c, r, dict(freq_au=freq, e0=1e-4, pol_mask=1, method=_lib.INVERSION)))
+    small = sodium_rod(30.0, 9.0)
+    c = Context(0)
+    c.set_atoms(small)
+    jobs.append((c, small, dict(freq_au=freq[:6], e0=1e-4, pol_mask=1, method=_lib.INVERSION)))
+    print("rod atoms", [r.n_var for _, r, _ in jobs], flush=True)
+    alone, routes = [], []
+    for c, r, kw in jobs:
+        alone.append(c.solve_spectrum(r.n_var, **kw))
+        routes.append(c.stats()["spectral_targets"])
+    assert routes == [1, 1, 1, 0]
+    srv = _lib.Server(0, hbm_budget=0, max_inflight=4)
+    try:
+        ids = [srv.submit(c, **kw) for c, _, kw in jobs]
+        mixed = [srv.wait(i) for i in ids]
+    finally:
+        srv.close()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a["polar"], m["polar"])
+    ref = pxo.solve_spectrum(jobs[0][1], freq[[0, 60]], 1e-4, pol_mask=1, method=0)
+    assert relerr(alone[0]["polar"][[0, 60]], ref["polar"]) < 1e-10
+    for c, _, _ in jobs:
+        c.close()

@@ -1,10 +1,11 @@
 """The premises of the spectral route of the homogeneous omegaFQ inversion,
 on the oracle's own matrices (CPU): K-bar symmetric, T_qq symmetric positive
-definite, the eigenvalues of K-bar T real and non-positive, and
-x(w) = X diag(1 / (lambda - z(w))) X^T T b (X from the generalized symmetric
-problem T K T x = lambda T x) equal to the oracle's LU solve of
-(K T - z(w)) x = b (wfq_module.f90:214-254, algorithm_module.f90:510-541).
-Also the case the route must refuse: a Coulomb kernel whose T is indefinite."""
+definite (Cholesky T = L L^T), the eigenvalues of S = L^T K L (those of
+K-bar T) non-positive, and x(w) = L^-T Q diag(1 / (lambda - z(w))) Q^T L^T b
+(S = Q diag(lambda) Q^T) equal to the oracle's LU solve of (K T - z(w)) x = b
+(wfq_module.f90:214-254, algorithm_module.f90:510-541), the steps the library
+takes on the device.  Also the case the route must refuse: a Coulomb kernel
+whose T is indefinite."""
 import dataclasses
 
 import numpy as np
@@ -19,18 +20,18 @@ from conftest import load_fixture
 def eigen_route(s, freq, e0):
     T, K = pxo.t_qq(s), pxo.kbar(s)
     assert np.array_equal(K, K.T) and np.array_equal(T, T.T)
-    assert np.linalg.eigvalsh(T).min() > 0.0
-    G = T @ K @ T
-    lam, X = sl.eigh(0.5 * (G + G.T), T)
+    L = np.linalg.cholesky(T)
+    S = L.T @ K @ L
+    lam, Q = np.linalg.eigh(0.5 * (S + S.T))
     assert lam.max() < 1e-10 * np.abs(lam).max()  # K-bar is negative semidefinite
     b = np.asarray(pxo.rhs(s, e0, freq[0]))
     assert np.abs(b.imag).max() == 0.0
-    C = X.T @ (T @ b.real.T)
+    C = Q.T @ (L.T @ b.real.T)
     t = s.types[0]
     out = []
     for w in freq:
         z = -(w / (2.0 * t.density * t.tau)) * (w * t.tau + 1j)
-        out.append(X @ (C / (lam - z)[:, None]))
+        out.append(sl.solve_triangular(L, Q @ (C / (lam - z)[:, None]), trans="T", lower=True))
     return np.array(out)  # [freq][atom][field]
 
 
