@@ -71,8 +71,8 @@ def main():
     ap.add_argument("--c5-rod-atoms", default="", help="c5: Na rod atoms, min,max (default: as --c5-atoms)")
     ap.add_argument("--c5-tesserae", default="2000,8000", help="c5: BEM rod tesserae, min,max (full: 2000,20000)")
     ap.add_argument("--c5-freqs", default="50,100", help="c5: frequencies per request, min,max (full: 50,400)")
-    ap.add_argument("--flakes", type=int, default=16, help="c4: flakes (the full batch is 96)")
-    ap.add_argument("--flake-max", type=int, default=4000, help="c4: largest flake (the full batch reaches 12000)")
+    ap.add_argument("--flakes", type=int, default=96, help="c4: flakes (the full batch is 96)")
+    ap.add_argument("--flake-max", type=int, default=12000, help="c4: largest flake (the full batch reaches 12000)")
     ap.add_argument("--c4-freqs", type=int, default=128,
                     help="c4: solve only this many of the 128 frequencies (evenly spread; for profiling)")
     ap.add_argument("--freq-blocks", type=int, default=1,
@@ -91,7 +91,8 @@ def main():
         return
     if not a.no_legs:
         legs = {"c1": leg_c1(a), "c3": leg_c3(a), "c5": leg_c5(a, n_requests=16, rate=2.0, atoms=(1000, 6000),
-                                                               tesserae=(2000, 6000), n_freq=(50, 100))}
+                                                               tesserae=(2000, 6000), n_freq=(50, 100)),
+                "c4": leg_c4(a)}
         line["legs"] = legs
         line["latency"] = dict(legs["c5"]["latency"], source="legs.c5: request latency of a seeded Poisson stream")
     print(json.dumps(line), flush=True)
@@ -460,16 +461,20 @@ def run_c3(a):
     emit(leg.pop("metric"), leg.pop("value"), leg.pop("unit"), leg.pop("ms_per_step"), a, leg.pop("config"), **leg)
 
 
-def run_c4(a):
-    """C4: seeded graphene flakes, 128 w in 0.1-1.0 eV, x and y fields: every
-    flake's dense inversion in one grouped solve (pxh_solve_group: flakes of
-    similar size padded to a common n, their (flake, w) matrices factorised
-    by the same LU launches)."""
+def leg_c4(a, n_flakes=96, flake_max=12000, n_freq=128, freq_blocks=1, steps=1):
+    """C4: seeded graphene flakes, 128 w in 0.1-1.0 eV, x and y fields, in
+    one grouped solve (pxh_solve_group): flakes for which one
+    eigendecomposition is estimated faster than 128 LUs take the spectral
+    route (Cholesky of T, eigenvectors of L^T K L, every frequency from
+    them), the others are padded to a common n in groups and factorised by
+    the same LU launches.  Roofline: the decompositions, bound by the
+    tridiagonal reduction's reads of the trailing matrix (HBM).  CPU
+    baseline: the oracle's LU on one flake, extrapolated by n^3 to the batch."""
     from plasmonx_amd import Context, _lib
     from plasmonx_amd.workloads import c4_flakes
-    flakes, freq = c4_flakes(a.flakes, hi=a.flake_max)
-    if a.c4_freqs < len(freq):
-        freq = freq[np.linspace(0, len(freq) - 1, max(1, a.c4_freqs)).astype(int)]
+    flakes, freq = c4_flakes(n_flakes, hi=flake_max)
+    if n_freq < len(freq):
+        freq = freq[np.linspace(0, len(freq) - 1, max(1, n_freq)).astype(int)]
     log(f"[bench] C4: {len(flakes)} flakes, {sum(f.n_atoms for f in flakes)} atoms, {len(freq)} w x 2 fields")
     t = time.perf_counter()
     ctxs = []
@@ -478,13 +483,23 @@ def run_c4(a):
         c.set_atoms(s)
         ctxs.append(c)
     t_setup = time.perf_counter() - t
-    # warm-up on a few frequencies (kernels loaded, constant matrices kept with the targets)
-    _lib.solve_group(ctxs, freq[:2], 1e-4, pol_mask=3)
+    # warm-up: the LU and the spectral kernels and libraries loaded on the two smallest flakes
+    small = sorted(range(len(flakes)), key=lambda k: flakes[k].n_var)[:2]
+    warm = []
+    for k in small:
+        c = Context(0)
+        c.set_atoms(flakes[k])
+        warm.append(c)
+    _lib.solve_group(warm, freq[:2], 1e-4, pol_mask=3)
+    warm[0].set_option("spectral", 2)
+    warm[0].solve_spectrum(flakes[small[0]].n_var, freq, 1e-4, pol_mask=3, method=_lib.INVERSION)
+    for c in warm:
+        c.close()
     times = []
     upd_ms = upd_flops = upd_n = 0.0
-    spec_n = spec_ms = 0.0
-    blocks = np.array_split(np.arange(len(freq)), max(1, a.freq_blocks))
-    for _ in range(a.steps):
+    spec_n = spec_ms = spec_fl = spec_bytes = 0.0
+    blocks = np.array_split(np.arange(len(freq)), max(1, freq_blocks))
+    for _ in range(steps):
         t = time.perf_counter()
         parts = []
         for b, idx in enumerate(blocks):
@@ -495,38 +510,74 @@ def run_c4(a):
             upd_n += st["lu_update_launches"]
             spec_n += st["spectral_targets"]
             spec_ms += st["spectral_ms"]
+            spec_fl += st["spectral_flops"]
+            spec_bytes += st["spectral_bytes"]
             if len(blocks) > 1:
                 log(f"[bench] C4 frequency block {b + 1}/{len(blocks)} done ({time.perf_counter() - t:.1f}s)")
         outs = [{"polar": np.concatenate([p[k]["polar"] for p in parts]),
                  "results": np.concatenate([p[k]["results"] for p in parts])} for k in range(len(flakes))]
         times.append(time.perf_counter() - t)
     dt = sum(times) / len(times)
-    # check: two flakes alone (pxh_solve_spectrum) at one frequency
+    # check: the largest flake and a middle one solved alone by the LU at two frequencies
     errs = []
-    for k in (0, len(flakes) // 2):
-        mid = len(freq) // 2
-        r = ctxs[k].solve_spectrum(flakes[k].n_var, freq[mid:mid + 1], 1e-4, pol_mask=3, method=_lib.INVERSION)
-        errs.append(float(np.linalg.norm(outs[k]["polar"][mid] - r["polar"][0]) / np.linalg.norm(r["polar"][0])))
+    by_n = sorted(range(len(flakes)), key=lambda k: flakes[k].n_var)
+    for k in (by_n[-1], by_n[len(by_n) // 2]):
+        pick = [len(freq) // 3, (2 * len(freq)) // 3]
+        ctxs[k].set_option("spectral", 0)
+        r = ctxs[k].solve_spectrum(flakes[k].n_var, freq[pick], 1e-4, pol_mask=3, method=_lib.INVERSION)
+        for q, f in enumerate(pick):
+            errs.append(float(np.linalg.norm(outs[k]["polar"][f] - r["polar"][q]) / np.linalg.norm(r["polar"][q])))
     for c in ctxs:
         c.close()
     pts = len(flakes) * len(freq)
-    flops = sum((8.0 / 3.0) * s.n_var ** 3 for s in flakes) * len(freq)
-    achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    log(f"[bench] C4 {len(flakes)} flakes x {len(freq)} w: {pts / dt:.1f} fp/s ({dt:.2f}s), err vs alone "
-        f"{max(errs):.1e}; spectral route: {int(spec_n)} flakes, decompositions {spec_ms / 1e3:.1f}s; LU updates {upd_ms / 1e3:.1f}s")
-    emit("flake-frequency solves/s (C4 graphene batch)", pts / dt, "freq-point solves/s", 1e3 * dt, a,
-         {"workload": f"{len(flakes)} seeded graphene flakes (500-{a.flake_max} atoms; the full batch is 96 flakes of "
-                     f"500-12k atoms), {len(freq)} w in 0.1-1.0 eV (of 128), x and y fields, grouped dense inversion "
-                     "(flakes of similar size padded to a common n and factorised by the same LU launches)",
-          "atoms_total": int(sum(s.n_atoms for s in flakes)), "setup_s": round(t_setup, 2),
-          "freq_blocks": len(blocks)},
-         dtype="fp64", max_rel_err_vs_alone=max(errs), unpadded_lu_tflops=round(flops / dt / 1e12, 2),
-         roofline={"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                   "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                   "kernel": "rocBLAS zgemm_strided_batched (LU trailing updates, padded matrices)",
-                   "launches": int(upd_n),
-                   "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4)},
-         cpu_baseline=None)
+    lu_flops = sum((8.0 / 3.0) * s.n_var ** 3 for s in flakes) * len(freq)
+    t_spec = spec_ms * 1e-3
+    gbs = spec_bytes / t_spec / 1e9 if t_spec > 0 else 0.0
+    cpu = None
+    if not a.no_cpu:
+        from oracle import pxo
+        threads, cpus = host_threads()
+        f0 = by_n[0]
+        kf = 4
+        t = time.perf_counter()
+        pxo.solve_spectrum(flakes[f0], freq[:kf], 1e-4, pol_mask=3, method=0)
+        dc = time.perf_counter() - t
+        n0 = flakes[f0].n_var
+        est = sum(dc / kf * (s.n_var / n0) ** 3 for s in flakes) * len(freq)
+        cpu = {"value": round(pts / est, 6), "unit": "flake-frequency solves/s", "cores": threads, "kind": "port",
+               "host_cpus": cpus, "extrapolated": True,
+               "sample": f"FP64 oracle dense LU of the smallest flake ({n0} atoms) at {kf} frequencies ({dc:.1f}s), "
+                         f"extrapolated by n^3 to every flake x {len(freq)} frequencies ({est / 3600:.1f} h)"}
+    log(f"[bench] C4 {len(flakes)} flakes x {len(freq)} w: {pts / dt:.1f} fp/s ({dt:.2f}s), err vs LU alone "
+        f"{max(errs):.1e}; spectral route: {int(spec_n)} flakes, decompositions {spec_ms / 1e3:.1f}s; LU updates "
+        f"{upd_ms / 1e3:.1f}s")
+    return {"metric": "flake-frequency solves/s (C4 graphene batch)", "value": round(pts / dt, 3),
+            "unit": "freq-point solves/s", "ms_per_step": round(1e3 * dt, 1), "dtype": "fp64",
+            "config": {"workload": f"{len(flakes)} seeded graphene flakes (500-{flake_max} atoms), {len(freq)} w in "
+                                   "0.1-1.0 eV, x and y fields, one grouped solve: flakes where one "
+                                   "eigendecomposition beats the LUs take the spectral route (T = L L^T, "
+                                   "eigenvectors of L^T K L for every frequency), the rest grouped dense LU",
+                       "atoms_total": int(sum(s.n_atoms for s in flakes)), "setup_s": round(t_setup, 2),
+                       "freq_blocks": len(blocks)},
+            "max_rel_err_vs_lu_alone": max(errs), "lu_equivalent_tflops": round(lu_flops / dt / 1e12, 2),
+            "spectral": {"flakes": int(spec_n / max(steps, 1)), "seconds": round(t_spec / max(steps, 1), 2),
+                         "nominal_tflops": round(spec_fl / t_spec / 1e12, 2) if t_spec > 0 else 0.0},
+            "lu_updates": {"achieved_tflops": round(upd_flops / (upd_ms * 1e-3) / 1e12, 2) if upd_ms > 0 else 0.0,
+                           "seconds": round(upd_ms * 1e-3 / max(steps, 1), 2), "launches": int(upd_n)},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "the spectral decompositions (rocSOLVER DPOTRF + DSYEVD, rocBLAS TRMM, the K-bar "
+                                   "kernel): algorithmic bytes of the tridiagonal reduction (the lower triangle of "
+                                   "the trailing matrix read once per column, 4 n^3 / 3 B per flake) over the "
+                                   "decomposition time (HIP events)"},
+            "cpu_baseline": cpu}
+
+
+def run_c4(a):
+    leg = leg_c4(a, n_flakes=a.flakes, flake_max=a.flake_max, n_freq=a.c4_freqs, freq_blocks=a.freq_blocks,
+                 steps=a.steps)
+    emit(leg["metric"], leg["value"], leg["unit"], leg["ms_per_step"], a, leg["config"],
+         **{k: v for k, v in leg.items() if k not in ("metric", "value", "unit", "ms_per_step", "config")})
 
 
 def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 8000), n_freq=(50, 100),

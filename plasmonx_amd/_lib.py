@@ -276,7 +276,8 @@ class Context:
                 "latency_p50_ms": s[14], "latency_p99_ms": s[15], "bem_gemm_ms": s[16], "bem_gemm_launches": s[17],
                 "bem_gemm_flops": s[18], "lu_update_ms": s[19], "lu_update_launches": s[20],
                 "lu_update_flops": s[21], "bem_gemm_bytes": s[22], "mfma_splits": s[23],
-                "spectral_targets": s[24], "spectral_ms": s[25], "spectral_flops": s[26]}
+                "spectral_targets": s[24], "spectral_ms": s[25], "spectral_flops": s[26],
+                "spectral_bytes": s[27]}
 
 
 def solve_group(contexts, freq_au, e0, rhs_form=RHS_FIELD, pol_mask=7):

@@ -1324,7 +1324,7 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
     for (int t : spec) {
         pxh_ctx *c = ctxs[t];
         if (c != c0)
-            for (int k = 24; k < 27; ++k) {
+            for (int k = 24; k < 28; ++k) {
                 c0->stats.v[k] += c->stats.v[k];
                 c->stats.v[k] = 0.0;
             }

@@ -283,6 +283,9 @@ bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
     // nominal flops: Cholesky n^3/3, S n^3, tridiagonal reduction 4/3 n^3,
     // back-transformation of the eigenvectors 2 n^3
     c->stats.v[26] += (1.0 / 3.0 + 1.0 + 4.0 / 3.0 + 2.0) * (double)n * n * n;
+    // algorithmic HBM bytes of the tridiagonal reduction: the lower triangle of
+    // the trailing matrix read once per column, sum 8 (n - k)^2 / 2 = 4 n^3 / 3
+    c->stats.v[27] += 4.0 / 3.0 * (double)n * n * n;
     S.n = n;
     S.mask = mask;
     return true;
