@@ -234,7 +234,7 @@ def run_c2(a):
 # HBM bytes per launch of the pair kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3
 # counters on the 48-system product, scaled by the same counters on a copy of
 # known size); None until measured.
-TRAFFIC_BYTES_PER_LAUNCH = 515.9e6
+TRAFFIC_BYTES_PER_LAUNCH = 308.8e6
 
 
 def traffic_run(ctx, s, freq):

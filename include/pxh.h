@@ -171,6 +171,9 @@ int pxh_solve_spectrum(pxh_ctx *ctx, int n_freq, const double *freq_au, double e
  * group's largest n, so one launch of each LU kernel covers many targets
  * (no counterpart in the reference, which solves one structure per run: the
  * loop of solve_w_inversion, algorithm_module.f90:189-265, over targets).
+ * Homogeneous omegaFQ targets for which one eigendecomposition is cheaper
+ * than their LUs (option "spectral") are solved by it, one after another on
+ * their own contexts' streams.
  * Work runs on ctxs[0]'s stream; errors are reported by ctxs[0].
  *   polar:   n_ctx x n_freq x (3 x 3) complex, may be NULL;
  *   results: n_ctx blocks of n_freq x 20 (column-major each), may be NULL. */
@@ -274,10 +277,16 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * source splits and keeps the fastest for the process), "graphs" (1, the
  * default: the multi-shift Arnoldi step on a BEM target is captured once as
  * a HIP graph and replayed, the graph kept with the context while its sizes
- * and buffers stay the same; 0 = every kernel launched directly). */
+ * and buffers stay the same; 0 = every kernel launched directly),
+ * "mfma_levels" (1 or 2, default 2: FP32 accumulation levels of the MFMA
+ * pair kernel), "spectral" (homogeneous omegaFQ inversion, one atom type:
+ * 1, the default, solves every frequency from one eigendecomposition of
+ * L^T K L, T = L L^T, when that is estimated faster than one LU per
+ * frequency; 2 whenever T is positive definite; 0 never; an indefinite T
+ * falls back to the LU). */
 int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
 
-/* Counters of the last pxh_solve_spectrum call (n <= 24): [0] vector
+/* Counters of the last pxh_solve_spectrum call (n <= 32): [0] vector
  * products, [1] FP32 block products, [2] FP64 block products, [3] / [4]
  * pair-kernel ms (FP32 / FP64, HIP events on the context's stream),
  * [5] wall ms, [6] pair evaluations, [7] outer refinements, [8] / [9]
@@ -291,7 +300,11 @@ int pxh_set_option(pxh_ctx *ctx, const char *key, double value);
  * multiply-add), [22] algorithmic bytes of the BEM matrix products (the
  * matrix once plus each vector read and written).  [16]-[18] and [22] count
  * the launches made directly only: the replays of a captured multi-shift
- * step (option "graphs") are neither timed nor counted there. */
+ * step (option "graphs") are neither timed nor counted there.  [23] source
+ * splits of the last MFMA product; [24] targets solved by the spectral
+ * route, [25] ms of their decompositions, [26] their nominal flops, [27]
+ * the algorithmic bytes of their tridiagonal reductions (4 n^3 / 3 each).
+ * pxh_solve_group reports on ctxs[0]. */
 int pxh_get_stats(pxh_ctx *ctx, double *stats, int n);
 
 /* ------------------------------------------------------------------ serving
