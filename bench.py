@@ -642,6 +642,8 @@ def leg_c5(a, n_requests=24, rate=200.0, atoms=(2000, 12000), tesserae=(2000, 80
         ref = c.solve_spectrum(n_var, r["freq"][k:k + 1], 1e-4, method=method, tol_abs=1e-14 * np.sqrt(n_var),
                                krylov_dim=1000, precision=_lib.FP64)
         errs.append(float(np.linalg.norm(o["polar"][k] - ref["polar"][0]) / np.linalg.norm(ref["polar"][0])))
+        log(f"[bench] C5 {r['kind']:16s} n_var {n_var:6d}: error vs FP64 at w[{k}] {errs[-1]:.1e}, true residual "
+            f"{o['true_residual'].max():.1e} (tol {1e-12 * np.sqrt(n_var):.1e})")
         c.close()
     lat = np.array([d[5] for d in done]) * 1e3
     arr = np.array([r["arrival"] for r in reqs])

@@ -1,6 +1,9 @@
 // C-ABI entry points of libpxh (declared in include/pxh.h).
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
+#include <mutex>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -20,6 +23,7 @@ template <typename F>
 int guarded(pxh_ctx *c, F &&f) {
     try {
         if (c) PXH_HIP(hipSetDevice(c->device));
+        pxh::DeviceShared gate;
         f();
         return PXH_OK;
     } catch (const Error &e) {
@@ -36,6 +40,62 @@ int guarded(pxh_ctx *c, F &&f) {
 }  // namespace
 
 namespace pxh {
+
+namespace {
+
+// Writer-preferring shared / exclusive gate over the process's device work.
+struct Gate {
+    std::mutex m;
+    std::condition_variable cv;
+    int readers = 0, writers_waiting = 0;
+    bool writer = false;
+} g_gate;
+thread_local int t_shared = 0;  // nesting depth of this thread's shared holds
+
+void gate_lock_shared() {
+    std::unique_lock<std::mutex> lk(g_gate.m);
+    g_gate.cv.wait(lk, [] { return !g_gate.writer && g_gate.writers_waiting == 0; });
+    ++g_gate.readers;
+}
+
+void gate_unlock_shared() {
+    std::lock_guard<std::mutex> lk(g_gate.m);
+    if (--g_gate.readers == 0) g_gate.cv.notify_all();
+}
+
+}  // namespace
+
+DeviceShared::DeviceShared() {
+    if (t_shared++ == 0) gate_lock_shared();
+}
+
+DeviceShared::~DeviceShared() {
+    if (--t_shared == 0) gate_unlock_shared();
+}
+
+DeviceExclusive::DeviceExclusive() : held_(t_shared) {
+    if (held_) gate_unlock_shared();
+    t_shared = 0;
+    {
+        std::unique_lock<std::mutex> lk(g_gate.m);
+        ++g_gate.writers_waiting;
+        g_gate.cv.wait(lk, [] { return !g_gate.writer && g_gate.readers == 0; });
+        --g_gate.writers_waiting;
+        g_gate.writer = true;
+    }
+    (void)hipDeviceSynchronize();  // work left on other streams by calls that have returned
+}
+
+DeviceExclusive::~DeviceExclusive() {
+    (void)hipDeviceSynchronize();
+    {
+        std::lock_guard<std::mutex> lk(g_gate.m);
+        g_gate.writer = false;
+    }
+    g_gate.cv.notify_all();
+    t_shared = held_;
+    if (held_) gate_lock_shared();
+}
 
 namespace {
 void timer_collect(pxh_ctx *c, int t) {

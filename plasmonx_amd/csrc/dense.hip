@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -1059,6 +1060,7 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     // is estimated faster than the LUs (the LU below when T is not positive
     // definite)
     if (!bem && rhs_fstride == 0 && spectral_applicable(c) && spectral_pays(c, n, n_freq)) {
+        DeviceExclusive alone;  // no other work of the process on the device meanwhile
         const bool ok = spectral_decompose(c, d_rhs_all, pol_mask);
         if (ok) {
             spectral_charges(c, n_freq, freq, d_sol, (size_t)rhs_stride);
@@ -1301,12 +1303,13 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
     }
     // homogeneous omegaFQ targets for which one eigendecomposition is
     // estimated faster than their LUs: solved alone by the spectral route,
-    // each on its own context (stream and rocBLAS handle), one after another
-    // (rocSOLVER decompositions run from several host threads at once gave
-    // wrong eigenpairs on this ROCm release)
+    // each on its own context (stream and rocBLAS handle), one after another,
+    // with the device gate held exclusively
     std::vector<int> order, spec;
     for (int t = 0; t < n_ctx; ++t) (by_spectral[t] ? spec : order).push_back(t);
     std::vector<std::vector<double2>> hpol(n_ctx);
+    std::unique_ptr<DeviceExclusive> alone;  // no other work of the process on the device meanwhile
+    if (!spec.empty()) alone.reset(new DeviceExclusive);
     for (int t : spec) {
         pxh_ctx *c = ctxs[t];
         if (spectral_decompose(c, rhs[t].p, pol_mask)) {
@@ -1321,6 +1324,7 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
         c->spectral.release();
         timer_drain(c);
     }
+    alone.reset();
     for (int t : spec) {
         pxh_ctx *c = ctxs[t];
         if (c != c0)

@@ -58,6 +58,7 @@ extern "C" int pxh_matvec_noalloc(pxh_ctx *c, double freq_au, int i_freq, int n_
     if (!c) return PXH_EINVAL;
     try {
         PXH_HIP(hipSetDevice(c->device));
+        pxh::DeviceShared gate;
         PXH_CHECK(c->target == PXH_WFQ || c->target == PXH_WFQFMU, PXH_EINVAL,
                   "pxh_matvec_noalloc: no atomistic target set (BEM products are not covered)");
         PXH_CHECK(n_vec > 0 && d_x && d_y, PXH_EINVAL, "pxh_matvec_noalloc: bad vectors");

@@ -289,6 +289,32 @@ void dynamic_rhs_bem(pxh_ctx *c, double e0, double2 *d_rhs);
 // results n_ctx blocks of n_freq x 20 (column-major)
 int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq, double e0, int rhs_form,
                 int pol_mask, double *polar, double *results);
+// api.hip: the device gate.  Every entry point that launches work holds the
+// gate shared for its duration (DeviceShared, taken by the entry points'
+// wrapper); the spectral route holds it exclusively (DeviceExclusive: the
+// caller's shared hold is given up while waiting, new shared holders wait
+// behind a waiting exclusive one, and the device is synchronised once the
+// gate is held), so its rocSOLVER / rocBLAS work runs with no other work of
+// the process on the device.  Decompositions that ran beside other
+// contexts' work (LU or GMRES requests of the request loop, other
+// decompositions) gave wrong eigenpairs on this ROCm release; alone they
+// match the LU to 1e-13.
+struct DeviceShared {
+    DeviceShared();
+    ~DeviceShared();
+    DeviceShared(const DeviceShared &) = delete;
+    DeviceShared &operator=(const DeviceShared &) = delete;
+};
+struct DeviceExclusive {
+    DeviceExclusive();
+    ~DeviceExclusive();
+    DeviceExclusive(const DeviceExclusive &) = delete;
+    DeviceExclusive &operator=(const DeviceExclusive &) = delete;
+
+  private:
+    int held_;  // shared holds of this thread given up while exclusive
+};
+
 // spectral.hip: the homogeneous omegaFQ inversion by one eigendecomposition
 bool spectral_applicable(const pxh_ctx *c);
 bool spectral_pays(const pxh_ctx *c, int n, int n_freq);

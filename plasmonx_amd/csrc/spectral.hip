@@ -30,7 +30,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <mutex>
 #include <vector>
 
 #include "pxh_internal.h"
@@ -40,12 +39,6 @@ namespace pxh {
 rocblas_handle blas_handle(pxh_ctx *c);
 
 namespace {
-
-// One decomposition at a time in the process: rocSOLVER DSYGVD called from
-// several host threads at once (each with its own rocBLAS handle and stream)
-// gave wrong eigenpairs on this ROCm release, so the request loop's workers
-// and every other caller take their turn here.
-std::mutex g_spectral_mu;
 
 // B6[:, k] = Re b_k, B6[:, 3 + k] = Im b_k for the selected directions (zero otherwise).
 __global__ void k_rhs_split(int n, int mask, const double2 *__restrict__ rhs, double *__restrict__ b6) {
@@ -219,8 +212,9 @@ bool spectral_pays(const pxh_ctx *c, int n, int n_freq) {
 // C = Q^T L^T b (n x 6: Re, Im of each direction).  Returns false when T is
 // not positive definite or the eigensolver does not converge (the caller
 // then uses the LU).
+// The caller holds the device gate exclusively (DeviceExclusive) from here
+// to the last use of the decomposition.
 bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
-    std::lock_guard<std::mutex> lk(g_spectral_mu);
     Atoms &At = c->atoms;
     const int n = At.n;
     hipStream_t st = c->stream;

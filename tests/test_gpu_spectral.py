@@ -160,3 +160,51 @@ def test_spectral_rods_in_the_request_loop_match_alone():
     assert relerr(alone[0]["polar"][[0, 60]], ref["polar"]) < 1e-10
     for c, _, _ in jobs:
         c.close()
+
+
+def test_spectral_beside_other_contexts_matches_alone():
+    # a spectral decomposition while other host threads run LU and GMRES-IR
+    # solves on their own contexts: the device gate keeps the decomposition
+    # alone on the device, and the result equals the one computed alone
+    import threading
+    from plasmonx_amd.workloads import core_shell_sphere, sodium_rod
+    big = sodium_rod(200.0, 14.0)
+    freq = np.linspace(2.5, 4.0, 120) / 27.211386245981
+    small = sodium_rod(60.0, 12.0)
+    fs = np.linspace(2.5, 4.0, 40) / 27.211386245981
+    cs = core_shell_sphere(0.6 * 24.0, 24.0)
+    fc = np.linspace(2.8, 4.2, 4) / 27.211386245981
+    print("atoms", big.n_var, small.n_var, cs.n_var, flush=True)
+    with Context(0) as ca, Context(0) as cb, Context(0) as cc:
+        ca.set_atoms(big)
+        cb.set_atoms(small)
+        cb.set_option("spectral", 0)
+        cc.set_atoms(cs)
+        cc.set_alpha(cs.alpha_table(fc))
+        alone = ca.solve_spectrum(big.n_var, freq, 1e-4, pol_mask=7, method=_lib.INVERSION)["polar"]
+        assert ca.stats()["spectral_targets"] == 1
+        stop = threading.Event()
+
+        def load(ctx, fn):
+            while not stop.is_set():
+                fn(ctx)
+
+        def lu(ctx):
+            ctx.solve_spectrum(small.n_var, fs, 1e-4, pol_mask=7, method=_lib.INVERSION)
+
+        def gmres(ctx):
+            ctx.solve_spectrum(cs.n_var, fc, 1e-4, pol_mask=7, method=_lib.ITERATIVE_OTF,
+                               tol_abs=1e-12 * np.sqrt(cs.n_var), krylov_dim=200, precision=_lib.MIXED)
+
+        th = [threading.Thread(target=load, args=(cb, lu)), threading.Thread(target=load, args=(cc, gmres))]
+        for t in th:
+            t.start()
+        try:
+            mixed = [ca.solve_spectrum(big.n_var, freq, 1e-4, pol_mask=7, method=_lib.INVERSION)["polar"]
+                     for _ in range(3)]
+        finally:
+            stop.set()
+            for t in th:
+                t.join()
+    for m in mixed:
+        assert np.array_equal(m, alone)
