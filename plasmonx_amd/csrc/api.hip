@@ -618,6 +618,9 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.matvec_mixed = value != 0.0;
         } else if (k == "mfma") {
             c->opt.mfma = value != 0.0;
+        } else if (k == "mfma_flush") {
+            PXH_CHECK(value >= 1 && value <= 1024, PXH_EINVAL, "mfma_flush must be 1..1024");
+            c->opt.mfma_flush = (int)value;
         } else if (k == "mfma_levels") {
             PXH_CHECK(value == 1 || value == 2, PXH_EINVAL, "mfma_levels must be 1 or 2");
             c->opt.mfma_levels = (int)value;

@@ -52,7 +52,6 @@ constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 constexpr double kFourOverSqrtPi = 2.2567583341910252;
 constexpr int kAtomsI = 64;        // target atoms per workgroup
 constexpr int kAtomsJ = 16;        // source atoms per step (K = 64)
-constexpr int kStepsPerFlush = 8;  // source steps between FP32 -> FP64 flushes
 
 // Per-column scale 2^e with max|x_col| * 2^e in [2^(kXTarget-1), 2^kXTarget),
 // in two stages: max |re| and max |im| of each system over chunks of
@@ -172,15 +171,19 @@ __device__ __forceinline__ void split4(float e0, float e1, float e2, float e3, h
 
 // The pair scalars of the 4 x 4 block of (i, j), scaled by tsc:
 // o = [T_qq, g1, -g2, c0, c1, dx, dy, dz] (rows q: [T_qq, g1 d];
-// mu_a: [-g2 d_a, c1 d_a d_b + c0 delta_ab]).  An excluded pair (i = j or
-// beyond n) gets r^2 = 3e38, whose scaled elements round to zero in fp16.
+// mu_a: [-g2 d_a, c1 d_a d_b + c0 delta_ab]).  The pair i = j (r^2 = 0) gets
+// r^2 = 3e38, whose scaled elements round to zero in fp16; the padding atoms
+// past n (the position array is padded to a multiple of 64 with atoms 1e15
+// bohr away) give elements that round to zero as well, and their X is zero.
 // Beyond the far-field radius the Coulomb limits share one scaled 1/r^3.
-__device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, const double4 &pj, int ti, int nt,
-                                             int hetero, float rfar2, float tsc, const float (*spc)[4], float *o) {
+template <bool HET>
+__device__ __forceinline__ void pair_scalars(const double4 &pi, const double4 &pj, int ti, int nt, float rfar2,
+                                             float tsc, const float (*spc)[4], float *o) {
     const float dx = (float)(pi.x - pj.x);
     const float dy = (float)(pi.y - pj.y);
     const float dz = (float)(pi.z - pj.z);
-    const float r2 = valid ? fmaf(dx, dx, fmaf(dy, dy, dz * dz)) : 3.0e38f;
+    const float r2a = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float r2 = r2a > 0.f ? r2a : 3.0e38f;  // i = j (also padding against padding)
     const float inv_r = __builtin_amdgcn_rsqf(r2);  // r2 is never denormal here
     const float inv_r2 = inv_r * inv_r;
     float tqq = inv_r * tsc;
@@ -193,14 +196,14 @@ __device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, cons
         const float r = r2 * inv_r;
         const float *pp = spc[ti + nt * tj];
         const float a1 = r * pp[0], a2 = r * pp[1], a3 = r * pp[2];
-        const float a2h = hetero ? r * spc[tj + nt * ti][1] : a2;
+        const float a2h = HET ? r * spc[tj + nt * ti][1] : a2;
         const float e3 = __expf(-a3 * a3);
         float fd1;
         if (__all(fminf(fminf(a1, a2), fminf(a3, a2h)) >= 1.0f)) {
             const float e1 = __expf(-a1 * a1), e2 = __expf(-a2 * a2);
             tqq = erf_as(a1, e1) * inv_r * tsc;
             g1 = (erf_as(a2, e2) - (float)kTwoOverSqrtPi * a2 * e2) * inv_r3 * tsc;
-            if (hetero) {
+            if constexpr (HET) {
                 const float e2h = __expf(-a2h * a2h);
                 g2 = -(erf_as(a2h, e2h) - (float)kTwoOverSqrtPi * a2h * e2h) * inv_r3 * tsc;
             } else {
@@ -210,7 +213,7 @@ __device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, cons
         } else {
             tqq = erff(a1) * inv_r * tsc;
             g1 = qmu_factor(a2, inv_r3) * tsc;
-            g2 = -(hetero ? qmu_factor(a2h, inv_r3) * tsc : g1);
+            g2 = -(HET ? qmu_factor(a2h, inv_r3) * tsc : g1);
             fd1 = erff(a3) - (float)kTwoOverSqrtPi * a3 * e3;
         }
         const float fd2 = (float)kFourOverSqrtPi * pp[2] * pp[2] * pp[2] * e3;
@@ -237,13 +240,14 @@ __device__ __forceinline__ void pair_scalars(bool valid, const double4 &pi, cons
 // (r, source 4s + 2h + p), p = 0, 1, builds its two rows of both 4 x 4
 // blocks in registers as the A fragments (fp16 hi / lo) and runs
 // 2 x NCB x 3 MFMAs; each X fragment read from LDS feeds both row blocks.
-// Accumulation: FP32 in the MFMA registers for kStepsPerFlush source
+// Accumulation: FP32 in the MFMA registers for flush_steps source
 // steps, then into a second FP32 accumulator (blocked summation).
 // Blocks are placed so that all workgroups of one source split share an XCD
 // (split = linear block id mod splits): with splits a multiple of 8 an XCD's
 // L2 holds the X fragments of its split.
-template <int NCB, int LV>
-__global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int nt, int hetero, float rfar2, float tsc,
+template <int NCB, int LV, bool HET>
+__global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int nt, int flush_steps, float rfar2,
+                                                                      float tsc,
                                                         const double4 *__restrict__ pos,
                                                         const PairConst *__restrict__ pc, int nsys,
                                                         const float *__restrict__ scale,
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
     }
     const int kh = lane / 32;
     const int gi = i0 + lane % 32;
-    const double4 pi = gi < n ? pos[gi] : make_double4(0, 0, 0, 0);
+    const double4 pi = pos[gi];  // padded: targets past n sit far away
     const int ti = (int)pi.w;
     f32x16 acc[2][NCB];
     float acc2[2][NCB][16];
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
                 (__attribute__((address_space(3))) void *)(&sBl[b][64 * wave + 256 * r]), 16, 0, 0);
         }
         if (wave == 0 && lane < 2 * kAtomsJ) {
-            const int e = min(jb * kAtomsJ + lane / 2, n - 1);  // sources past n are excluded by gj < n
+            const int e = jb * kAtomsJ + lane / 2;  // padded: sources past n sit far away, with zero X
             __builtin_amdgcn_global_load_lds(
                 (const __attribute__((address_space(1))) void *)(reinterpret_cast<const char *>(pos + e) +
                                                                  16 * (lane % 2)),
@@ -307,8 +311,8 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
     };
     if (jb0 < jb_end) issue(jb0, 0);
     int buf = 0;
-    for (int jbb = jb0; jbb < jb_end; jbb += kStepsPerFlush) {
-        const int jbe = min(jb_end, jbb + kStepsPerFlush);
+    for (int jbb = jb0; jbb < jb_end; jbb += flush_steps) {
+        const int jbe = min(jb_end, jbb + flush_steps);
         for (int jb = jbb; jb < jbe; ++jb) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();  // step jb is in LDS; every wave is done with step jb - 1's buffer
@@ -323,8 +327,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
                     const int jl = 4 * s + 2 * kh + p;
                     const int gj = jb * kAtomsJ + jl;
                     float sc[8];
-                    pair_scalars(gi < n && gj < n && gi != gj, pi, spos[buf][jl], ti, nt, hetero, rfar2, tsc, spc,
-                                 sc);
+                    pair_scalars<HET>(pi, spos[buf][jl], ti, nt, rfar2, tsc, spc, sc);
                     const float tqq = sc[0], g1 = sc[1], g2 = sc[2], c0 = sc[3], c1 = sc[4];
                     const float dx = sc[5], dy = sc[6], dz = sc[7];
                     if (cp == 0) {  // rows q and mu_x
@@ -428,15 +431,15 @@ template <int K>
 void launch_sweep(pxh_ctx *c, int iblocks, int splits, int per, float rfar2, float tsc, int ns, int nsys, int goff) {
     const Atoms &A = c->atoms;
     auto go = [&](auto kern) {
-        kern<<<iblocks * splits, 256, 0, c->stream>>>(A.n, A.nt, A.hetero ? 1 : 0, rfar2, tsc, A.pos.p, A.pair.p, ns,
+        kern<<<iblocks * splits, 256, 0, c->stream>>>(A.n, A.nt, c->opt.mfma_flush, rfar2, tsc, A.pos.p, A.pair.p, ns,
                                                       c->work_scale.p, c->work_xh.p, c->work_xl.p,
                                                       reinterpret_cast<float2 *>(c->work_part.p), A.n_var(), splits,
                                                       per, nsys, goff);
     };
     if (c->opt.mfma_levels == 1)
-        go(k_tsweep_mfma<K, 1>);
+        A.hetero ? go(k_tsweep_mfma<K, 1, true>) : go(k_tsweep_mfma<K, 1, false>);
     else
-        go(k_tsweep_mfma<K, 2>);
+        A.hetero ? go(k_tsweep_mfma<K, 2, true>) : go(k_tsweep_mfma<K, 2, false>);
 }
 
 }  // namespace

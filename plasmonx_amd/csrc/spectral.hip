@@ -293,7 +293,10 @@ void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *sol, 
     const int n = S.n;
     hipStream_t st = c->stream;
     rocblas_handle h = blas_handle(c);
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_freq, (size_t(1) << 30) / ((size_t)96 * n)));
+    // frequencies per block: Y and Q within 1 GB each, and 3 per frequency
+    // within the grid's y dimension
+    const int chunk = (int)std::max<size_t>(
+        1, std::min<size_t>(std::min<size_t>(n_freq, 16384), (size_t(1) << 30) / ((size_t)48 * n)));
     DevBuf<double> Y, Q;
     DevBuf<double2> z;
     std::vector<double2> hz(n_freq);
