@@ -50,11 +50,15 @@ def main():
     ap.add_argument("--radius", type=float, default=40.0)
     ap.add_argument("--batch", type=int, default=48)
     ap.add_argument("--krylov", type=int, default=120)
-    ap.add_argument("--tol", type=float, default=1e-12)
+    # GMRES threshold tol * sqrt(n_var) (input_module.f90:251); the reference's
+    # default tol is 1e-9 (python/input_/validation.py:303-304), which leaves
+    # errors near 1e-5 here; 1e-11 keeps every checked frequency more than 10x
+    # under the 1e-6 bar (1e-12: ~7e-9, 1e-11: ~7e-8, 1e-10: ~6e-7 measured)
+    ap.add_argument("--tol", type=float, default=1e-11)
     ap.add_argument("--splits", type=int, default=0, help="c2: source splits of the pair kernel (0: tuned)")
     ap.add_argument("--mfma-levels", type=int, default=0, help="c2: FP32 accumulation levels of the pair kernel "
                                                                "(0: the library's default)")
-    ap.add_argument("--check", type=int, default=21, help="frequencies checked against FP64 (plus the peak)")
+    ap.add_argument("--check", type=int, default=41, help="frequencies checked against FP64 (plus the peak)")
     ap.add_argument("--plain-check", type=int, default=3, help="frequencies solved by the plain FP64 path")
     ap.add_argument("--no-legs", action="store_true", help="C2 alone, without the C1 / C3 / C5 legs")
     ap.add_argument("--no-cpu", action="store_true")
@@ -154,15 +158,17 @@ def run_c2(a):
     assert np.all(out["true_residual"] < tol)
 
     # accuracy against FP64: a solve with FP64 products only (batched FP64
-    # GMRES, 100x tighter tolerance) at frequencies spread over the spectrum
-    # and at the plasmon peak (largest Im alpha_iso of the mixed result)
+    # GMRES to a true residual of 1e-14 sqrt(n_var)) at frequencies spread
+    # over the spectrum and at the plasmon peak (largest Im alpha_iso of the
+    # mixed result)
     peak = int(np.argmax(out["results"][:, 1]))
     chk = np.unique(np.append(np.linspace(0, len(freq) - 1, max(1, a.check)).astype(int), peak))
     ctx.set_option("batch", a.batch)
     tp = time.perf_counter()
-    ref = solve(chk, precision=_lib.FP64, tol_abs=1e-2 * tol, kdim=a.krylov)
+    ref_tol = 1e-14 * np.sqrt(s.n_var)
+    ref = solve(chk, precision=_lib.FP64, tol_abs=ref_tol, kdim=a.krylov)
     t_ref = time.perf_counter() - tp
-    assert np.all(ref["true_residual"] < 1e-2 * tol)
+    assert np.all(ref["true_residual"] < ref_tol)
     errs = [float(np.linalg.norm(out["polar"][f] - ref["polar"][k]) / np.linalg.norm(ref["polar"][k]))
             for k, f in enumerate(chk)]
     err_peak = errs[int(np.searchsorted(chk, peak))]
@@ -215,7 +221,10 @@ def run_c2(a):
         "config": {"workload": "omegaFQFmu Ag fcc sphere a=4.09A r=4nm, Lorentz-Drude (Rakic 1998) interband "
                                "alpha via a 'wfqfmu file' table, 201 w 2.8-4.2 eV x 3 fields, GMRES on the fly",
                    "n_atoms": s.n_atoms, "n_var": s.n_var, "n_freq": len(freq), "fields": 3,
-                   "tol_abs": tol, "batch": a.batch, "krylov_dim": a.krylov, "parallelism": "single GPU"},
+                   "tol_abs": tol, "tol": a.tol, "tol_rule": "tol * sqrt(n_var) on the FP64 true residual "
+                                                           "(input_module.f90:251); the reference's default tol is "
+                                                           "1e-9 (validation.py:303-304)",
+                   "batch": a.batch, "krylov_dim": a.krylov, "parallelism": "single GPU"},
         "max_rel_err_vs_fp64": max(errs),
         "err_vs_fp64": {"frequencies": len(chk), "max": max(errs), "at_peak": err_peak,
                         "peak_ev": round(float(freq[peak] * 27.211386245981), 4),
