@@ -195,14 +195,15 @@ bool spectral_applicable(const pxh_ctx *c) {
 
 // Estimated seconds: the LU route (8/3 n^3 flop per frequency at the
 // whole-path rate measured on the batched LU) against one decomposition
-// (DSYGVD times measured on MI355X: 0.33 s at n = 4000, 1.08 s at 8000,
-// 2.61 s at 12000, ~n^1.9, and at least ~0.12 s below that).
+// (measured on MI355X, DSYEVD with the Cholesky and the products around it:
+// ~0.15 s at n = 4000, 0.7 s at 8000, 1.9 s at 12000, ~n^2.3, and at least
+// ~0.06 s below that, where the eigensolver's per-column launches dominate).
 bool spectral_pays(const pxh_ctx *c, int n, int n_freq) {
     if (c->opt.spectral == 0) return false;
     if (c->opt.spectral == 2) return true;
     const double nn = (double)n;
     const double t_lu = n_freq * (8.0 / 3.0) * nn * nn * nn / 23e12;
-    const double t_eig = std::max(0.12, 0.33 * std::pow(nn / 4000.0, 1.9)) + 2.0 * nn * nn * nn / 40e12;
+    const double t_eig = std::max(0.06, 0.15 * std::pow(nn / 4000.0, 2.3));
     return t_eig < 0.8 * t_lu;
 }
 
