@@ -618,6 +618,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.matvec_mixed = value != 0.0;
         } else if (k == "mfma") {
             c->opt.mfma = value != 0.0;
+        } else if (k == "mfma_r4") {
+            c->opt.mfma_r4 = value != 0.0;
         } else if (k == "mfma_flush") {
             PXH_CHECK(value >= 1 && value <= 1024, PXH_EINVAL, "mfma_flush must be 1..1024");
             c->opt.mfma_flush = (int)value;

@@ -913,10 +913,10 @@ SysParam make_sysparam(const pxh_ctx *c, double freq, int i_freq) {
 void build_atoms_device(pxh_ctx *c) {
     Atoms &A = c->atoms;
     A.Mconst_valid = false;
-    // padded to a multiple of 64 atoms with atoms 1e15 bohr away (type 0):
+    // padded to a multiple of 128 atoms with atoms 1e15 bohr away (type 0):
     // the MFMA pair kernel reads whole blocks of targets and sources, and the
     // interactions with these atoms round to zero
-    const int n_pad = (A.n + 63) / 64 * 64;
+    const int n_pad = (A.n + 127) / 128 * 128;
     std::vector<double4> hp(n_pad, make_double4(1e15, 1e15, 1e15, 0.0));
     for (int i = 0; i < A.n; ++i)
         hp[i] = make_double4(A.h_coord[3 * i], A.h_coord[3 * i + 1], A.h_coord[3 * i + 2], (double)A.h_type[i]);

@@ -173,7 +173,7 @@ __device__ __forceinline__ void split4(float e0, float e1, float e2, float e3, h
 // o = [T_qq, g1, -g2, c0, c1, dx, dy, dz] (rows q: [T_qq, g1 d];
 // mu_a: [-g2 d_a, c1 d_a d_b + c0 delta_ab]).  The pair i = j (r^2 = 0) gets
 // r^2 = 3e38, whose scaled elements round to zero in fp16; the padding atoms
-// past n (the position array is padded to a multiple of 64 with atoms 1e15
+// past n (the position array is padded to a multiple of 128 with atoms 1e15
 // bohr away) give elements that round to zero as well, and their X is zero.
 // Beyond the far-field radius the Coulomb limits share one scaled 1/r^3.
 template <bool HET>
@@ -245,7 +245,7 @@ __device__ __forceinline__ void pair_scalars(const double4 &pi, const double4 &p
 // Blocks are placed so that all workgroups of one source split share an XCD
 // (split = linear block id mod splits): with splits a multiple of 8 an XCD's
 // L2 holds the X fragments of its split.
-template <int NCB, int LV, bool HET>
+template <int NCB, int LV, bool HET, bool R4>
 __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int nt, int flush_steps, float rfar2,
                                                                       float tsc,
                                                         const double4 *__restrict__ pos,
@@ -259,11 +259,12 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
     __shared__ half8 sBl[2][4 * NCB * 64];
     __shared__ double4 spos[2][kAtomsJ];
     __shared__ float spc[kMaxTypes * kMaxTypes][4];
+    constexpr int NRB = R4 ? 4 : 2;  // row blocks (components) per wave
     const int tid = threadIdx.x, lane = tid % 64;
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-    const int cp = wave % 2;  // rows 2 cp and 2 cp + 1
+    const int cp = R4 ? 0 : wave % 2;  // two row blocks: rows 2 cp and 2 cp + 1
     const int split = blockIdx.x % splits;
-    const int i0 = (blockIdx.x / splits) * kAtomsI + 32 * (wave / 2);
+    const int i0 = R4 ? (blockIdx.x / splits) * 4 * 32 + 32 * wave : (blockIdx.x / splits) * kAtomsI + 32 * (wave / 2);
     const int jb0 = split * jsteps_per_split;
     const int jb_end = min((n + kAtomsJ - 1) / kAtomsJ, jb0 + jsteps_per_split);
     for (int e = tid; e < nt * nt; e += 256) {
@@ -276,10 +277,10 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
     const int gi = i0 + lane % 32;
     const double4 pi = pos[gi];  // padded: targets past n sit far away
     const int ti = (int)pi.w;
-    f32x16 acc[2][NCB];
-    float acc2[2][NCB][16];
+    f32x16 acc[NRB][NCB];
+    float acc2[NRB][NCB][16];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
             const half8 *bl_ = sBl[buf];
 #pragma unroll 1
             for (int s = 0; s < 4; ++s) {
-                half4 h[2][2], l[2][2];  // [row block][pair]
+                half4 h[NRB][2], l[NRB][2];  // [row block][pair]
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
                     const int jl = 4 * s + 2 * kh + p;
@@ -330,14 +331,18 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
                     pair_scalars<HET>(pi, spos[buf][jl], ti, nt, rfar2, tsc, spc, sc);
                     const float tqq = sc[0], g1 = sc[1], g2 = sc[2], c0 = sc[3], c1 = sc[4];
                     const float dx = sc[5], dy = sc[6], dz = sc[7];
-                    if (cp == 0) {  // rows q and mu_x
+                    // the same element expressions in both layouts, so that
+                    // their products agree bit for bit
+                    if (R4 || cp == 0) {  // rows q and mu_x
                         const float ux = c1 * dx;
                         split4(tqq, g1 * dx, g1 * dy, g1 * dz, h[0][p], l[0][p]);
                         split4(g2 * dx, fmaf(ux, dx, c0), ux * dy, ux * dz, h[1][p], l[1][p]);
-                    } else {  // rows mu_y and mu_z
+                    }
+                    if (R4 || cp == 1) {  // rows mu_y and mu_z
+                        constexpr int o = R4 ? 2 : 0;
                         const float uy = c1 * dy, uz = c1 * dz;
-                        split4(g2 * dy, uy * dx, fmaf(uy, dy, c0), uy * dz, h[0][p], l[0][p]);
-                        split4(g2 * dz, uz * dx, uz * dy, fmaf(uz, dz, c0), h[1][p], l[1][p]);
+                        split4(g2 * dy, uy * dx, fmaf(uy, dy, c0), uy * dz, h[o][p], l[o][p]);
+                        split4(g2 * dz, uz * dx, uz * dy, fmaf(uz, dz, c0), h[o + 1][p], l[o + 1][p]);
                     }
                 }
                 // A row = target lane % 32, k = 8 (lane / 32) + e = component e % 4 of
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
                     const half8 bh = bh_[(s * NCB + cb) * 64 + lane];
                     const half8 bl = bl_[(s * NCB + cb) * 64 + lane];
 #pragma unroll
-                    for (int rb = 0; rb < 2; ++rb) {
+                    for (int rb = 0; rb < NRB; ++rb) {
                         const half8 ah = (half8){h[rb][0][0], h[rb][0][1], h[rb][0][2], h[rb][0][3],
                                                  h[rb][1][0], h[rb][1][1], h[rb][1][2], h[rb][1][3]};
                         const half8 al = (half8){l[rb][0][0], l[rb][0][1], l[rb][0][2], l[rb][0][3],
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
         }
         if constexpr (LV == 2) {
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
                 for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
@@ -382,12 +387,12 @@ __global__ __launch_bounds__(256, LV == 1 ? 3 : 2) void k_tsweep_mfma(int n, int
         if (sys >= nsys) continue;
         const double cs = scale[col] > 0.f ? tinv / (double)scale[col] : 0.0;
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int i = i0 + (v / 4) * 8 + (lane / 32) * 4 + (v % 4);
                 if (i >= n) continue;
-                const int c = 2 * cp + rb;
+                const int c = R4 ? rb : 2 * cp + rb;
                 const int row = c == 0 ? i : n + 3 * i + c - 1;
                 float *dst = reinterpret_cast<float *>(
                     part + ((size_t)split * nv + row) * nsys_total_in_part + sys_offset + sys);
@@ -430,16 +435,32 @@ std::map<std::pair<int, int>, int> g_tuned_splits;
 template <int K>
 void launch_sweep(pxh_ctx *c, int iblocks, int splits, int per, float rfar2, float tsc, int ns, int nsys, int goff) {
     const Atoms &A = c->atoms;
+    // one column block: four row blocks per wave (each pair's elements built
+    // once per workgroup instead of twice), 128 targets per workgroup
+    constexpr bool R4c = K == 1;
+    const bool R4 = R4c && c->opt.mfma_r4;
+    const int blocks = R4 ? (A.n + 127) / 128 : iblocks;
     auto go = [&](auto kern) {
-        kern<<<iblocks * splits, 256, 0, c->stream>>>(A.n, A.nt, c->opt.mfma_flush, rfar2, tsc, A.pos.p, A.pair.p, ns,
-                                                      c->work_scale.p, c->work_xh.p, c->work_xl.p,
-                                                      reinterpret_cast<float2 *>(c->work_part.p), A.n_var(), splits,
-                                                      per, nsys, goff);
+        kern<<<blocks * splits, 256, 0, c->stream>>>(A.n, A.nt, c->opt.mfma_flush, rfar2, tsc, A.pos.p, A.pair.p, ns,
+                                                     c->work_scale.p, c->work_xh.p, c->work_xl.p,
+                                                     reinterpret_cast<float2 *>(c->work_part.p), A.n_var(), splits,
+                                                     per, nsys, goff);
     };
-    if (c->opt.mfma_levels == 1)
-        A.hetero ? go(k_tsweep_mfma<K, 1, true>) : go(k_tsweep_mfma<K, 1, false>);
-    else
-        A.hetero ? go(k_tsweep_mfma<K, 2, true>) : go(k_tsweep_mfma<K, 2, false>);
+    auto pick = [&](auto r4) {
+        constexpr bool F = decltype(r4)::value;
+        if (c->opt.mfma_levels == 1)
+            A.hetero ? go(k_tsweep_mfma<K, 1, true, F>) : go(k_tsweep_mfma<K, 1, false, F>);
+        else
+            A.hetero ? go(k_tsweep_mfma<K, 2, true, F>) : go(k_tsweep_mfma<K, 2, false, F>);
+    };
+    if constexpr (R4c) {
+        if (R4)
+            pick(std::true_type{});
+        else
+            pick(std::false_type{});
+    } else {
+        pick(std::false_type{});
+    }
 }
 
 }  // namespace

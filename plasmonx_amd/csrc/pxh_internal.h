@@ -137,6 +137,7 @@ struct Options {
     int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
     int mfma_levels = 2;     // FP32 accumulation levels of the MFMA pair kernel (1 or 2)
     int mfma_flush = 8;      // source steps (16 atoms each) summed in the MFMA registers between flushes
+    int mfma_r4 = 1;         // one-column-block MFMA passes: four row blocks per wave (else two)
     int matvec_mixed = 0;    // pxh_matvec uses the mixed-precision product (testing)
     double reorth_kappa = 0.70710678118654752;  // batched CGS: second pass when ||w'|| < kappa ||w||
     int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies

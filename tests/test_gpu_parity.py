@@ -259,6 +259,35 @@ def test_mixed_product_accuracy(gpu_ctx, mfma, nvec):
         assert err < 2e-5, (k, err)
 
 
+@pytest.mark.parametrize("alloy", [False, True])
+def test_four_row_block_layout_is_bit_identical(gpu_ctx, alloy):
+    """One-column-block MFMA passes (<= 16 systems) run four row blocks per
+    wave (each pair's elements built once per workgroup): the products equal
+    those of the two-row-block layout bit for bit (same elements, same
+    accumulation order), for one and for two atom types."""
+    from plasmonx_amd.workloads import c2_silver_sphere, core_shell_sphere
+    if alloy:
+        s = core_shell_sphere(7.0, 12.0)
+        freq = np.linspace(2.8, 4.2, 3) / 27.211386245981
+    else:
+        s, freq = c2_silver_sphere(radius=14.0, n_freq=3)
+    gpu_ctx.set_option("kbar_cutoff", 1)
+    gpu_ctx.set_atoms(s)
+    gpu_ctx.set_alpha(s.alpha_table(freq))
+    rng = np.random.default_rng(16)
+    x = rng.normal(size=(16, s.n_var)) + 1j * rng.normal(size=(16, s.n_var))
+    gpu_ctx.set_option("matvec_mixed", 1)
+    got = []
+    for r4 in (1, 0):
+        gpu_ctx.set_option("mfma_r4", r4)
+        got.append(gpu_ctx.matvec(freq[1], 1, x))
+    gpu_ctx.set_option("mfma_r4", 1)
+    gpu_ctx.set_option("matvec_mixed", 0)
+    assert np.array_equal(got[0], got[1])
+    ref = gpu_ctx.matvec(freq[1], 1, x)
+    assert max(np.linalg.norm(got[0][k] - ref[k]) / np.linalg.norm(ref[k]) for k in range(16)) < 2e-5
+
+
 def test_autotuned_mfma_product(gpu_ctx):
     """3.4k-atom Ag sphere, 48 vectors: the first full MFMA pass of this atom
     count times the candidate source splits and keeps the fastest; the
