@@ -826,6 +826,162 @@ __global__ __launch_bounds__(256) void k_dcgs_update(int nv, const int *__restri
     if (threadIdx.x == 0) npart[(size_t)s * gridDim.x + blockIdx.x] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
 }
 
+// The same two passes for a complex FP32 basis with nv even: each work-item
+// takes row pairs (one 16-byte load per basis vector instead of two 8-byte
+// ones).  Measured at the C2 shape (62 732 rows, 48 slots, 2-41 vectors):
+// the dots 2.6-2.9 -> 4.2-4.7 TB/s, the update 4.7-5.6 -> 4.9-6.3 TB/s.
+constexpr int kPairRowsU = 2;                        // row pairs per work-item in the dots
+constexpr int kDotPairRows = 2 * 256 * kPairRowsU;  // rows per workgroup (one chunk)
+constexpr int kDotPairGroup = 8;                     // basis vectors per workgroup layer
+
+// 32 values per lane summed over the wavefront by halving exchanges (31
+// shuffles instead of 192); on return v[0] of lane L holds the sum of value
+// L >> 1.
+__device__ __forceinline__ void wave_transpose_sum32(double (&v)[32], int lane) {
+    int n = 32;
+#pragma unroll
+    for (int o = 32; o >= 2; o >>= 1) {
+        const int h = n / 2;
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < h) {
+                const double send = up ? v[k] : v[k + h];
+                const double keep = up ? v[k + h] : v[k];
+                v[k] = keep + __shfl_xor(send, o, 64);
+            }
+        }
+        n = h;
+    }
+    v[0] += __shfl_xor(v[0], 1, 64);
+}
+
+// As k_dots2_part: chunk partials of conj(q_i) . W and conj(q_i) . u_{jc-1}
+// for kDotPairGroup basis vectors per workgroup layer, one chunk of
+// kDotPairRows rows per workgroup.
+__global__ __launch_bounds__(256) void k_dots2_pairs(int nv, const int *__restrict__ slots,
+                                                     const int *__restrict__ jcount, const float2 *__restrict__ V,
+                                                     size_t ldv_slot, const double2 *__restrict__ W,
+                                                     double2 *__restrict__ pw, double2 *__restrict__ pu, int ldh,
+                                                     int nch) {
+    constexpr int G = kDotPairGroup, U2 = kPairRowsU;
+    static_assert(4 * G == 32, "four values per basis vector, 32 per lane");
+    __shared__ double red[4][32];
+    const int ch = blockIdx.x;
+    const int s = slots[blockIdx.y];
+    const int jc = jcount[s];
+    const int i0 = blockIdx.z * G;
+    if (i0 >= jc) return;
+    const int t = threadIdx.x, lane = t % 64, wv = t / 64;
+    const int np = nv / 2;
+    const int p0 = ch * 256 * U2 + t;
+    double2 w[2 * U2], u[2 * U2];
+    const double2 *ws = W + (size_t)nv * s;
+    const float4 *us = reinterpret_cast<const float4 *>(V + ldv_slot * s + (size_t)nv * (jc - 1));
+#pragma unroll
+    for (int k = 0; k < U2; ++k) {
+        const int p = p0 + 256 * k;
+        const bool in = p < np;
+        const float4 uu = in ? us[p] : make_float4(0, 0, 0, 0);
+        u[2 * k] = make_double2(uu.x, uu.y);
+        u[2 * k + 1] = make_double2(uu.z, uu.w);
+        w[2 * k] = in ? ws[2 * p] : make_double2(0, 0);
+        w[2 * k + 1] = in ? ws[2 * p + 1] : make_double2(0, 0);
+    }
+    double v[32];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        double2 aw = make_double2(0, 0), au = make_double2(0, 0);
+        if (i0 + g < jc) {
+            const float4 *vp = reinterpret_cast<const float4 *>(V + ldv_slot * s + (size_t)nv * (i0 + g));
+#pragma unroll
+            for (int k = 0; k < U2; ++k) {
+                const int p = p0 + 256 * k;
+                if (p < np) {
+                    const float4 vv = vp[p];
+                    const double2 a = make_double2(vv.x, vv.y), b = make_double2(vv.z, vv.w);
+                    const double2 p1 = cconjmul(a, w[2 * k]), p2 = cconjmul(b, w[2 * k + 1]);
+                    const double2 q1 = cconjmul(a, u[2 * k]), q2 = cconjmul(b, u[2 * k + 1]);
+                    aw.x += p1.x + p2.x;
+                    aw.y += p1.y + p2.y;
+                    au.x += q1.x + q2.x;
+                    au.y += q1.y + q2.y;
+                }
+            }
+        }
+        v[4 * g] = aw.x;
+        v[4 * g + 1] = aw.y;
+        v[4 * g + 2] = au.x;
+        v[4 * g + 3] = au.y;
+    }
+    wave_transpose_sum32(v, lane);
+    if ((lane & 1) == 0) red[wv][lane >> 1] = v[0];
+    __syncthreads();
+    if (t < 32) {
+        const double x = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        const int g = t / 4, c = t % 4;
+        if (i0 + g < jc) {
+            double *dst = reinterpret_cast<double *>((c < 2 ? pw : pu) + ((size_t)s * ldh + i0 + g) * nch + ch);
+            dst[c & 1] = x;
+        }
+    }
+}
+
+// As k_dcgs_update, one row pair per work-item.
+__global__ __launch_bounds__(256) void k_dcgs_update_pairs(int nv, const int *__restrict__ slots,
+                                                           const int *__restrict__ jcount, float2 *__restrict__ V,
+                                                           size_t ldv_slot, const double2 *__restrict__ acoef,
+                                                           const double2 *__restrict__ fcoef, int ldh,
+                                                           const double *__restrict__ nu_in, double2 *__restrict__ W,
+                                                           double *__restrict__ npart) {
+    __shared__ double sq[4];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slots[blockIdx.y];
+    const int j = jcount[s];
+    const double nu = nu_in[s] > 0.0 ? nu_in[s] : 1.0;
+    const int np = nv / 2;
+    double q2 = 0.0;
+    if (p < np) {
+        float4 *vs = reinterpret_cast<float4 *>(V + ldv_slot * s) + p;
+        const size_t ld4 = (size_t)np;
+        const double2 *as = acoef + (size_t)s * ldh;
+        const double2 *fs = fcoef + (size_t)s * ldh;
+        double2 qa0 = make_double2(0, 0), qf0 = qa0, qa1 = qa0, qf1 = qa0;
+        for (int i = 0; i < j; ++i) {
+            const float4 v4 = vs[ld4 * i];
+            const double2 a = as[i], f = fs[i];
+            const double2 v0 = make_double2(v4.x, v4.y), v1 = make_double2(v4.z, v4.w);
+            const double2 pa0 = cmul(v0, a), pf0 = cmul(v0, f), pa1 = cmul(v1, a), pf1 = cmul(v1, f);
+            qa0.x += pa0.x;
+            qa0.y += pa0.y;
+            qf0.x += pf0.x;
+            qf0.y += pf0.y;
+            qa1.x += pa1.x;
+            qa1.y += pa1.y;
+            qf1.x += pf1.x;
+            qf1.y += pf1.y;
+        }
+        const float4 u4 = vs[ld4 * j];
+        const double2 qj0 = make_double2((u4.x - qa0.x) / nu, (u4.y - qa0.y) / nu);
+        const double2 qj1 = make_double2((u4.z - qa1.x) / nu, (u4.w - qa1.y) / nu);
+        vs[ld4 * j] = make_float4((float)qj0.x, (float)qj0.y, (float)qj1.x, (float)qj1.y);
+        const double2 fj = fs[j];
+        const double2 qfj0 = cmul(qj0, fj), qfj1 = cmul(qj1, fj);
+        double2 *wp = W + (size_t)nv * s + 2 * p;
+        const double2 w0 = wp[0], w1 = wp[1];
+        const double2 n0 = make_double2((w0.x - qf0.x - qfj0.x) / nu, (w0.y - qf0.y - qfj0.y) / nu);
+        const double2 n1 = make_double2((w1.x - qf1.x - qfj1.x) / nu, (w1.y - qf1.y - qfj1.y) / nu);
+        wp[0] = n0;
+        wp[1] = n1;
+        q2 = (n0.x * n0.x + n0.y * n0.y) + (n1.x * n1.x + n1.y * n1.y);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q2 += __shfl_xor(q2, off, 64);
+    if (threadIdx.x % 64 == 0) sq[threadIdx.x / 64] = q2;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[(size_t)s * gridDim.x + blockIdx.x] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+}
+
 // pend[s][jcount[s] + 1] = ||w|| (the pending column's subdiagonal).
 __global__ void k_dcgs_sub(const int *__restrict__ slots, const int *__restrict__ jcount,
                            const double *__restrict__ nrm, double2 *__restrict__ pend, int ldh) {
@@ -1692,21 +1848,41 @@ static int run_batched_t(pxh_ctx *c, int nv, const std::vector<System> &systems,
         int *jp1_dp = S.jp1_d.p;
         if (dcgs) {
             const int nb = (nv + 255) / 256;
-            const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDot2Group - 1) / kDot2Group);
+            // complex FP32 basis with an even row count: row-pair kernels
+            // (fewer chunks, so the partial buffers sized for nch suffice)
+            constexpr bool f32_basis = std::is_same<VT, float2>::value;
+            const bool pairs = f32_basis && nv % 2 == 0;
+            const int nch_u = pairs ? (nv + kDotPairRows - 1) / kDotPairRows : nch;
+            const int nb_u = pairs ? (nv / 2 + 255) / 256 : nb;
             const dim3 gs((jmax + 1 + 3) / 4, na);
             // one pass of dots over q_0..q_{j-1}, u_j against w and u_j
-            k_dots2_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
-                                                 dc_part_u.p, S.ldh, nch);
-            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch, S.hbuf.p);
-            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, dc_part_u.p, S.ldh, nch, dc_a.p);
+            if constexpr (f32_basis) {
+                if (pairs)
+                    k_dots2_pairs<<<dim3(nch_u, na, (jmax + 1 + kDotPairGroup - 1) / kDotPairGroup), 256, 0, st>>>(
+                        nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p, dc_part_u.p, S.ldh, nch_u);
+            }
+            if (!pairs) {
+                const dim3 gd((nch + 3) / 4, na, (jmax + 1 + kDot2Group - 1) / kDot2Group);
+                k_dots2_part<VT><<<gd, 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, V.p, S.ldv_slot, S.W.p, S.dpart.p,
+                                                     dc_part_u.p, S.ldh, nch);
+            }
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, S.dpart.p, S.ldh, nch_u, S.hbuf.p);
+            k_dots_sum<<<gs, 256, 0, st>>>(S.slots_d.p, jp1_dp, dc_part_u.p, S.ldh, nch_u, dc_a.p);
             k_dcgs_coef<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.hbuf.p, dc_a.p, S.ldh, dc_pend.p, dc_Hu.p,
                                            S.ldH, S.ldH_slot, dc_gcol.p, dc_ghn.p, dc_acoef.p, dc_fcoef.p,
                                            dc_nu.p);
             // one pass of updates: q_j into the basis and the new w
-            k_dcgs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot,
-                                                            dc_acoef.p, dc_fcoef.p, S.ldh, dc_nu.p, S.W.p,
-                                                            S.npart.p);
-            k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.npart.p, nb, nullptr, 1, S.nrm.p);
+            if constexpr (f32_basis) {
+                if (pairs)
+                    k_dcgs_update_pairs<<<dim3(nb_u, na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p,
+                                                                        S.ldv_slot, dc_acoef.p, dc_fcoef.p, S.ldh,
+                                                                        dc_nu.p, S.W.p, S.npart.p);
+            }
+            if (!pairs)
+                k_dcgs_update<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, S.jcount_d.p, V.p, S.ldv_slot,
+                                                                dc_acoef.p, dc_fcoef.p, S.ldh, dc_nu.p, S.W.p,
+                                                                S.npart.p);
+            k_sum_slot<<<na, 64, 0, st>>>(S.slots_d.p, S.npart.p, nb_u, nullptr, 1, S.nrm.p);
             k_dcgs_sub<<<na, 64, 0, st>>>(S.slots_d.p, S.jcount_d.p, S.nrm.p, dc_pend.p, S.ldh);
             k_scale_into<VT><<<grid_rows(na), 256, 0, st>>>(nv, S.slots_d.p, jp1_dp, S.W.p, (size_t)nv, S.nrm.p,
                                                            V.p, S.ldv_slot);
