@@ -773,6 +773,219 @@ int fp64_kv(const Atoms &A, int nsys) {
     return 2;
 }
 
+// ----------------------------------------------- FP64 products on matrix cores
+// omegaFQFmu T sweep with the contraction on v_mfma_f64_16x16x4f64: for
+// target components a and source components b, D_a += A_ab B_b with
+// A_ab[m][k] = T_ab(target m, source k) (16 targets x 4 sources) and
+// B_b[k][c] = x_b(source k) in column c (8 systems x re / im).  Lane l builds
+// the 16 elements of its pair (target l % 16, source 4 s + l / 16) once and
+// feeds all 16 products; it holds D_a[l / 16 + 4 v][l % 16], v = 0..3.  The
+// elements and their FP64 scalars are those of k_tsweep (same far-field
+// radius); the sums run in FP64 in the matrix cores, in a different order.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kF64Targets = 64;  // targets per workgroup (16 per wave)
+constexpr int kF64Src = 32;      // sources per LDS tile
+constexpr int kF64Sys = 8;       // systems per pass (16 real columns)
+
+template <int KERN, bool HET>
+__global__ __launch_bounds__(256, 2) void k_tsweep_f64mfma(int n, int nt, const double4 *__restrict__ pos,
+                                                          const PairConst *__restrict__ pc, int nsys,
+                                                          const int *__restrict__ sys_list,
+                                                          const double2 *__restrict__ x, int ld,
+                                                          double2 *__restrict__ part, int nv, int j_chunk,
+                                                          double rfar2, const int *__restrict__ tperm) {
+    __shared__ double4 spos[kF64Src];
+    __shared__ double sx[4][kF64Src][2 * kF64Sys];  // [component][source][column]
+    __shared__ PairR<double> spc[kMaxTypes * kMaxTypes];
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const int tw = blockIdx.x * kF64Targets + wave * 16;  // first target (Morton position) of the wave
+    const int m = lane % 16, kq = lane / 16;
+    const int i = tw + m < n ? tperm[tw + m] : -1;        // target of this lane's A rows
+    const int split = blockIdx.y;
+    const int g0 = blockIdx.z * kF64Sys;
+    const int nk = min(kF64Sys, nsys - g0);
+    for (int e = tid; e < nt * nt; e += 256) {
+        spc[e].inv_rqq = pc[e].inv_rqq;
+        spc[e].inv_rqmu = pc[e].inv_rqmu;
+        spc[e].inv_rmumu = pc[e].inv_rmumu;
+        spc[e].eta_bar = pc[e].eta_bar;
+    }
+    const double4 pi = i >= 0 ? pos[i] : make_double4(0, 0, 0, 0);
+    const int ti = (int)pi.w;
+    f64x4 acc[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc[a] = f64x4{0, 0, 0, 0};
+    const int j_begin = split * j_chunk;
+    const int j_end = min(n, j_begin + j_chunk);
+    for (int jt = j_begin; jt < j_end; jt += kF64Src) {
+        const int cnt = min(kF64Src, j_end - jt);
+        __syncthreads();
+        if (tid < kF64Src) spos[tid] = tid < cnt ? pos[jt + tid] : make_double4(0, 0, 0, 0);
+        // x of the tile: per system kF64Src charges then 3 kF64Src dipole
+        // components, both contiguous in x; two systems per pass of 256
+        for (int e = tid; e < 4 * kF64Src * kF64Sys; e += 256) {
+            const int v = e / (4 * kF64Src), r = e % (4 * kF64Src);
+            const int src = r < kF64Src ? r : (r - kF64Src) / 3;
+            const int comp = r < kF64Src ? 0 : 1 + (r - kF64Src) % 3;
+            double2 val = make_double2(0, 0);
+            if (v < nk && src < cnt) {
+                const double2 *xs = x + (size_t)sys_list[g0 + v] * ld;
+                val = r < kF64Src ? xs[jt + r] : xs[n + 3 * jt + (r - kF64Src)];
+            }
+            sx[comp][src][2 * v] = val.x;
+            sx[comp][src][2 * v + 1] = val.y;
+        }
+        __syncthreads();
+        for (int s = 0; 4 * s < cnt; ++s) {
+            const int jl = 4 * s + kq;
+            const int j = jt + jl;
+            const double4 pj = spos[jl];
+            const bool valid = i >= 0 && jl < cnt && j != i;
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double r2v = dx * dx + dy * dy + dz * dz;
+            const double r2 = valid ? r2v : 1.0;
+            const double inv_r = d_rsqrt<double>(r2);
+            const double r = r2 * inv_r;
+            const int tj = (int)pj.w;
+            const PairR<double> p = spc[ti + nt * tj];
+            const double inv_r2 = inv_r * inv_r;
+            const double inv_r3 = inv_r2 * inv_r;
+            double tqq, g1, g2, c0, c1;
+            if (KERN == 1 && r2 >= rfar2) {
+                tqq = inv_r;
+                g1 = g2 = c0 = inv_r3;
+                c1 = -3.0 * inv_r3 * inv_r2;
+            } else {
+                if (KERN == 1)
+                    tqq = erf(r * p.inv_rqq) * inv_r;
+                else if (KERN == 2)
+                    tqq = inv_r;
+                else
+                    tqq = p.eta_bar * d_rsqrt<double>(1.0 + p.eta_bar * p.eta_bar * r2);
+                const double a1 = r * p.inv_rqmu;
+                const double e1 = exp(-a1 * a1);
+                g1 = (erf(a1) - kTwoOverSqrtPi * a1 * e1) * inv_r3;
+                g2 = g1;
+                if (HET) {
+                    const double a2 = r * spc[tj + nt * ti].inv_rqmu;
+                    const double e2 = exp(-a2 * a2);
+                    g2 = (erf(a2) - kTwoOverSqrtPi * a2 * e2) * inv_r3;
+                }
+                const double a3 = r * p.inv_rmumu;
+                const double e3 = exp(-a3 * a3);
+                const double fd1 = erf(a3) - kTwoOverSqrtPi * a3 * e3;
+                const double fd2 = kFourOverSqrtPi * p.inv_rmumu * p.inv_rmumu * p.inv_rmumu * e3;
+                c0 = fd1 * inv_r3;
+                c1 = fd2 * inv_r2 - 3.0 * fd1 * inv_r3 * inv_r2;
+            }
+            if (!valid) {
+                tqq = g1 = g2 = c0 = c1 = 0.0;
+                dx = dy = dz = 0.0;
+            }
+            // rows of the 4x4 block: [tqq, g1 d^T; -g2 d, c1 d d^T + c0 I]
+            const double d[3] = {dx, dy, dz};
+            double t[4][4];
+            t[0][0] = tqq;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                t[0][1 + b] = g1 * d[b];
+                t[1 + b][0] = -g2 * d[b];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) t[1 + a][1 + b] = c1 * d[a] * d[b] + (a == b ? c0 : 0.0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double xb = sx[b][jl][m];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(t[a][b], xb, acc[a], 0, 0, 0);
+            }
+        }
+    }
+    // part[split][row][system]: lane l holds targets tw + l / 16 + 4 v, column l % 16
+    double *out = reinterpret_cast<double *>(part + (size_t)split * nv * nsys);
+    const int col = lane % 16, sv = col / 2;
+    if (sv >= nk) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int it = tw + kq + 4 * v;
+        if (it >= n) continue;
+        const int ia = tperm[it];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const size_t row = a == 0 ? (size_t)ia : (size_t)n + 3 * (size_t)ia + (a - 1);
+            out[2 * (row * nsys + g0 + sv) + (col & 1)] = acc[a][v];
+        }
+    }
+}
+
+// The FP64 omegaFQFmu products of five or more systems run on the matrix
+// cores (k_tsweep_f64mfma), fewer on k_tsweep: a pass of the matrix-core
+// kernel costs the same for 1 to 8 systems.  Measured at 15 683 atoms (C2
+// residual products, ~11 systems each): 4.47 against 4.67 ms per launch; the
+// counters show the FP64 MFMA time (64 cycles each) and the VALU time of the
+// element generation adding up, not overlapping.
+constexpr int kF64MfmaMinSys = 5;
+bool use_f64mfma(const pxh_ctx *c, int nsys) {
+    return c->opt.fp64_mfma && c->atoms.model == PXH_WFQFMU && nsys >= kF64MfmaMinSys;
+}
+
+SweepShape f64mfma_shape(const pxh_ctx *c, int n, int nsys) {
+    const int groups = (nsys + kF64Sys - 1) / kF64Sys;
+    const int xblocks = (n + kF64Targets - 1) / kF64Targets;
+    // >= 2048 workgroups, each split spanning at least two LDS tiles
+    int s = c->opt.splits > 0 ? c->opt.splits : std::max(1, 2048 / std::max(1, xblocks * groups));
+    s = std::min(s, std::max(1, (n + 2 * kF64Src - 1) / (2 * kF64Src)));
+    const int j_chunk = ((n + s - 1) / s + kF64Src - 1) / kF64Src * kF64Src;
+    return {j_chunk, (n + j_chunk - 1) / j_chunk};
+}
+
+void apply_atoms_f64mfma(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
+                         double2 *y, int ld, double2 *ext_part, double2 *ext_t) {
+    Atoms &A = c->atoms;
+    const int n = A.n, nv = A.n_var();
+    const int groups = (nsys + kF64Sys - 1) / kF64Sys;
+    const SweepShape sh = f64mfma_shape(c, n, nsys);
+    double2 *part = ext_part, *tt = ext_t;
+    if (!part || !tt) {
+        c->work_part.alloc((size_t)sh.splits * nsys * nv);
+        c->work_t.alloc((size_t)nsys * nv);
+        part = c->work_part.p;
+        tt = c->work_t.p;
+    }
+    const dim3 grid((n + kF64Targets - 1) / kF64Targets, sh.splits, groups);
+    const double rfar2 = far_field_r2(A, true);
+    const int timer = timer_begin(c);
+#define PXH_F64MFMA(KERN, HET)                                                                                      \
+    k_tsweep_f64mfma<KERN, HET><<<grid, 256, 0, c->stream>>>(n, A.nt, A.pos.p, A.pair.p, nsys, d_sys_list, x, ld, \
+                                                             part, nv, sh.j_chunk, rfar2, A.tperm.p)
+    const int kern = A.kernel == PXH_COULOMB ? 2 : A.kernel == PXH_OHNO ? 3 : 1;
+    if (A.hetero) {
+        if (kern == 2)
+            PXH_F64MFMA(2, true);
+        else if (kern == 3)
+            PXH_F64MFMA(3, true);
+        else
+            PXH_F64MFMA(1, true);
+    } else {
+        if (kern == 2)
+            PXH_F64MFMA(2, false);
+        else if (kern == 3)
+            PXH_F64MFMA(3, false);
+        else
+            PXH_F64MFMA(1, false);
+    }
+#undef PXH_F64MFMA
+    PXH_HIP(hipGetLastError());
+    timer_end(c, timer, 4);
+    finish_product(c, nsys, d_sys_list, d_par, x, y, ld, sh.splits, false, part, tt, false);
+    const double pairs = (double)n * (double)(n - 1);
+    c->stats.v[9] += 1;
+    c->stats.v[11] += pairs * ((double)kPairEvalFlopsFmu * groups + (double)kPairVecFlopsFmu * nsys);
+    c->stats.v[6] += (double)n * (double)n * groups;
+}
+
 // ext_part / ext_t: caller-provided room for the split partials and t (both
 // null: the context's product buffers, grown as needed).
 template <typename R, int KV>
@@ -838,6 +1051,10 @@ __global__ void k_stored_t(int nv, const double *__restrict__ T, int nsys, const
 void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysParam *d_par, const double2 *x,
                       double2 *y, int ld, double2 *ext_part, double2 *ext_t) {
     if (nsys <= 0) return;
+    if (use_f64mfma(c, nsys)) {
+        apply_atoms_f64mfma(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
+        return;
+    }
     switch (fp64_kv(c->atoms, nsys)) {
         case 1:
             apply_atoms_impl<double, 1>(c, nsys, d_sys_list, d_par, x, y, ld, ext_part, ext_t);
@@ -855,6 +1072,7 @@ void apply_atoms_fp64(pxh_ctx *c, int nsys, const int *d_sys_list, const SysPara
 
 size_t fp64_product_partials(const pxh_ctx *c, int nsys) {
     const Atoms &A = c->atoms;
+    if (use_f64mfma(c, nsys)) return (size_t)f64mfma_shape(c, A.n, nsys).splits * nsys * A.n_var();
     const int kv = fp64_kv(A, nsys);
     const SweepShape sh = sweep_shape(c, A.n, (nsys + kv - 1) / kv);
     return (size_t)sh.splits * nsys * A.n_var();

@@ -378,3 +378,37 @@ def test_minimal_deck_to_log_results(gpu_ctx, case):
     got = gpu_ctx.solve_spectrum(s.n_var, freq, c["e0"], method=_lib.INVERSION)
     exp = np.array(log["results"])
     assert rel_close(got["results"][:, :12], exp[:, :12], 1e-5, 1e-6).all()
+
+
+@pytest.mark.parametrize("case", ["wfqfmu_ag_inversion", "wfqfmu_hetero_agau_inversion", "sphere20"])
+@pytest.mark.parametrize("nvec", [5, 8, 11])
+def test_fp64_matrix_core_product_matches_vector_kernel(gpu_ctx, case, nvec):
+    """FP64 omegaFQFmu products of >= 5 vectors run on the FP64 matrix cores
+    (k_tsweep_f64mfma); the same sums as the FP64 vector kernel in another
+    order: 1e-13 between the two, 1e-12 against the oracle."""
+    if case == "sphere20":
+        from plasmonx_amd import geometry
+        from plasmonx_amd.model import TOBOHR, AtomSystem
+        from plasmonx_amd.workloads import silver_type
+        coord = geometry.fcc_sphere(4.09, 20.0) * TOBOHR
+        s = AtomSystem(model="wfqfmu", kernel="gaussian", symbols=["Ag"] * len(coord), coord_bohr=coord,
+                       types=[silver_type()], fermi_d=np.array([[12.0]]), fermi_s=np.array([[1.1]]),
+                       r0=np.array([[2.885 * TOBOHR]]))
+        freq = np.array([3.4 / 27.211386245981])
+        gpu_ctx.set_atoms(s)
+        aw = s.alpha_table(freq)
+        gpu_ctx.set_alpha(aw)
+    else:
+        s, freq, aw = setup(gpu_ctx, load_fixture(case), 1)
+    print(f"{case}: {s.n_atoms} atoms, {nvec} vectors", flush=True)
+    rng = np.random.default_rng(nvec)
+    x = rng.normal(size=(nvec, s.n_var)) + 1j * rng.normal(size=(nvec, s.n_var))
+    gpu_ctx.set_option("fp64_mfma", 1)
+    got = gpu_ctx.matvec(freq[0], 0, x)
+    gpu_ctx.set_option("fp64_mfma", 0)
+    vec = gpu_ctx.matvec(freq[0], 0, x)
+    gpu_ctx.set_option("fp64_mfma", 1)
+    assert relerr(got, vec) < 1e-13
+    if s.n_atoms < 1000:
+        ref = pxo.matvec_otf(s, freq[0], aw[:, 0], x[nvec - 1])
+        assert relerr(got[nvec - 1], ref) < 1e-12
