@@ -272,7 +272,7 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * methods start from the caller's `variables`; default 0, x0 = 0 as the
  * reference), "dcgs2" (1: the batched solver's Gram-Schmidt delays the
  * second pass of each vector to the next step), "res_batch" / "res_wait"
- * (FP64 residual products wait for 8 systems, at most 5 steps), "autotune"
+ * (FP64 residual products wait for 16 systems, at most 5 steps), "autotune"
  * (1: the first full low-precision pass of a new atom count times candidate
  * source splits and keeps the fastest for the process), "graphs" (1, the
  * default: the multi-shift Arnoldi step on a BEM target is captured once as

@@ -144,7 +144,7 @@ struct Options {
     int multishift = 1;      // scalar-shift frequency families: one Krylov basis for all frequencies
     int initial_guess = 0;   // iterative methods start from the caller's variables
     int dcgs2 = 1;           // batched solver: Gram-Schmidt with delayed re-orthogonalisation
-    int res_batch = 8;       // batched solver: FP64 residual products wait for this many systems ...
+    int res_batch = 16;      // batched solver: FP64 residual products wait for this many systems ...
     int res_wait = 5;        // ... for at most this many steps
     int autotune = 1;        // time the candidate launch shapes of a new size once per process
     int x0_projection = 8;   // batched solver: start from the best x in the span of this many converged neighbours
