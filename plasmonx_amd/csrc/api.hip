@@ -51,6 +51,7 @@ struct Gate {
     bool writer = false;
 } g_gate;
 thread_local int t_shared = 0;  // nesting depth of this thread's shared holds
+thread_local int t_excl = 0;    // nesting depth of this thread's exclusive holds
 
 void gate_lock_shared() {
     std::unique_lock<std::mutex> lk(g_gate.m);
@@ -65,15 +66,21 @@ void gate_unlock_shared() {
 
 }  // namespace
 
+// Inside this thread's exclusive hold the device is already its own: shared
+// and nested exclusive holds taken there do nothing (the holds nest LIFO, so
+// each destructor sees the state its constructor saw).
 DeviceShared::DeviceShared() {
+    if (t_excl) return;
     if (t_shared++ == 0) gate_lock_shared();
 }
 
 DeviceShared::~DeviceShared() {
+    if (t_excl) return;
     if (--t_shared == 0) gate_unlock_shared();
 }
 
 DeviceExclusive::DeviceExclusive() : held_(t_shared) {
+    if (t_excl++) return;
     if (held_) gate_unlock_shared();
     t_shared = 0;
     {
@@ -87,6 +94,7 @@ DeviceExclusive::DeviceExclusive() : held_(t_shared) {
 }
 
 DeviceExclusive::~DeviceExclusive() {
+    if (--t_excl) return;
     (void)hipDeviceSynchronize();
     {
         std::lock_guard<std::mutex> lk(g_gate.m);

@@ -301,7 +301,8 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
 // the process on the device.  Decompositions that ran beside other
 // contexts' work (LU or GMRES requests of the request loop, other
 // decompositions) gave wrong eigenpairs on this ROCm release; alone they
-// match the LU to 1e-13.
+// match the LU to 1e-13.  Holds taken by a thread inside its own exclusive
+// hold (shared or exclusive) do nothing.
 struct DeviceShared {
     DeviceShared();
     ~DeviceShared();
