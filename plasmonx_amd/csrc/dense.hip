@@ -1061,7 +1061,7 @@ int solve_dense_inversion(pxh_ctx *c, int n_freq, const double *freq, const doub
     // definite)
     if (!bem && rhs_fstride == 0 && spectral_applicable(c) && spectral_pays(c, n, n_freq)) {
         DeviceExclusive alone;  // no other work of the process on the device meanwhile
-        const bool ok = spectral_decompose(c, d_rhs_all, pol_mask);
+        const bool ok = spectral_decompose(c, d_rhs_all, pol_mask, true);
         if (ok) {
             spectral_charges(c, n_freq, freq, d_sol, (size_t)rhs_stride);
             c->spectral.release();
@@ -1312,7 +1312,7 @@ int solve_group(pxh_ctx *const *ctxs, int n_ctx, int n_freq, const double *freq,
     if (!spec.empty()) alone.reset(new DeviceExclusive);
     for (int t : spec) {
         pxh_ctx *c = ctxs[t];
-        if (spectral_decompose(c, rhs[t].p, pol_mask)) {
+        if (spectral_decompose(c, rhs[t].p, pol_mask, false)) {  // polarizabilities only
             DevBuf<double2> pol;
             pol.alloc((size_t)9 * n_freq);
             spectral_polar(c, n_freq, freq, pol.p);

@@ -158,6 +158,7 @@ struct Options {
 // right-hand sides and coordinates in that basis.
 struct SpectralWs {
     int n = 0, mask = 0;   // n = 0: no decomposition held
+    bool have_q = false;   // Q holds the eigenvectors (else the tridiagonal reduction's reflectors)
     DevBuf<double> L;      // n x n Cholesky factor of T (lower triangle)
     DevBuf<double> Q;      // n x n orthonormal eigenvectors of L^T K L
     DevBuf<double> lam;    // n eigenvalues
@@ -170,6 +171,7 @@ struct SpectralWs {
         c6.release();
         w3.release();
         n = 0;
+        have_q = false;
     }
 };
 
@@ -322,7 +324,7 @@ struct DeviceExclusive {
 // spectral.hip: the homogeneous omegaFQ inversion by one eigendecomposition
 bool spectral_applicable(const pxh_ctx *c);
 bool spectral_pays(const pxh_ctx *c, int n, int n_freq);
-bool spectral_decompose(pxh_ctx *c, const double2 *d_rhs /* n x 3 */, int pol_mask);
+bool spectral_decompose(pxh_ctx *c, const double2 *d_rhs /* n x 3 */, int pol_mask, bool need_q);
 void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *d_sol, size_t stride);
 void spectral_polar(pxh_ctx *c, int n_freq, const double *freq, double2 *d_out /* 9 per frequency */);
 // solver.hip: the 20 result columns of one frequency from its 3 x 3 complex polarizability

@@ -209,18 +209,23 @@ bool spectral_pays(const pxh_ctx *c, int n, int n_freq) {
 
 // The decomposition for the right-hand sides rhs (n x 3, the selected
 // directions of mask); on success the context's spectral workspace holds
-// lambda (n), Q (n x n), L (n x n, lower), W = Q^T L^-1 r (n x 3) and
-// C = Q^T L^T b (n x 6: Re, Im of each direction).  Returns false when T is
-// not positive definite or the eigensolver does not converge (the caller
-// then uses the LU).
+// lambda (n), L (n x n, lower), W = Q^T L^-1 r (n x 3),
+// C = Q^T L^T b (n x 6: Re, Im of each direction) and, with need_q, Q
+// (n x n; DSYEVD).  Without need_q (polarizabilities only) Q is never
+// formed: S = H T H^T by DSYTRD, T = Z Lambda Z^T by DSTEDC, and
+// Q^T v = Z^T (H^T v) for the 9 vectors (DORMTR on n x 9 instead of the
+// back-transformation of all n eigenvectors).  Returns false when T is not
+// positive definite or the eigensolver does not converge (the caller then
+// uses the LU).
 // The caller holds the device gate exclusively (DeviceExclusive) from here
 // to the last use of the decomposition.
-bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
+bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask, bool need_q) {
     Atoms &At = c->atoms;
     const int n = At.n;
     hipStream_t st = c->stream;
     SpectralWs &S = c->spectral;
     S.n = 0;
+    S.have_q = false;
     DevBuf<double> P, B6, R3, E;
     DevBuf<int> info;
     S.L.alloc((size_t)n * n);
@@ -257,8 +262,19 @@ bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
     check_blas(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
                              rocblas_diagonal_non_unit, n, n, &one, S.L.p, n, P.p, n, S.Q.p, n),
                "rocblas_dtrmm failed");
-    check_blas(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, S.Q.p, n, S.lam.p, E.p, info.p + 1),
-               "rocsolver_dsyevd failed");
+    DevBuf<double> tau;
+    if (need_q) {
+        check_blas(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, S.Q.p, n, S.lam.p, E.p,
+                                    info.p + 1),
+                   "rocsolver_dsyevd failed");
+    } else {
+        // reflectors of H into S.Q, the tridiagonal's eigenvectors Z into P
+        tau.alloc(n);
+        check_blas(rocsolver_dsytrd(h, rocblas_fill_lower, n, S.Q.p, n, S.lam.p, E.p, tau.p),
+                   "rocsolver_dsytrd failed");
+        check_blas(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, S.lam.p, E.p, P.p, n, info.p + 1),
+                   "rocsolver_dstedc failed");
+    }
     PXH_HIP(hipMemcpyAsync(hinfo + 1, info.p + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     PXH_HIP(hipStreamSynchronize(st));
     if (hinfo[1] != 0) {  // no convergence
@@ -267,22 +283,34 @@ bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
         S.release();
         return false;
     }
-    // C = Q^T (L^T b), W = Q^T (L^-1 r)
+    // C = Q^T (L^T b), W = Q^T (L^-1 r); without Q: Z^T (H^T v)
     tri_left(h, false, true, n, 6, S.L.p, B6.p);
-    dgemm(h, true, n, 6, n, S.Q.p, n, B6.p, n, S.c6.p, n);
     tri_left(h, true, false, n, 3, S.L.p, R3.p);
-    dgemm(h, true, n, 3, n, S.Q.p, n, R3.p, n, S.w3.p, n);
+    if (need_q) {
+        dgemm(h, true, n, 6, n, S.Q.p, n, B6.p, n, S.c6.p, n);
+        dgemm(h, true, n, 3, n, S.Q.p, n, R3.p, n, S.w3.p, n);
+    } else {
+        check_blas(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose, n, 6,
+                                    S.Q.p, n, tau.p, B6.p, n),
+                   "rocsolver_dormtr failed");
+        check_blas(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose, n, 3,
+                                    S.Q.p, n, tau.p, R3.p, n),
+                   "rocsolver_dormtr failed");
+        dgemm(h, true, n, 6, n, P.p, n, B6.p, n, S.c6.p, n);
+        dgemm(h, true, n, 3, n, P.p, n, R3.p, n, S.w3.p, n);
+    }
     timer_end(c, tm, 25);
     PXH_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
     c->stats.v[24] += 1;
     // nominal flops: Cholesky n^3/3, S n^3, tridiagonal reduction 4/3 n^3,
-    // back-transformation of the eigenvectors 2 n^3
-    c->stats.v[26] += (1.0 / 3.0 + 1.0 + 4.0 / 3.0 + 2.0) * (double)n * n * n;
+    // back-transformation of the eigenvectors 2 n^3 (with Q only)
+    c->stats.v[26] += (1.0 / 3.0 + 1.0 + 4.0 / 3.0 + (need_q ? 2.0 : 0.0)) * (double)n * n * n;
     // algorithmic HBM bytes of the tridiagonal reduction: the lower triangle of
     // the trailing matrix read once per column, sum 8 (n - k)^2 / 2 = 4 n^3 / 3
     c->stats.v[27] += 4.0 / 3.0 * (double)n * n * n;
     S.n = n;
     S.mask = mask;
+    S.have_q = need_q;
     return true;
 }
 
@@ -291,6 +319,7 @@ bool spectral_decompose(pxh_ctx *c, const double2 *rhs, int mask) {
 // x = L^-T (Q Y).
 void spectral_charges(pxh_ctx *c, int n_freq, const double *freq, double2 *sol, size_t stride) {
     SpectralWs &S = c->spectral;
+    PXH_CHECK(S.have_q, PXH_EINVAL, "spectral charges need the eigenvectors (decomposition with need_q)");
     const int n = S.n;
     hipStream_t st = c->stream;
     rocblas_handle h = blas_handle(c);
