@@ -281,7 +281,9 @@ int pxh_last_error(pxh_ctx *ctx, char *buf, int len);
  * "mfma_levels" (1 or 2, default 2: FP32 accumulation levels of the MFMA
  * pair kernel), "fp64_mfma" (1, the default: omegaFQFmu FP64 products of
  * five or more systems contract on the FP64 matrix cores; 0 = the FP64
- * vector kernel), "spectral" (homogeneous omegaFQ inversion, one atom type:
+ * vector kernel), "bem_gemv" (1, the default: BEM FP64 products of up to 4
+ * systems, the multi-shift spectra's, by a streaming matrix-vector pass; 0 =
+ * the matrix-core tiles), "spectral" (homogeneous omegaFQ inversion, one atom type:
  * 1, the default, solves every frequency from one eigendecomposition of
  * L^T K L, T = L L^T, when that is estimated faster than one LU per
  * frequency; 2 whenever T is positive definite; 0 never; an indefinite T

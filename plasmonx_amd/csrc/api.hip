@@ -626,6 +626,8 @@ int pxh_set_option(pxh_ctx *c, const char *key, double value) {
             c->opt.matvec_mixed = value != 0.0;
         } else if (k == "mfma") {
             c->opt.mfma = value != 0.0;
+        } else if (k == "bem_gemv") {
+            c->opt.bem_gemv = value != 0.0;
         } else if (k == "fp64_mfma") {
             c->opt.fp64_mfma = value != 0.0;
         } else if (k == "mfma_r4") {

@@ -383,6 +383,91 @@ __global__ __launch_bounds__(256) void k_bem_gemm(int n, int cpad, int steps_per
     }
 }
 
+// Y = M X for up to kGemvSys systems (the multi-shift spectra: one vector per
+// field direction): a streaming matrix-vector pass on the FP64 VALU, as the
+// few columns leave the matrix cores nothing to gain and their padded
+// 16-system tiles cost MFMA time that does not overlap the loads.  One
+// work-item per row (pair), the columns split over blockIdx.y (their slice of X
+// staged in LDS, read as broadcasts), row pairs by 16-byte loads when n is
+// even; partials in k_gemm_reduce's layout
+// (one column tile), which adds them in order and applies the epilogue.
+constexpr int kGemvSys = 4;
+constexpr int kGemvMaxCols = 256;  // columns per workgroup (LDS slice of X)
+
+template <int RPT>
+__global__ __launch_bounds__(256) void k_bem_gemv(int n, int cols_per, const double *__restrict__ Mat,
+                                                  const double2 *__restrict__ x, int ldx, int x_by_k,
+                                                  const int *__restrict__ slots, int nsys, int rowpad,
+                                                  double *__restrict__ part) {
+    __shared__ double2 xs[kGemvSys][kGemvMaxCols];
+    const int r = RPT * (blockIdx.x * 256 + threadIdx.x);  // first of the work-item's RPT rows
+    const int ks = blockIdx.y;
+    const int c0 = ks * cols_per, c1 = min(n, c0 + cols_per);
+    for (int e = threadIdx.x; e < kGemvSys * cols_per; e += 256) {
+        const int k = e / cols_per, cc = e % cols_per;
+        const int cidx = c0 + cc;
+        xs[k][cc] = (k < nsys && cidx < n) ? x[(size_t)(x_by_k ? k : slots[k]) * ldx + cidx] : make_double2(0, 0);
+    }
+    __syncthreads();
+    if (r >= n) return;
+    double re[RPT][kGemvSys], im[RPT][kGemvSys];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+#pragma unroll
+        for (int k = 0; k < kGemvSys; ++k) re[q][k] = im[q][k] = 0.0;
+    const double *m = Mat + r;
+    // one column's RPT rows: a 16-byte load for row pairs (n even, r even)
+    auto ld = [&](int cl, double (&a)[RPT]) {
+        if constexpr (RPT == 2) {
+            const double2 v = *reinterpret_cast<const double2 *>(m + (size_t)n * cl);
+            a[0] = v.x;
+            a[1] = v.y;
+        } else {
+            a[0] = m[(size_t)n * cl];
+        }
+    };
+    int col = c0;
+    for (; col + 8 <= c1; col += 8) {
+        double a[8][RPT];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ld(col + u, a[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < kGemvSys; ++k) {
+                const double2 xv = xs[k][col - c0 + u];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    re[q][k] = fma(a[u][q], xv.x, re[q][k]);
+                    im[q][k] = fma(a[u][q], xv.y, im[q][k]);
+                }
+            }
+    }
+    for (; col < c1; ++col) {
+        double a[RPT];
+        ld(col, a);
+#pragma unroll
+        for (int k = 0; k < kGemvSys; ++k) {
+            const double2 xv = xs[k][col - c0];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                re[q][k] = fma(a[q], xv.x, re[q][k]);
+                im[q][k] = fma(a[q], xv.y, im[q][k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        double *pr = part + ((size_t)ks * rowpad + r + q) * 32;
+#pragma unroll
+        for (int k = 0; k < kGemvSys; ++k) {
+            if (k >= nsys) break;
+            pr[2 * k] = re[q][k];
+            pr[2 * k + 1] = im[q][k];
+        }
+    }
+}
+
 // Sum of the ksplit partials of each (row, real column) in order, then the
 // epilogue of k_bem_gemm.  One work-item per (column tile, row, real column).
 __global__ void k_gemm_reduce(int n, int rowpad, int coltiles, int ksplit, const double *__restrict__ part,
@@ -534,7 +619,41 @@ void apply_bem(pxh_ctx *c, int nsys, const int *d_sys_list, const double2 *d_lam
     const unsigned pack_blocks = (unsigned)((pack_items + 255) / 256);
     const unsigned gemm_blocks = 8u * (unsigned)((rowtiles * coltiles * ksplit + 7) / 8);
     const unsigned red_blocks = (unsigned)(((size_t)coltiles * n * 32 + 255) / 256);
+    // few systems in FP64: the streaming pass (k_bem_gemv), the columns split
+    // so that the launch has >= ~640 workgroups (at least 16 columns each)
+    // and a slice fits in LDS
+    const bool gemv = !fp32 && nsys <= kGemvSys && c->opt.bem_gemv;
+    // rows per work-item: row pairs (16-byte loads, n even) pay where the
+    // matrix streams from HBM; one row where it stays in the last-level cache
+    // (measured: n = 10240 201 against 213 us per product, n = 5120 73 against
+    // 53-56 us; the MFMA tiles 204 and 60-62 us)
+    const int vrpt = (n % 2 == 0 && n >= 8192) ? 2 : 1;
+    const int vrows = (n + 256 * vrpt - 1) / (256 * vrpt);
+    const int vsplit =
+        std::max((n + kGemvMaxCols - 1) / kGemvMaxCols, std::min((640 + vrows - 1) / vrows, std::max(1, n / 16)));
+    const int vcols = (n + vsplit - 1) / vsplit;
+    const int vsplit_eff = (n + vcols - 1) / vcols;
+    const int vrowpad = vrows * 256 * vrpt;
+    if (gemv) c->work_gemm.alloc((size_t)vsplit_eff * vrowpad * 32);
     auto gemm = [&](const double *Md, const float *Mf, const double2 *xin, int x_by_k, const GemmOut &o) {
+        if (gemv) {
+            const int t_gemv = timer_begin(c);
+            if (vrpt == 2)
+                k_bem_gemv<2><<<dim3(vrows, vsplit_eff), 256, 0, st>>>(n, vcols, Md, xin, x_by_k ? n : ld, x_by_k,
+                                                                       d_sys_list, nsys, vrowpad, c->work_gemm.p);
+            else
+                k_bem_gemv<1><<<dim3(vrows, vsplit_eff), 256, 0, st>>>(n, vcols, Md, xin, x_by_k ? n : ld, x_by_k,
+                                                                       d_sys_list, nsys, vrowpad, c->work_gemm.p);
+            k_gemm_reduce<<<(unsigned)(((size_t)n * 32 + 255) / 256), 256, 0, st>>>(
+                n, vrowpad, 1, vsplit_eff, c->work_gemm.p, d_sys_list, nsys, o);
+            timer_end(c, t_gemv, 16);
+            if (!c->capturing) {
+                c->stats.v[17] += 1;
+                c->stats.v[18] += 4.0 * n * (double)n * nsys;
+                c->stats.v[22] += (double)n * n * 8.0 + 32.0 * n * (double)nsys;
+            }
+            return;
+        }
         k_gemm_pack<<<pack_blocks, 256, 0, st>>>(n, cpad, nsys, d_sys_list, xin, x_by_k ? n : ld, x_by_k, coltiles,
                                                  c->work_pack.p);
         double *part = ksplit > 1 ? c->work_gemm.p : nullptr;

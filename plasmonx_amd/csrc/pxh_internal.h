@@ -136,6 +136,7 @@ struct Options {
     int precond = 1;         // diagonal right preconditioning in the batched path
     int mfma = 1;            // omegaFQFmu FP32-class products on MFMA (fp16 hi/lo split)
     int fp64_mfma = 1;       // omegaFQFmu FP64 products of >= 5 systems on the FP64 matrix cores
+    int bem_gemv = 1;        // BEM FP64 products of <= 4 systems by the streaming pass (else the MFMA tiles)
     int mfma_levels = 2;     // FP32 accumulation levels of the MFMA pair kernel (1 or 2)
     int mfma_flush = 8;      // source steps (16 atoms each) summed in the MFMA registers between flushes
     int mfma_r4 = 1;         // one-column-block MFMA passes: four row blocks per wave (else two)

@@ -412,3 +412,43 @@ def test_fp64_matrix_core_product_matches_vector_kernel(gpu_ctx, case, nvec):
     if s.n_atoms < 1000:
         ref = pxo.matvec_otf(s, freq[0], aw[:, 0], x[nvec - 1])
         assert relerr(got[nvec - 1], ref) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["bem_dpcm_accurate_inversion", "bem_ief_accurate_inversion"])
+@pytest.mark.parametrize("nvec", [1, 3, 4])
+def test_bem_streaming_product_matches_mfma_tiles(gpu_ctx, case, nvec):
+    """BEM FP64 products of <= 4 systems take the streaming pass
+    (k_bem_gemv, the multi-shift spectra's product); the MFMA tiles form the
+    same sums in another order: 1e-13 between the two (the oracle's D and
+    S A^-1 are checked by test_bem_matvec_matches_oracle)."""
+    fx = load_fixture(case)
+    m = mesh_from_fixture(fx)
+    freq = np.array([3.2, 3.6]) / 27.211386245981
+    gpu_ctx.set_mesh(m)
+    gpu_ctx.set_permittivity(pm.silver_etchegoin(freq))
+    rng = np.random.default_rng(nvec)
+    x = rng.normal(size=(nvec, m.n)) + 1j * rng.normal(size=(nvec, m.n))
+    gpu_ctx.set_option("bem_gemv", 1)
+    got = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("bem_gemv", 0)
+    ref = gpu_ctx.matvec(freq[1], 1, x)
+    gpu_ctx.set_option("bem_gemv", 1)
+    assert relerr(got, ref) < 1e-13
+
+
+def test_bem_streaming_product_row_pairs_on_the_rod():
+    """10240 tesserae (the C3 rod): the streaming pass with row pairs
+    (16-byte loads) against the MFMA tiles, 1e-13."""
+    from plasmonx_amd import Context
+    from plasmonx_amd.workloads import c3_meshes, c3_permittivity
+    rod, freq = c3_meshes("dpcm")[1], c3_meshes("dpcm")[2][:2]
+    ctx = Context(0)
+    ctx.set_mesh(rod)
+    ctx.set_permittivity(c3_permittivity(freq))
+    print(f"rod: {rod.n} tesserae", flush=True)
+    x = np.random.default_rng(5).normal(size=(3, rod.n)) + 1j * np.random.default_rng(6).normal(size=(3, rod.n))
+    got = ctx.matvec(freq[1], 1, x)
+    ctx.set_option("bem_gemv", 0)
+    ref = ctx.matvec(freq[1], 1, x)
+    ctx.close()
+    assert relerr(got, ref) < 1e-13
