@@ -376,8 +376,8 @@ def run_c1(a):
 def leg_c3(a, which=(("sphere_dpcm", "dpcm", 0),)):
     """C3: BEM on the icosphere (5120) and the capped rod, Brendel-Bormann
     silver, 400 w x 3 fields, GMRES on the stored D: one multi-shift Krylov
-    basis per field for all 400 w (FP64 MFMA products, FP64 true residual per
-    system).  The roofline is that of the matrix products; the CPU baseline
+    basis per field for all 400 w (FP64 products of the 3 field vectors by a
+    streaming pass over D, FP64 true residual per system).  The roofline is that of the matrix products; the CPU baseline
     the oracle's GMRES on its stored D for a sample of frequencies."""
     from plasmonx_amd import Context, _lib
     from plasmonx_amd.workloads import c3_meshes, c3_permittivity
@@ -451,10 +451,11 @@ def leg_c3(a, which=(("sphere_dpcm", "dpcm", 0),)):
         achieved = gm_fl / t_s / 1e12 if t_s > 0 else 0.0
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4)}
-    roof.update({"traffic": None, "kernel": "k_bem_gemm<double> (+ split-K reduction): the stored D applied to the "
-                                           "3 field directions' Arnoldi vectors (4 flop per element and system, "
-                                           "D read once per launch); HIP events per launch, in one spectrum "
-                                           "run with direct launches after the timed (graph-replayed) run",
+    roof.update({"traffic": None, "kernel": "k_bem_gemv (streaming FP64 pass, + ordered split reduction): the "
+                                           "stored D applied to the 3 field directions' Arnoldi vectors (4 flop "
+                                           "per element and system, D read once per launch); HIP events per "
+                                           "launch, in one spectrum run with direct launches after the timed "
+                                           "(graph-replayed) run",
                  "launches": gm_n, "avg_launch_ms": round(gm_ms / max(gm_n, 1), 4),
                  "flop_per_byte": round(ai, 3)})
     return {"metric": "freq-point solves/s (C3 BEM, 400 w x 3 fields)", "value": round(total_pts / total_t, 3),
@@ -575,7 +576,7 @@ def leg_c4(a, n_flakes=96, flake_max=12000, n_freq=128, freq_blocks=1, steps=1):
                            "seconds": round(upd_ms * 1e-3 / max(steps, 1), 2), "launches": int(upd_n)},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                         "kernel": "the spectral decompositions (rocSOLVER DPOTRF + DSYEVD, rocBLAS TRMM, the K-bar "
+                         "kernel": "the spectral decompositions (rocSOLVER DPOTRF, DSYTRD, DSTEDC, DORMTR on 9 vectors, rocBLAS TRMM, the K-bar "
                                    "kernel): algorithmic bytes of the tridiagonal reduction (the lower triangle of "
                                    "the trailing matrix read once per column, 4 n^3 / 3 B per flake) over the "
                                    "decomposition time (HIP events)"},
